@@ -1,0 +1,137 @@
+/*
+ * nfft2d.h -- C ABI of the MI355X (gfx950) 2D NFFT library (libnfft2d.so).
+ *
+ * Drop-in replacement for the reference's 2D NFFT path
+ * (OrangeOwlSolutions/Miscellaneous, paths relative to its root):
+ *   NFFT/NFFT2_2D.cuh:4-11   NFFT2_2D_GPU (float2 / double2 overloads), the
+ *                            global cuFFT plan and its create/destroy calls;
+ *   NFFT/NDFT2_2D.cuh:4-6    NDFT2_2D_GPU;
+ *   BP_FBP_FFBP/NFFT1_2D.m   the type-2 direction, which the reference has
+ *                            only in MATLAB, exported here as NFFT1_2D_GPU.
+ *
+ * Conventions (the reference's, kept unchanged):
+ *   - every pointer argument is DEVICE memory owned by the caller;
+ *   - complex values are interleaved (re, im): nfft2d_c64 has the layout of
+ *     float2 / cufftComplex, nfft2d_c128 that of double2 / double2_;
+ *   - coordinates are in grid-index units: x has period N1, y has period N2
+ *     (nominal range [-N/2, N/2)); any finite value is wrapped;
+ *   - mode arrays are N1 x N2 row-major, mode (k1, k2), k in [-N/2, N/2), at
+ *     index (k1 + N1/2) * N2 + (k2 + N2/2)  (NFFT/NFFT2_2D.cu:83);
+ *   - transforms use e^{-2 pi i (k1 x / N1 + k2 y / N2)}, unnormalised;
+ *   - c = 2 and K = 6 (NFFT/NFFT2_2D.cu:20-21);
+ *   - result buffers are overwritten, never accumulated into.
+ * Differences, all in the caller's favour:
+ *   - every entry point returns a status (0 = NFFT2D_OK, < 0 = error; text in
+ *     nfft2d_last_error()) instead of aborting the process;
+ *   - no plan needs to exist before a drop-in call (plans are cached per
+ *     size, precision and device), and the calls allocate nothing per call;
+ *   - work is stream-ordered on the legacy default stream for the drop-ins,
+ *     or on the caller's stream for the plan API; the host is not blocked.
+ * Thread safety: the plan cache is internally locked.  Two calls that share
+ * one plan on different streams race on the plan's scratch; calls on one
+ * stream are ordered.
+ */
+#ifndef NFFT2D_H
+#define NFFT2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y; } nfft2d_c64;
+typedef struct { double x, y; } nfft2d_c128;
+typedef struct nfft2d_plan_s nfft2d_plan;
+
+enum {
+    NFFT2D_OK = 0,
+    NFFT2D_ERR_ARG = -1,          /* null pointer, negative count, bad flag   */
+    NFFT2D_ERR_SIZE = -2,         /* N odd / non-positive / above the maximum */
+    NFFT2D_ERR_HIP = -3,          /* a HIP runtime call failed                */
+    NFFT2D_ERR_ALLOC = -4,        /* device allocation failed                 */
+    NFFT2D_ERR_STATE = -5,        /* plan used before nfft2d_setpts           */
+    NFFT2D_ERR_UNSUPPORTED = -6   /* K != 6 or c != 2                         */
+};
+
+enum { NFFT2D_C64 = 0, NFFT2D_C128 = 1 };
+
+/* Last error of the calling thread, "" if none. */
+const char *nfft2d_last_error(void);
+/* Library version, major * 10000 + minor * 100 + patch. */
+int nfft2d_version(void);
+
+/* ------------------------------------------------------------------ drop-ins */
+
+/* Type-1 (non-equispaced data -> equispaced result), the reference's
+ * NFFT2_2D_GPU (NFFT/NFFT2_2D.cu:329 float, :413 double; decl.
+ * NFFT/NFFT2_2D.cuh:8-11):
+ *   result[k1][k2] = sum_{l<M} data[l] e^{-2 pi i (k1 x[l] / N1 + k2 y[l] / N2)}.
+ * result: N1*N2, data: M, x, y: M.  N1, N2 even. */
+int NFFT2_2D_GPU_c64(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y,
+                     int N1, int N2, int M);
+int NFFT2_2D_GPU_c128(nfft2d_c128 *result, const nfft2d_c128 *data, const double *x, const double *y,
+                      int N1, int N2, int M);
+
+/* Type-2 (equispaced data -> non-equispaced result), the reference's MATLAB
+ * NFFT1_2D (BP_FBP_FFBP/NFFT1_2D.m:1, called as NFFT1_2D(data, x1, x2)):
+ *   result[l] = sum_{k1,k2} data[k1][k2] e^{-2 pi i (k1 x[l] / N1 + k2 y[l] / N2)}.
+ * result: M, data: N1*N2 row-major [k1][k2] (MATLAB's data(k1,k2)). */
+int NFFT1_2D_GPU_c64(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y,
+                     int N1, int N2, int M);
+int NFFT1_2D_GPU_c128(nfft2d_c128 *result, const nfft2d_c128 *data, const double *x, const double *y,
+                      int N1, int N2, int M);
+
+/* Plan warm-up hooks, NFFT/NFFT2_2D.cu:49-50,55 (decl. NFFT/NFFT2_2D.cuh:5-7).
+ * The reference builds one process-global cuFFT plan of size (2 N1) x (2 N2)
+ * that the next NFFT2_2D_GPU call silently assumes; here they only pre-build
+ * (or drop) the cached plans, and are optional. */
+void Calculate_cuFFT_plan_C2C_NFFT2_2D(int N1, int N2);
+void Calculate_cuFFT_plan_Z2Z_NFFT2_2D(int N1, int N2);
+void Destroy_cuFFT_plan_NFFT2_2D(void);
+
+/* Direct NED sum, NFFT/NDFT2_2D.cu:50 (decl. NFFT/NDFT2_2D.cuh:4-6):
+ *   out[n] = sum_{m<M} in[m] e^{-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)},  n < N,
+ * in FP64.  `handle` took a cublasHandle_t; it is accepted and ignored.  The
+ * reference materialises the M x N kernel matrix and writes it to a text file
+ * (:57,69); neither happens here. */
+int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, const double *u, const double *v,
+                 const nfft2d_c128 *in, nfft2d_c128 *out, int Nu, int Nv, int M, int N);
+
+/* ------------------------------------------------------------------ plan API */
+
+/* A plan holds the twiddle and deconvolution tables, the oversampled grid and
+ * the tile-sorted points.  dtype: NFFT2D_C64 or NFFT2D_C128.  K must be 6 and
+ * c must be 2 (the reference's constants); flags must be 0. */
+int nfft2d_plan_create(nfft2d_plan **plan, int N1, int N2, int dtype, int K, int c, unsigned flags);
+int nfft2d_plan_destroy(nfft2d_plan *plan);
+
+/* Tile-sort M points (x, y: float for C64 plans, double for C128 plans).  The
+ * sorted copy lives in the plan; x and y may be freed afterwards. */
+int nfft2d_setpts(nfft2d_plan *plan, const void *x, const void *y, int M, void *stream);
+
+/* Type-2: result[l] = sum_k data[k] e^{-2 pi i k.x_l / N}  (result: M, data: N1*N2). */
+int nfft2d_type2(nfft2d_plan *plan, void *result, const void *data, void *stream);
+/* Type-1: result[k] = sum_l data[l] e^{-2 pi i k.x_l / N}  (result: N1*N2, data: M). */
+int nfft2d_type1(nfft2d_plan *plan, void *result, const void *data, void *stream);
+/* Adjoint of type-2: result[k] = sum_l data[l] e^{+2 pi i k.x_l / N}. */
+int nfft2d_adjoint(nfft2d_plan *plan, void *result, const void *data, void *stream);
+/* Type-2 with the coordinate derivatives: y[l], dy1[l] = d y_l / d x_l,
+ * dy2[l] = d y_l / d y_l-coordinate (all M complex). */
+int nfft2d_type2_dx(nfft2d_plan *plan, void *y, void *dy1, void *dy2, const void *data, void *stream);
+/* Gradient of a real loss with respect to the coordinates, given
+ * g = dL/d(conj y) * 2 (PyTorch's convention for complex outputs):
+ *   gx1[l] = Re(conj(g[l]) dy_l/dx_l),  gx2[l] likewise (M reals each).
+ * Recomputes the oversampled grid from `data` (nothing is stored between the
+ * forward and the backward). */
+int nfft2d_type2_grad_x(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, const void *data,
+                        void *stream);
+
+/* Points per subproblem of the spread / gather (default 1024). */
+int nfft2d_plan_set_maxpts(nfft2d_plan *plan, int maxpts);
+/* Number of points set on the plan, or < 0 on error. */
+int nfft2d_plan_num_points(const nfft2d_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NFFT2D_H */

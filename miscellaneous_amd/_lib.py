@@ -1,0 +1,95 @@
+"""ctypes binding of libnfft2d.so (the C ABI in include/nfft2d.h).
+
+The library is built in-tree by ``miscellaneous_amd.build.build_library()``
+(hipcc, gfx950).  Loading fails loudly when it is missing: there is no CPU or
+PyTorch fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libnfft2d.so")
+HEADER = os.path.join(ROOT, "include", "nfft2d.h")
+
+NFFT2D_OK = 0
+NFFT2D_C64 = 0
+NFFT2D_C128 = 1
+
+_ERR_NAMES = {-1: "NFFT2D_ERR_ARG", -2: "NFFT2D_ERR_SIZE", -3: "NFFT2D_ERR_HIP", -4: "NFFT2D_ERR_ALLOC",
+              -5: "NFFT2D_ERR_STATE", -6: "NFFT2D_ERR_UNSUPPORTED"}
+
+
+class NFFT2DError(RuntimeError):
+    """A libnfft2d entry point returned a non-zero status."""
+
+    def __init__(self, code: int, where: str, msg: str):
+        self.code = code
+        super().__init__(f"{where}: {_ERR_NAMES.get(code, code)}: {msg}")
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_SIGS = {
+    "nfft2d_last_error": (ctypes.c_char_p, []),
+    "nfft2d_version": (_i, []),
+    "NFFT2_2D_GPU_c64": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "NFFT2_2D_GPU_c128": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "NFFT1_2D_GPU_c64": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "NFFT1_2D_GPU_c128": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "Calculate_cuFFT_plan_C2C_NFFT2_2D": (None, [_i, _i]),
+    "Calculate_cuFFT_plan_Z2Z_NFFT2_2D": (None, [_i, _i]),
+    "Destroy_cuFFT_plan_NFFT2_2D": (None, []),
+    "NDFT2_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i]),
+    "nfft2d_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _i, ctypes.c_uint]),
+    "nfft2d_plan_destroy": (_i, [_vp]),
+    "nfft2d_setpts": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nfft2d_type2": (_i, [_vp, _vp, _vp, _vp]),
+    "nfft2d_type1": (_i, [_vp, _vp, _vp, _vp]),
+    "nfft2d_adjoint": (_i, [_vp, _vp, _vp, _vp]),
+    "nfft2d_type2_dx": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "nfft2d_type2_grad_x": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "nfft2d_plan_set_maxpts": (_i, [_vp, _i]),
+    "nfft2d_plan_num_points": (_i, [_vp]),
+}
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/nfft2d.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return [n for n in names if n not in ("if", "while", "defined")]
+
+
+def load_library(path: str | None = None):
+    """Load libnfft2d.so and attach the ctypes signatures.  Raises if missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"libnfft2d.so is not built ({p}); run miscellaneous_amd.build.build_library() "
+                          "or __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load_library().nfft2d_last_error().decode()
+
+
+def check(rc: int, where: str) -> None:
+    if rc != NFFT2D_OK:
+        raise NFFT2DError(rc, where, last_error())

@@ -1,0 +1,56 @@
+"""Differentiable type-2 NFFT: a thin torch.autograd.Function over the C ABI.
+
+forward:   y = A(x) f,  y_l = sum_k f_k e^{-2 pi i (k1 x1_l/N1 + k2 x2_l/N2)}
+backward:  grad_f = A^H g                       (nfft2d_adjoint, the adjoint NFFT)
+           grad_x = Re(conj(g) dy/dx)            (nfft2d_type2_grad_x, the
+                                                  derivative-window gather)
+with g the incoming gradient in PyTorch's convention for complex outputs.
+The backward recomputes the oversampled grid from f instead of storing it
+between forward and backward.
+
+The reference never differentiates its NFFT (its only optimisers are
+derivative-free: fminsearch and PSO, e.g. Synthesis/Aper_Synth_1D/Aper_Synth_1D.m:161,
+Synthesis/PSO/PSO.cu:372); this is the new form the GPU path adds.
+"""
+from __future__ import annotations
+
+import torch
+
+from .nfft import Plan
+
+
+class NFFT2DType2(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor, plan: Plan):
+        f = f.contiguous()
+        x1c, x2c = x1.detach().contiguous(), x2.detach().contiguous()
+        plan.setpts(x1c, x2c)
+        y = plan.type2(f)
+        ctx.plan = plan
+        ctx.version = plan.pts_version
+        ctx.save_for_backward(f, x1c, x2c)
+        return y
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        f, x1, x2 = ctx.saved_tensors
+        plan: Plan = ctx.plan
+        if plan.pts_version != ctx.version:      # the plan was re-pointed since the forward
+            plan.setpts(x1, x2)
+            ctx.version = plan.pts_version
+        g = g.contiguous()
+        grad_f = grad_x1 = grad_x2 = None
+        if ctx.needs_input_grad[0]:
+            grad_f = plan.adjoint(g).reshape(f.shape)
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            gx1, gx2 = plan.type2_grad_x(g, f)
+            grad_x1 = gx1 if ctx.needs_input_grad[1] else None
+            grad_x2 = gx2 if ctx.needs_input_grad[2] else None
+        return grad_f, grad_x1, grad_x2, None
+
+
+def nfft2d_type2(f: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor, plan: Plan | None = None) -> torch.Tensor:
+    """Differentiable type-2 NFFT of the (N1, N2) centred mode array f at (x1, x2) (grid units)."""
+    if plan is None:
+        plan = Plan(f.shape[0], f.shape[1], dtype=f.dtype, device=f.device)
+    return NFFT2DType2.apply(f, x1, x2, plan)

@@ -1,0 +1,50 @@
+"""Build libnfft2d.so in-tree with hipcc for gfx950.
+
+Every HIP source under csrc/ is compiled into one shared library next to this
+file.  The packed-FP32 target feature is disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_PATH = os.path.join(PKG_DIR, "libnfft2d.so")
+SOURCES = ["nfft2d_api.hip", "ndft2d.hip"]
+ARCH = os.environ.get("NFFT2D_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def compile_cmd(out: str) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument",
+            "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile (always) and atomically install miscellaneous_amd/libnfft2d.so."""
+    tmp = LIB_PATH + f".tmp{os.getpid()}"
+    cmd = compile_cmd(tmp)
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise RuntimeError("hipcc failed:\n" + res.stdout[-8000:])
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build_library(verbose=True))

@@ -1,0 +1,591 @@
+// nfft2d_api.hip -- host side of libnfft2d.so: plans, FP64 tables, launches,
+// and the C ABI declared in include/nfft2d.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/nfft2d.h"
+#include "nfft2d_internal.hpp"
+#include "nfft2d_kernels.hpp"
+
+using namespace nfft2d;
+
+// ============================================================== errors
+static thread_local std::string g_last_error;
+
+int nfft2d_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" const char *nfft2d_last_error(void) { return g_last_error.c_str(); }
+extern "C" int nfft2d_version(void) { return 100; }
+
+// ============================================================== FP64 window math (host)
+static constexpr double kPi = 3.141592653589793238462643383279502884;
+static double host_alpha() { return (2.0 - 1.0 / kC) * kPi - 0.01; }   // NFFT/NFFT2_2D.cu:27
+
+// I0 by its power series sum_j (x^2/4)^j / (j!)^2.
+static double host_i0(double x)
+{
+    double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+    for (int j = 1; j < 1000; ++j) {
+        term *= q / ((double)j * (double)j);
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+// f(P) = sinh(alpha sqrt P)/sqrt P continued through P <= 0, and f'(P).
+static double knab_f(double P, double a)
+{
+    if (std::fabs(P) < 1.0) {
+        double a2 = a * a, term = a, sum = a;
+        for (int j = 1; j < 60; ++j) {
+            term *= a2 * P / ((2.0 * j) * (2.0 * j + 1.0));
+            sum += term;
+        }
+        return sum;
+    }
+    if (P > 0) { double s = std::sqrt(P); return std::sinh(a * s) / s; }
+    double s = std::sqrt(-P);
+    return std::sin(a * s) / s;
+}
+static double knab_fprime(double P, double a)
+{
+    if (std::fabs(P) < 1.0) {
+        double a2 = a * a, c = a * a2 / 6.0, Pp = 1.0, sum = c;
+        for (int j = 2; j < 60; ++j) {
+            c *= a2 / ((2.0 * j) * (2.0 * j + 1.0));
+            Pp *= P;
+            sum += (double)j * c * Pp;
+        }
+        return sum;
+    }
+    if (P > 0) { double s = std::sqrt(P); return (a * s * std::cosh(a * s) - std::sinh(a * s)) / (2 * s * s * s); }
+    double s = std::sqrt(-P);
+    return -(a * s * std::cos(a * s) - std::sin(a * s)) / (2 * s * s * s);
+}
+// phi_hat(d) and phi_hat'(d) (NFFT/NFFT2_2D.cu:112-122).
+static double host_phihat(double d) { return knab_f(kK * kK - d * d, host_alpha()) / kPi; }
+static double host_phihat_d(double d) { return -2.0 * d * knab_fprime(kK * kK - d * d, host_alpha()) / kPi; }
+
+// Interpolating polynomial of degree kPolyDeg at the Chebyshev nodes of
+// [-1/2, 1/2], monomial coefficients, solved in long double.
+template <typename F>
+static void fit_poly(F f, double *coef)
+{
+    constexpr int n = kPolyDeg + 1;
+    long double A[n][n + 1];
+    for (int i = 0; i < n; ++i) {
+        long double t = 0.5L * std::cos(kPi * (i + 0.5) / n);
+        long double p = 1;
+        for (int j = 0; j < n; ++j) { A[i][j] = p; p *= t; }
+        A[i][n] = f((double)t);
+    }
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs((double)A[r][c]) > std::fabs((double)A[piv][c])) piv = r;
+        for (int k = 0; k <= n; ++k) std::swap(A[c][k], A[piv][k]);
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            long double m = A[r][c] / A[c][c];
+            for (int k = c; k <= n; ++k) A[r][k] -= m * A[c][k];
+        }
+    }
+    for (int i = 0; i < n; ++i) coef[i] = (double)(A[i][n] / A[i][i]);
+}
+
+static void make_window_args(WinArgs &wa)
+{
+    const double S = host_i0(kK * host_alpha());
+    std::memset(&wa, 0, sizeof(wa));
+    double cf[kPolyDeg + 1];
+    for (int j = 0; j <= kK; ++j) {
+        fit_poly([&](double t) { return host_phihat(t - j) / S; }, cf);
+        for (int q = 0; q < kNE; ++q) wa.val.e[j][q] = (float)cf[2 * q];
+        for (int q = 0; q < kNO; ++q) wa.val.o[j][q] = j == 0 ? 0.f : (float)cf[2 * q + 1];
+        fit_poly([&](double t) { return host_phihat_d(t - j) / S; }, cf);
+        for (int q = 0; q < kNE; ++q) wa.der.e[j][q] = j == 0 ? 0.f : (float)cf[2 * q];
+        for (int q = 0; q < kNO; ++q) wa.der.o[j][q] = (float)cf[2 * q + 1];
+    }
+    wa.ex.alpha = host_alpha();
+    wa.ex.inv_pi_S = 1.0 / (kPi * S);
+    wa.ex.atK = host_alpha() / (kPi * S);
+}
+
+// ============================================================== plan
+struct nfft2d_plan_s {
+    int N1 = 0, N2 = 0, L1 = 0, L2 = 0, dtype = 0, device = 0;
+    int TS = 0, tiles1 = 0, tiles2 = 0, ntiles = 0, maxpts = 1024;
+    size_t esz = 0;                       // sizeof(complex)
+    void *tw1 = nullptr, *tw2 = nullptr;  // 2N1 / 2N2 twiddles e^{-2 pi i q/(2N)}
+    void *D1 = nullptr, *D2 = nullptr;    // deconvolution S / phi(xi_k)
+    void *grid = nullptr;                 // L2 x L1, transposed oversampled grid
+    void *tmp = nullptr;                  // N1 x L2 intermediate
+    int M = -1, Mcap = 0, subcap = 0;
+    void *xy = nullptr;
+    int *idx = nullptr, *rank = nullptr, *counts = nullptr, *offsets = nullptr, *nsub = nullptr;
+    int4 *sub = nullptr;
+    WinArgs wa;
+};
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static int max_fft_n(int dtype) { return dtype == NFFT2D_C64 ? 4096 : 2048; }
+static bool fast_path(int N, int dtype) { return is_pow2(N) && N >= 8 && N <= max_fft_n(dtype); }
+
+static void hfree(void *ptr)
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
+static void free_points(nfft2d_plan *p)
+{
+    hfree(p->xy); hfree(p->idx); hfree(p->rank); hfree(p->sub);
+    p->xy = nullptr; p->idx = p->rank = nullptr; p->sub = nullptr;
+    p->Mcap = 0;
+    p->subcap = 0;
+}
+
+static void free_plan(nfft2d_plan *p)
+{
+    if (!p) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(p->device);
+    hfree(p->tw1); hfree(p->tw2); hfree(p->D1); hfree(p->D2);
+    hfree(p->grid); hfree(p->tmp);
+    hfree(p->counts); hfree(p->offsets); hfree(p->nsub);
+    free_points(p);
+    (void)hipSetDevice(cur);
+    delete p;
+}
+
+template <typename T>
+static int upload_tables(nfft2d_plan *p)
+{
+    using C = cplx<T>;
+    const double S = host_i0(kK * host_alpha());
+    const double a = host_alpha();
+    for (int dim = 0; dim < 2; ++dim) {
+        const int N = dim == 0 ? p->N1 : p->N2;
+        std::vector<C> tw(2 * N);
+        for (int q = 0; q < 2 * N; ++q) {
+            double ang = -kPi * (double)q / (double)N;   // -2 pi q / (2N)
+            tw[q].x = (T)std::cos(ang);
+            tw[q].y = (T)std::sin(ang);
+        }
+        // D[k] = S / I0(K sqrt(alpha^2 - xi_k^2)), xi_k = 2 pi k / (c N)  (NFFT/NFFT2_2D.cu:60-69)
+        std::vector<T> D(N);
+        for (int i = 0; i < N; ++i) {
+            double xi = 2.0 * kPi * (double)(i - N / 2) / ((double)kC * N);
+            D[i] = (T)(S / host_i0(kK * std::sqrt(a * a - xi * xi)));
+        }
+        void *&twd = dim == 0 ? p->tw1 : p->tw2;
+        void *&Dd = dim == 0 ? p->D1 : p->D2;
+        if (hipMalloc(&twd, sizeof(C) * 2 * N) != hipSuccess || hipMalloc(&Dd, sizeof(T) * N) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan tables: hipMalloc failed");
+        NFFT2D_HIP_TRY(hipMemcpy(twd, tw.data(), sizeof(C) * 2 * N, hipMemcpyHostToDevice));
+        NFFT2D_HIP_TRY(hipMemcpy(Dd, D.data(), sizeof(T) * N, hipMemcpyHostToDevice));
+    }
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, int K, int c, unsigned flags)
+{
+    if (!out) return nfft2d_fail(NFFT2D_ERR_ARG, "plan_create: null output pointer");
+    *out = nullptr;
+    if (dtype != NFFT2D_C64 && dtype != NFFT2D_C128) return nfft2d_fail(NFFT2D_ERR_ARG, "plan_create: bad dtype %d", dtype);
+    if (flags != 0) return nfft2d_fail(NFFT2D_ERR_ARG, "plan_create: flags must be 0");
+    if (K != kK || c != kC) return nfft2d_fail(NFFT2D_ERR_UNSUPPORTED, "plan_create: only K=6, c=2 are supported (got K=%d, c=%d)", K, c);
+    const int maxN = dtype == NFFT2D_C64 ? 4096 : 2048;
+    if (N1 <= 0 || N2 <= 0 || (N1 & 1) || (N2 & 1) || N1 > maxN || N2 > maxN)
+        return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: N1, N2 must be even and in [2, %d] (got %d, %d)", maxN, N1, N2);
+    if ((!fast_path(N1, dtype) && N1 > 2048) || (!fast_path(N2, dtype) && N2 > 2048))
+        return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: non-power-of-two N above 2048 is not supported");
+    nfft2d_plan *p = new nfft2d_plan_s();
+    p->N1 = N1; p->N2 = N2; p->L1 = kC * N1; p->L2 = kC * N2; p->dtype = dtype;
+    p->esz = dtype == NFFT2D_C64 ? sizeof(float2) : sizeof(double2);
+    NFFT2D_HIP_TRY(hipGetDevice(&p->device));
+    p->TS = dtype == NFFT2D_C64 ? tile_size<float>() : tile_size<double>();
+    p->tiles1 = (p->L1 + p->TS - 1) / p->TS;
+    p->tiles2 = (p->L2 + p->TS - 1) / p->TS;
+    p->ntiles = p->tiles1 * p->tiles2;
+    make_window_args(p->wa);
+    int rc = dtype == NFFT2D_C64 ? upload_tables<float>(p) : upload_tables<double>(p);
+    if (rc) { free_plan(p); return rc; }
+    if (hipMalloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
+        hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
+        hipMalloc(&p->counts, sizeof(int) * p->ntiles) != hipSuccess ||
+        hipMalloc(&p->offsets, sizeof(int) * p->ntiles) != hipSuccess ||
+        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess) {
+        free_plan(p);
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the grid failed");
+    }
+    *out = p;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_plan_destroy(nfft2d_plan *p)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "plan_destroy: null plan");
+    free_plan(p);
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_plan_set_maxpts(nfft2d_plan *p, int maxpts)
+{
+    if (!p || maxpts < 32 || maxpts > (1 << 20)) return nfft2d_fail(NFFT2D_ERR_ARG, "set_maxpts: bad argument");
+    p->maxpts = maxpts;
+    p->M = -1;   // subproblems must be rebuilt
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_plan_num_points(const nfft2d_plan *p) { return p ? p->M : nfft2d_fail(NFFT2D_ERR_ARG, "null plan"); }
+
+static unsigned grid1d(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+static int zero_fill(void *ptr, long long n, hipStream_t s)
+{
+    if (n <= 0) return NFFT2D_OK;
+    unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(blocks), dim3(256), 0, s, (T *)ptr, n);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+template <typename T>
+static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_t s)
+{
+    const int subcap = p->ntiles + (M + p->maxpts - 1) / p->maxpts;
+    if (M > p->Mcap || subcap > p->subcap) {
+        free_points(p);
+        const int cap = std::max(M, 1);
+        if (hipMalloc(&p->xy, sizeof(cplx<T>) * cap) != hipSuccess || hipMalloc(&p->idx, sizeof(int) * cap) != hipSuccess ||
+            hipMalloc(&p->rank, sizeof(int) * cap) != hipSuccess ||
+            hipMalloc(&p->sub, sizeof(int4) * std::max(subcap, 1)) != hipSuccess) {
+            free_points(p);
+            p->M = -1;
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "setpts: hipMalloc for %d points failed", M);
+        }
+        p->Mcap = cap;
+        p->subcap = std::max(subcap, 1);
+    }
+    int rc = zero_fill<int>(p->counts, p->ntiles, s);
+    if (rc) return rc;
+    if (M > 0) {
+        hipLaunchKernelGGL(k_bin_count<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, p->L1, p->L2, p->TS,
+                           p->tiles1, p->counts, p->rank);
+        NFFT2D_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_scan_subprob, dim3(1), dim3(kScanThreads), 0, s, p->counts, p->ntiles, p->maxpts, p->offsets,
+                       p->sub, p->nsub);
+    NFFT2D_LAUNCH_CHECK();
+    if (M > 0) {
+        hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, p->L1, p->L2, p->TS,
+                           p->tiles1, p->rank, p->offsets, (cplx<T> *)p->xy, p->idx);
+        NFFT2D_LAUNCH_CHECK();
+    }
+    p->M = M;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_setpts(nfft2d_plan *p, const void *x, const void *y, int M, void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: null plan");
+    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: M < 0");
+    if (M > 0 && (!x || !y)) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: null coordinates");
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? setpts_impl<float>(p, (const float *)x, (const float *)y, M, s)
+                                  : setpts_impl<double>(p, (const double *)x, (const double *)y, M, s);
+}
+
+// ============================================================== pass dispatch
+template <typename F>
+static bool dispatch_n(int N, F &&f)
+{
+    switch (N) {
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    case 512: f(std::integral_constant<int, 512>{}); return true;
+    case 1024: f(std::integral_constant<int, 1024>{}); return true;
+    case 2048: f(std::integral_constant<int, 2048>{}); return true;
+    case 4096: f(std::integral_constant<int, 4096>{}); return true;
+    default: return false;
+    }
+}
+
+template <typename T> static constexpr int max_n() { return sizeof(T) == 4 ? 4096 : 2048; }
+
+// Type-2 passes (a)+(b): data (N1 x N2) -> transposed oversampled grid.
+template <typename T>
+static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
+{
+    using C = cplx<T>;
+    const C *f = (const C *)data;
+    C *Tm = (C *)p->tmp, *G = (C *)p->grid;
+    const T *D1 = (const T *)p->D1, *D2 = (const T *)p->D2;
+    const C *tw1 = (const C *)p->tw1, *tw2 = (const C *)p->tw2;
+    if (fast_path(p->N2, p->dtype)) {
+        dispatch_n(p->N2, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            if constexpr (N <= max_n<T>()) {
+                constexpr int R = rows_per_block<N>();
+                hipLaunchKernelGGL((k_t2_rows<T, N, -1>), dim3(grid1d(p->N1, R)), dim3(R * (N / 8)), 0, s, f, Tm, D1,
+                                   D2, tw2, p->N1);
+            }
+        });
+    } else {
+        hipLaunchKernelGGL((k_t2_rows_direct<T, -1>), dim3(p->N1), dim3(256), sizeof(C) * p->N2, s, f, Tm, D1, D2, tw2,
+                           p->N1, p->N2);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    if (fast_path(p->N1, p->dtype)) {
+        dispatch_n(p->N1, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            if constexpr (N <= max_n<T>()) {
+                constexpr int B = cols_per_block<N>();
+                hipLaunchKernelGGL((k_t2_cols<T, N, -1>), dim3(grid1d(p->L2, B)), dim3(B * (N / 8)), 0, s, Tm, G, tw1,
+                                   p->L2);
+            }
+        });
+    } else {
+        hipLaunchKernelGGL((k_t2_cols_direct<T, -1>), dim3(p->L2), dim3(256), sizeof(C) * p->N1, s, Tm, G, tw1, p->N1,
+                           p->L2);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+// Type-1 passes: transposed grid -> crop + deconvolve -> result (N1 x N2).
+template <typename T, int SIGN>
+static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
+{
+    using C = cplx<T>;
+    C *Tp = (C *)p->tmp, *G = (C *)p->grid, *F = (C *)result;
+    const T *D1 = (const T *)p->D1, *D2 = (const T *)p->D2;
+    const C *tw1 = (const C *)p->tw1, *tw2 = (const C *)p->tw2;
+    if (fast_path(p->N1, p->dtype)) {
+        dispatch_n(p->N1, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            if constexpr (N <= max_n<T>()) {
+                constexpr int B = cols_per_block<N>();
+                hipLaunchKernelGGL((k_t1_cols<T, N, SIGN>), dim3(grid1d(p->L2, B)), dim3(B * (N / 8)), 0, s, G, Tp,
+                                   tw1, p->L2);
+            }
+        });
+    } else {
+        hipLaunchKernelGGL((k_t1_cols_direct<T, SIGN>), dim3(p->L2), dim3(256), sizeof(C) * p->L1, s, G, Tp, tw1, p->N1,
+                           p->L2);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    if (fast_path(p->N2, p->dtype)) {
+        dispatch_n(p->N2, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            if constexpr (N <= max_n<T>()) {
+                constexpr int R = rows_per_block<N>();
+                hipLaunchKernelGGL((k_t1_rows<T, N, SIGN>), dim3(grid1d(p->N1, R)), dim3(R * (N / 8)), 0, s, Tp, F, D1,
+                                   D2, tw2, p->N1);
+            }
+        });
+    } else {
+        hipLaunchKernelGGL((k_t1_rows_direct<T, SIGN>), dim3(p->N1), dim3(256), sizeof(C) * p->L2, s, Tp, F, D1, D2,
+                           tw2, p->N1, p->N2);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+static unsigned sub_blocks(const nfft2d_plan *p) { return (unsigned)p->subcap; }
+
+template <typename T, int MODE>
+static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, void *gx1, void *gx2, hipStream_t s)
+{
+    using C = cplx<T>;
+    if (p->M == 0) return NFFT2D_OK;
+    hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(256), 0, s, (const C *)p->grid, p->L1, p->L2,
+                       (const C *)p->xy, p->idx, p->sub, p->nsub, p->tiles1, p->wa, (C *)out, (C *)d1, (C *)d2,
+                       (const C *)g, (T *)gx1, (T *)gx2);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+template <typename T, int SIGN>
+static int type1_impl(nfft2d_plan *p, void *result, const void *data, hipStream_t s)
+{
+    using C = cplx<T>;
+    int rc = zero_fill<T>(p->grid, 2LL * p->L1 * p->L2, s);
+    if (rc) return rc;
+    if (p->M > 0) {
+        hipLaunchKernelGGL((k_spread<T>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->L1, p->L2,
+                           (const C *)p->xy, p->idx, (const C *)data, p->sub, p->nsub, p->tiles1, p->wa);
+        NFFT2D_LAUNCH_CHECK();
+    }
+    return t1_passes<T, SIGN>(p, result, s);
+}
+
+static int check_exec(const nfft2d_plan *p, const void *a, const void *b)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    if (p->M < 0) return nfft2d_fail(NFFT2D_ERR_STATE, "plan has no points: call nfft2d_setpts first");
+    if (p->M > 0 && (!a || !b)) return nfft2d_fail(NFFT2D_ERR_ARG, "null data or result pointer");
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_type2(nfft2d_plan *p, void *result, const void *data, void *stream)
+{
+    int rc = check_exec(p, result, data);
+    if (rc || p->M == 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = t2_grid<float>(p, data, s))) return rc;
+        return gather<float, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+    }
+    if ((rc = t2_grid<double>(p, data, s))) return rc;
+    return gather<double, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+}
+
+extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, const void *data, void *stream)
+{
+    int rc = check_exec(p, y, data);
+    if (rc || p->M == 0) return rc;
+    if (!dy1 || !dy2) return nfft2d_fail(NFFT2D_ERR_ARG, "type2_dx: null derivative output");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = t2_grid<float>(p, data, s))) return rc;
+        return gather<float, 1>(p, y, dy1, dy2, nullptr, nullptr, nullptr, s);
+    }
+    if ((rc = t2_grid<double>(p, data, s))) return rc;
+    return gather<double, 1>(p, y, dy1, dy2, nullptr, nullptr, nullptr, s);
+}
+
+extern "C" int nfft2d_type2_grad_x(nfft2d_plan *p, void *gx1, void *gx2, const void *g, const void *data, void *stream)
+{
+    int rc = check_exec(p, gx1, data);
+    if (rc || p->M == 0) return rc;
+    if (!gx2 || !g) return nfft2d_fail(NFFT2D_ERR_ARG, "type2_grad_x: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = t2_grid<float>(p, data, s))) return rc;
+        return gather<float, 2>(p, nullptr, nullptr, nullptr, g, gx1, gx2, s);
+    }
+    if ((rc = t2_grid<double>(p, data, s))) return rc;
+    return gather<double, 2>(p, nullptr, nullptr, nullptr, g, gx1, gx2, s);
+}
+
+extern "C" int nfft2d_type1(nfft2d_plan *p, void *result, const void *data, void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    int rc = check_exec(p, data, data);
+    if (rc) return rc;
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? type1_impl<float, -1>(p, result, data, s) : type1_impl<double, -1>(p, result, data, s);
+}
+
+extern "C" int nfft2d_adjoint(nfft2d_plan *p, void *result, const void *data, void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    int rc = check_exec(p, data, data);
+    if (rc) return rc;
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? type1_impl<float, +1>(p, result, data, s) : type1_impl<double, +1>(p, result, data, s);
+}
+
+// ============================================================== drop-in entry points
+namespace {
+std::mutex g_cache_mu;
+std::map<std::tuple<int, int, int, int>, nfft2d_plan *> g_cache;
+
+int cached_plan(int N1, int N2, int dtype, nfft2d_plan **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "hipGetDevice failed");
+    auto key = std::make_tuple(N1, N2, dtype, dev);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) { *out = it->second; return NFFT2D_OK; }
+    int rc = nfft2d_plan_create(out, N1, N2, dtype, kK, kC, 0);
+    if (rc) return rc;
+    g_cache[key] = *out;
+    return NFFT2D_OK;
+}
+
+int dropin(int dtype, bool type2, void *result, const void *data, const void *x, const void *y, int N1, int N2, int M)
+{
+    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "M < 0");
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
+    if (M > 0 && (!data || !x || !y)) return nfft2d_fail(NFFT2D_ERR_ARG, "null data or coordinate pointer");
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    nfft2d_plan *p = nullptr;
+    int rc = cached_plan(N1, N2, dtype, &p);
+    if (rc) return rc;
+    if ((rc = nfft2d_setpts(p, x, y, M, nullptr))) return rc;
+    if (type2) return M == 0 ? NFFT2D_OK : nfft2d_type2(p, result, data, nullptr);
+    return nfft2d_type1(p, result, data, nullptr);
+}
+}  // namespace
+
+extern "C" int NFFT2_2D_GPU_c64(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y, int N1,
+                                int N2, int M)
+{
+    return dropin(NFFT2D_C64, false, result, data, x, y, N1, N2, M);
+}
+extern "C" int NFFT2_2D_GPU_c128(nfft2d_c128 *result, const nfft2d_c128 *data, const double *x, const double *y,
+                                 int N1, int N2, int M)
+{
+    return dropin(NFFT2D_C128, false, result, data, x, y, N1, N2, M);
+}
+extern "C" int NFFT1_2D_GPU_c64(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y, int N1,
+                                int N2, int M)
+{
+    return dropin(NFFT2D_C64, true, result, data, x, y, N1, N2, M);
+}
+extern "C" int NFFT1_2D_GPU_c128(nfft2d_c128 *result, const nfft2d_c128 *data, const double *x, const double *y,
+                                 int N1, int N2, int M)
+{
+    return dropin(NFFT2D_C128, true, result, data, x, y, N1, N2, M);
+}
+
+extern "C" void Calculate_cuFFT_plan_C2C_NFFT2_2D(int N1, int N2)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    nfft2d_plan *p;
+    cached_plan(N1, N2, NFFT2D_C64, &p);
+}
+extern "C" void Calculate_cuFFT_plan_Z2Z_NFFT2_2D(int N1, int N2)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    nfft2d_plan *p;
+    cached_plan(N1, N2, NFFT2D_C128, &p);
+}
+extern "C" void Destroy_cuFFT_plan_NFFT2_2D(void)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : g_cache) free_plan(kv.second);
+    g_cache.clear();
+}

@@ -1,0 +1,301 @@
+/*
+ * nfft2d_oracle.c -- CPU restatement of the reference's 2D NFFT path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it.  The
+ * product (miscellaneous_amd/libnfft2d.so) never links or calls it.
+ *
+ * What it restates, step by step (paths relative to the reference root):
+ *   orc_alpha            NFFT/NFFT2_2D.cu:27, BP_FBP_FFBP/NFFT1_2D.m:11
+ *   orc_bessel_i0        the I0 behind Kernel_Bessel (NFFT/NFFT2_2D.cu:67) and
+ *                        besseli(0,.) (BP_FBP_FFBP/NFFT1_2D.m:21-22); the
+ *                        reference's own bessi0 lives in a header that is not in
+ *                        the repository, so this is the textbook power series
+ *                        sum_j (x^2/4)^j / (j!)^2.
+ *   orc_window           spatial window phi_hat, NFFT/NFFT2_2D.cu:112-122 and
+ *                        BP_FBP_FFBP/NFFT1_2D.m:34-41 (three branches).
+ *   orc_window_deriv     d phi_hat / d d -- new (the reference never
+ *                        differentiates); closed form plus a series near |d|=K.
+ *   orc_nfft1_2d_{d,f}   type-2 (uniform -> non-uniform), BP_FBP_FFBP/NFFT1_2D.m
+ *                        whole file: windows :17-42, scale+pad :48-49,
+ *                        U = fft2(ifftshift(u)) :55, 13x13 gather :61-67.
+ *   orc_nfft2_2d_{d,f}   type-1 (non-uniform -> uniform), NFFT/NFFT2_2D.cu
+ *                        :329-411 (float) / :413-495 (double): Bessel table
+ *                        :60-69, spread :163-218 with series_terms :105-130,
+ *                        fftshift -> forward FFT -> fftshift :383-393,
+ *                        Decimation_and_Scaling :74-100.  Also the MATLAB twin
+ *                        NFFT/NFFT2/Matlab/NFFT2_2D.m.
+ *   orc_ndft2_2d         direct NED sum of NFFT/NDFT2_2D.cu:28-45,80
+ *                        (Kernel_Matrix_Filling + cublasZgemv OP_T).
+ *   orc_ndft_type2(_dx)  direct type-2 sum y_l = sum_k f_k e^{-2 pi i k.x_l/N}
+ *                        (sign pinned by NFFT/NFFT1/1D/Matlab/NFFT1_1D_BLAS.m:5-16)
+ *                        and its analytic derivative in x.
+ *
+ * Conventions: complex arrays are interleaved (re, im).  Coordinates are in
+ * grid-index units (period N), modes k in [-N/2, N/2) stored at index k+N/2,
+ * 2-D arrays row-major [k1][k2].
+ *
+ * round_mode: 0 = rint (half to even, the CUDA path NFFT/NFFT2_2D.cu:168),
+ *             1 = MATLAB round (half away from zero, NFFT1_2D.m:27).
+ * mutation:   0 in every normal use.  Non-zero values plant a known mistake
+ *             so that the tests can show their checks catch it:
+ *             1 = flip the sign of the FFT exponent,
+ *             2 = skip the ifftshift before the FFT,
+ *             3 = window tap offset shifted by one grid cell.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.141592653589793238462643383279502884
+
+double orc_alpha(int c) { return (2.0 - 1.0 / (double)c) * ORC_PI - 0.01; }
+
+double orc_bessel_i0(double x)
+{
+    double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+    for (int j = 1; j < 1000; ++j) {
+        term *= q / ((double)j * (double)j);
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+/* f(P) = sinh(alpha sqrt(P)) / sqrt(P), continued through P <= 0 (sin branch,
+ * value alpha at P == 0).  Power series sum_j alpha^(2j+1) P^j / (2j+1)! for
+ * |P| < 1, closed forms otherwise. */
+static double knab_f(double P, double alpha)
+{
+    if (fabs(P) < 1.0) {
+        double a2 = alpha * alpha, term = alpha, sum = alpha;
+        for (int j = 1; j < 60; ++j) {
+            term *= a2 * P / ((2.0 * j) * (2.0 * j + 1.0));
+            sum += term;
+            if (fabs(term) < 1e-18 * fabs(sum)) break;
+        }
+        return sum;
+    }
+    if (P > 0) { double s = sqrt(P); return sinh(alpha * s) / s; }
+    double s = sqrt(-P);
+    return sin(alpha * s) / s;
+}
+
+/* f'(P) */
+static double knab_fprime(double P, double alpha)
+{
+    if (fabs(P) < 1.0) {
+        /* sum_{j>=1} j alpha^(2j+1) P^(j-1) / (2j+1)! */
+        double a2 = alpha * alpha;
+        double c = alpha * a2 / 6.0;           /* alpha^3/3!, j = 1 */
+        double Pp = 1.0, sum = c;
+        for (int j = 2; j < 60; ++j) {
+            c *= a2 / ((2.0 * j) * (2.0 * j + 1.0));
+            Pp *= P;
+            double t = (double)j * c * Pp;
+            sum += t;
+            if (fabs(t) < 1e-18 * fabs(sum)) break;
+        }
+        return sum;
+    }
+    if (P > 0) {
+        double s = sqrt(P);
+        return (alpha * s * cosh(alpha * s) - sinh(alpha * s)) / (2.0 * s * s * s);
+    }
+    double s = sqrt(-P);
+    /* f = sin(as)/s with s = sqrt(-P): df/dP = -(as cos(as) - sin(as)) / (2 s^3) */
+    return -(alpha * s * cos(alpha * s) - sin(alpha * s)) / (2.0 * s * s * s);
+}
+
+/* Spatial window phi_hat(d) = (1/pi) sinh(alpha sqrt(K^2-d^2)) / sqrt(K^2-d^2)
+ * (NFFT/NFFT2_2D.cu:112-116; BP_FBP_FFBP/NFFT1_2D.m:34-36).  The reference
+ * evaluates the branches directly; near P = 0 the series above agrees with
+ * them to rounding. */
+double orc_window(double d, int K, double alpha)
+{
+    double P = (double)K * (double)K - d * d;
+    if (P == 0.0) return alpha / ORC_PI;
+    if (P > 0.0) { double s = sqrt(P); return sinh(alpha * s) / (s * ORC_PI); }
+    double s = sqrt(-P);
+    return sin(alpha * s) / (s * ORC_PI);
+}
+
+double orc_window_series(double d, int K, double alpha)
+{
+    double P = (double)K * (double)K - d * d;
+    return knab_f(P, alpha) / ORC_PI;
+}
+
+double orc_window_deriv(double d, int K, double alpha)
+{
+    double P = (double)K * (double)K - d * d;
+    return -2.0 * d * knab_fprime(P, alpha) / ORC_PI;
+}
+
+/* Spectral window phi(xi) = I0(K sqrt(alpha^2 - xi^2)) at xi = 2 pi k / (c N),
+ * k = -N/2 .. N/2-1 (BP_FBP_FFBP/NFFT1_2D.m:17-22; NFFT/NFFT2_2D.cu:66-67). */
+void orc_spectral_window(double *phi, int N, int K, int c)
+{
+    double alpha = orc_alpha(c);
+    for (int i = 0; i < N; ++i) {
+        double xi = 2.0 * ORC_PI * (double)(i - N / 2) / ((double)c * (double)N);
+        phi[i] = orc_bessel_i0((double)K * sqrt(alpha * alpha - xi * xi));
+    }
+}
+
+static double round_half_away(double v) { return v >= 0 ? floor(v + 0.5) : -floor(-v + 0.5); }
+static double orc_round(double v, int mode) { return mode == 1 ? round_half_away(v) : nearbyint(v); }
+static long pos_mod(long a, long n) { long r = a % n; return r < 0 ? r + n : r; }
+
+/* -------------------------------------------------------------------------- */
+/* Instantiate the transform bodies for double and float.                      */
+#define REAL double
+#define SFX(name) name##_d
+#include "oracle_body.inc"
+#undef REAL
+#undef SFX
+
+#define REAL float
+#define SFX(name) name##_f
+#include "oracle_body.inc"
+#undef REAL
+#undef SFX
+
+/* -------------------------------------------------------------------------- */
+/* Direct sums (FP64 only).                                                     */
+
+/* exp(-2 pi i t) with t reduced mod 1 first, so large phases keep full
+ * precision. */
+static void cis_neg(double t, double *re, double *im)
+{
+    t -= floor(t);
+    double a = 2.0 * ORC_PI * t;
+    *re = cos(a);
+    *im = -sin(a);
+}
+
+/* NFFT/NDFT2_2D.cu:28-45,80: out[n] = sum_m in[m] exp(-2 pi i (u_n X_m / Nu + v_n Y_m / Nv)). */
+void orc_ndft2_2d(const double *X, const double *Y, const double *u, const double *v,
+                  const double *in, double *out, int Nu, int Nv, int M, int N)
+{
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < N; ++n) {
+        double sr = 0, si = 0;
+        for (int m = 0; m < M; ++m) {
+            double t = u[n] * X[m] / (double)Nu + v[n] * Y[m] / (double)Nv;
+            double cr, ci;
+            cis_neg(t, &cr, &ci);
+            sr += in[2 * m] * cr - in[2 * m + 1] * ci;
+            si += in[2 * m] * ci + in[2 * m + 1] * cr;
+        }
+        out[2 * n] = sr;
+        out[2 * n + 1] = si;
+    }
+}
+
+/* Direct type-2 sum y_l = sum_{k1,k2} f[k1][k2] e^{-2 pi i (k1 x1_l / N1 + k2 x2_l / N2)},
+ * k in [-N/2, N/2) at index k + N/2.  When dx1/dx2 are non-NULL also the
+ * derivatives dy_l/dx1_l and dy_l/dx2_l. */
+void orc_ndft_type2(double *y, double *dx1, double *dx2, const double *f,
+                    const double *x1, const double *x2, int N1, int N2, int M)
+{
+#pragma omp parallel
+    {
+        double *e1 = (double *)malloc(sizeof(double) * 2 * N1);
+        double *e2 = (double *)malloc(sizeof(double) * 2 * N2);
+#pragma omp for schedule(static)
+        for (int l = 0; l < M; ++l) {
+            for (int i = 0; i < N1; ++i) cis_neg((double)(i - N1 / 2) * x1[l] / (double)N1, &e1[2 * i], &e1[2 * i + 1]);
+            for (int j = 0; j < N2; ++j) cis_neg((double)(j - N2 / 2) * x2[l] / (double)N2, &e2[2 * j], &e2[2 * j + 1]);
+            double yr = 0, yi = 0, ar = 0, ai = 0, br = 0, bi = 0;
+            for (int i = 0; i < N1; ++i) {
+                double sr = 0, si = 0, tr = 0, ti = 0;
+                const double *row = f + (size_t)2 * i * N2;
+                for (int j = 0; j < N2; ++j) {
+                    double pr = row[2 * j] * e2[2 * j] - row[2 * j + 1] * e2[2 * j + 1];
+                    double pi = row[2 * j] * e2[2 * j + 1] + row[2 * j + 1] * e2[2 * j];
+                    sr += pr; si += pi;
+                    double k2 = (double)(j - N2 / 2);
+                    tr += k2 * pr; ti += k2 * pi;
+                }
+                double cr = e1[2 * i], ci = e1[2 * i + 1];
+                double qr = sr * cr - si * ci, qi = sr * ci + si * cr;
+                yr += qr; yi += qi;
+                double k1 = (double)(i - N1 / 2);
+                ar += k1 * qr; ai += k1 * qi;
+                br += tr * cr - ti * ci; bi += tr * ci + ti * cr;
+            }
+            y[2 * l] = yr; y[2 * l + 1] = yi;
+            /* d/dx e^{-2 pi i k x / N} = (-2 pi i k / N) e^{...} */
+            if (dx1) { double s = -2.0 * ORC_PI / (double)N1; dx1[2 * l] = -s * ai; dx1[2 * l + 1] = s * ar; }
+            if (dx2) { double s = -2.0 * ORC_PI / (double)N2; dx2[2 * l] = -s * bi; dx2[2 * l + 1] = s * br; }
+        }
+        free(e1);
+        free(e2);
+    }
+}
+
+/* Direct type-1 sum F[k1][k2] = sum_l d_l e^{-2 pi i (k1 x1_l / N1 + k2 x2_l / N2)}. */
+void orc_ndft_type1(double *F, const double *d, const double *x1, const double *x2,
+                    int N1, int N2, int M)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N1; ++i) {
+        double *acc = (double *)calloc((size_t)2 * N2, sizeof(double));
+        for (int l = 0; l < M; ++l) {
+            double cr, ci;
+            cis_neg((double)(i - N1 / 2) * x1[l] / (double)N1, &cr, &ci);
+            double ar = d[2 * l] * cr - d[2 * l + 1] * ci, ai = d[2 * l] * ci + d[2 * l + 1] * cr;
+            for (int j = 0; j < N2; ++j) {
+                double er, ei;
+                cis_neg((double)(j - N2 / 2) * x2[l] / (double)N2, &er, &ei);
+                acc[2 * j] += ar * er - ai * ei;
+                acc[2 * j + 1] += ar * ei + ai * er;
+            }
+        }
+        memcpy(F + (size_t)2 * i * N2, acc, sizeof(double) * 2 * N2);
+        free(acc);
+    }
+}
+
+/* Direct type-1 sum at a subset of modes: F[q] = sum_l d_l e^{-2 pi i (k1_q x1_l / N1 + k2_q x2_l / N2)}
+ * with (k1_q, k2_q) = (i1[q] - N1/2, i2[q] - N2/2). */
+void orc_ndft_type1_modes(double *F, const double *d, const double *x1, const double *x2, int N1, int N2, int M,
+                          const int *i1, const int *i2, int nq)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q) {
+        double k1 = (double)(i1[q] - N1 / 2), k2 = (double)(i2[q] - N2 / 2);
+        double sr = 0, si = 0;
+        for (int l = 0; l < M; ++l) {
+            double cr, ci;
+            cis_neg(k1 * x1[l] / (double)N1 + k2 * x2[l] / (double)N2, &cr, &ci);
+            sr += d[2 * l] * cr - d[2 * l + 1] * ci;
+            si += d[2 * l] * ci + d[2 * l + 1] * cr;
+        }
+        F[2 * q] = sr;
+        F[2 * q + 1] = si;
+    }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -1,0 +1,232 @@
+"""Pins for the CPU oracle (the checker every GPU result is compared with).
+
+Each function of the oracle is checked against something it was not derived
+from: 50-digit mpmath closed forms, scipy/numpy library routines, committed
+FP64 direct-sum golden vectors, invariants of the transform (adjoint,
+shift theorem, periodicity) and the reference's own known-answer relation
+NFFT/NFFT2/Matlab/Test_NFFT2_2D.m:24-29.  Planted mistakes in the oracle
+(``mutation``) must fail these checks.
+"""
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.special
+
+import oracle
+
+K = 6
+C = 2
+TOL_FAST = 1e-10      # c=2, K=6 has an intrinsic error of ~5e-12 rel-L2
+
+
+# ------------------------------------------------------------------ scalar pieces
+def test_alpha_matches_reference_constant(golden):
+    w = golden("window_mp.npz")
+    # NFFT/NFFT2_2D.cu:27: (2 - 1/cc) * pi - 0.01
+    assert oracle.alpha(2) == pytest.approx(float(w["alpha"]), rel=1e-15)
+    assert oracle.alpha(2) == pytest.approx(1.5 * np.pi - 0.01, rel=1e-15)
+
+
+def test_bessel_i0_vs_mpmath_and_scipy(golden):
+    w = golden("window_mp.npz")
+    for x, ref in zip(w["i0_x"], w["i0"]):
+        assert oracle.bessel_i0(x) == pytest.approx(ref, rel=2e-15)
+        assert oracle.bessel_i0(x) == pytest.approx(float(scipy.special.i0(x)), rel=1e-13)
+
+
+def test_spectral_window_vs_mpmath(golden):
+    w = golden("window_mp.npz")
+    for N in (16, 22, 128):
+        np.testing.assert_allclose(oracle.spectral_window(N), w[f"spec_{N}"], rtol=1e-14)
+
+
+def test_window_vs_mpmath(golden):
+    w = golden("window_mp.npz")
+    got = oracle.window(w["d"])
+    np.testing.assert_allclose(got, w["phi"], rtol=1e-13, atol=1e-12 * np.abs(w["phi"]).max())
+    got_s = oracle.window(w["d"], series=True)
+    np.testing.assert_allclose(got_s, w["phi"], rtol=1e-13, atol=1e-12 * np.abs(w["phi"]).max())
+
+
+def test_window_is_fourier_pair_of_spectral_window():
+    """phi_hat(d) = (1/2pi) int_{-alpha}^{alpha} I0(K sqrt(alpha^2 - xi^2)) cos(xi d) dxi."""
+    a = oracle.alpha(C)
+    for d in (0.0, 0.37, 1.3, 4.0, 5.9, 6.0, 6.3):
+        val, _ = scipy.integrate.quad(lambda xi: scipy.special.i0(K * np.sqrt(max(a * a - xi * xi, 0.0)))
+                                      * np.cos(xi * d), -a, a, limit=400, epsabs=0, epsrel=1e-12)
+        ref = val / (2 * np.pi)
+        scale = oracle.window([0.0])[0]
+        assert abs(oracle.window([d])[0] - ref) <= 1e-9 * scale
+
+
+def test_window_deriv_vs_mpmath(golden):
+    w = golden("window_mp.npz")
+    got = oracle.window_deriv(w["d"])
+    np.testing.assert_allclose(got, w["dphi"], rtol=1e-9, atol=1e-10 * np.abs(w["dphi"]).max())
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("shape", [(8, 8), (16, 32), (44, 44), (12, 20), (6, 10)])
+def test_fft2_vs_numpy(shape):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    np.testing.assert_allclose(oracle.fft2(a), np.fft.fft2(a), rtol=0, atol=1e-12 * np.abs(a).sum())
+    np.testing.assert_allclose(oracle.fft2(a, sign=+1), np.fft.ifft2(a) * a.size, rtol=0,
+                               atol=1e-12 * np.abs(a).sum())
+
+
+def test_fft2_impulse_and_exponential():
+    L1, L2 = 16, 8
+    a = np.zeros((L1, L2), complex)
+    a[3, 5] = 1.0
+    n1, n2 = np.meshgrid(np.arange(L1), np.arange(L2), indexing="ij")
+    np.testing.assert_allclose(oracle.fft2(a), np.exp(-2j * np.pi * (3 * n1 / L1 + 5 * n2 / L2)), atol=1e-14)
+    e = np.exp(2j * np.pi * (2 * n1 / L1 + 1 * n2 / L2))
+    ref = np.zeros((L1, L2), complex)
+    ref[2, 1] = L1 * L2
+    np.testing.assert_allclose(oracle.fft2(e), ref, atol=1e-12)
+
+
+# ------------------------------------------------------------------ full transforms
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+@pytest.mark.parametrize("round_mode", [0, 1])
+def test_type2_vs_golden_direct_sum(golden, tag, round_mode):
+    g = golden("tiny.npz")
+    f = g[f"{tag}_f"].astype(np.complex128)
+    y = oracle.nfft1_2d(f, g[f"{tag}_x1"], g[f"{tag}_x2"], round_mode=round_mode)
+    assert oracle.rel_l2(y, g[f"{tag}_y"]) < TOL_FAST
+
+
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+@pytest.mark.parametrize("round_mode", [0, 1])
+def test_type1_vs_golden_direct_sum(golden, tag, round_mode):
+    g = golden("tiny.npz")
+    f = g[f"{tag}_f"]
+    N1, N2 = f.shape
+    F = oracle.nfft2_2d(g[f"{tag}_d"].astype(np.complex128), g[f"{tag}_x1"], g[f"{tag}_x2"], N1, N2,
+                        round_mode=round_mode)
+    assert oracle.rel_l2(F, g[f"{tag}_F"]) < TOL_FAST
+
+
+def test_c1_type2_and_type1_vs_golden(golden):
+    g = golden("c1_128_16384.npz")
+    y = oracle.nfft1_2d(g["f"].astype(np.complex128), g["x1"], g["x2"])
+    assert oracle.rel_l2(y, g["y"]) < TOL_FAST
+    F = oracle.nfft2_2d(g["d"].astype(np.complex128), g["x1"], g["x2"], 128, 128)
+    assert oracle.rel_l2(F, g["F"]) < TOL_FAST
+
+
+def test_fp32_variant_within_single_precision(golden):
+    g = golden("c1_128_16384.npz")
+    y = oracle.nfft1_2d(g["f"], g["x1"], g["x2"], precision="float")
+    assert oracle.rel_l2(y, g["y"]) < 2e-6
+    F = oracle.nfft2_2d(g["d"], g["x1"], g["x2"], 128, 128, precision="float")
+    assert oracle.rel_l2(F, g["F"]) < 2e-6
+
+
+def test_direct_sums_vs_golden(golden):
+    g = golden("tiny.npz")
+    for tag in "abc":
+        f = g[f"{tag}_f"].astype(np.complex128)
+        y, d1, d2 = oracle.ndft_type2(f, g[f"{tag}_x1"], g[f"{tag}_x2"], with_dx=True)
+        assert oracle.rel_l2(y, g[f"{tag}_y"]) < 1e-12
+        assert oracle.rel_l2(d1, g[f"{tag}_dy1"]) < 1e-12
+        assert oracle.rel_l2(d2, g[f"{tag}_dy2"]) < 1e-12
+        N1, N2 = f.shape
+        F = oracle.ndft_type1(g[f"{tag}_d"].astype(np.complex128), g[f"{tag}_x1"], g[f"{tag}_x2"], N1, N2)
+        assert oracle.rel_l2(F, g[f"{tag}_F"]) < 1e-12
+
+
+def test_derivative_vs_central_differences():
+    rng = np.random.default_rng(7)
+    N1, N2, M = 16, 16, 64
+    f = rng.standard_normal((N1, N2)) + 1j * rng.standard_normal((N1, N2))
+    x1 = rng.uniform(-8, 8, M)
+    x2 = rng.uniform(-8, 8, M)
+    _, d1, d2 = oracle.ndft_type2(f, x1, x2, with_dx=True)
+    h = 1e-3
+    fd1 = (oracle.ndft_type2(f, x1 + h, x2) - oracle.ndft_type2(f, x1 - h, x2)) / (2 * h)
+    fd2 = (oracle.ndft_type2(f, x1, x2 + h) - oracle.ndft_type2(f, x1, x2 - h)) / (2 * h)
+    assert oracle.rel_l2(d1, fd1) < 1e-4     # O(h^2 (2 pi k/N)^2 / 6)
+    assert oracle.rel_l2(d2, fd2) < 1e-4
+
+
+def test_ndft2_2d_vs_golden(golden):
+    g = golden("ndft.npz")
+    for p in ("g", "m"):
+        out = oracle.ndft2_2d(g[f"{p}_X"], g[f"{p}_Y"], g[f"{p}_u"], g[f"{p}_v"], g[f"{p}_in"],
+                              int(g[f"{p}_Nu"]), int(g[f"{p}_Nv"]))
+        assert oracle.rel_l2(out, g[f"{p}_out"]) < 1e-13
+
+
+def test_reference_known_answer_relation(golden):
+    """Test_NFFT2_2D.m:24-29: NDFT2_2D(data,x,y,N,M) vs NFFT2_2D(data,y,x,M,N)."""
+    g = golden("kat_test_nfft2_2d.npz")
+    N, M = int(g["N"]), int(g["M"])
+    t2 = oracle.nfft2_2d(g["data"].astype(np.complex128), g["y"], g["x"], M, N, round_mode=1)
+    assert 100 * oracle.rel_l2(t2, g["transf1"]) < 1e-8      # the script prints 100*rel-L2
+    t2r = oracle.nfft2_2d(g["data"].astype(np.complex128), g["y"], g["x"], M, N, round_mode=0)
+    assert 100 * oracle.rel_l2(t2r, g["transf1"]) < 1e-8
+
+
+# ------------------------------------------------------------------ invariants
+def _case(seed=3, N1=16, N2=24, M=200):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((N1, N2)) + 1j * rng.standard_normal((N1, N2))
+    x1 = rng.uniform(-N1 / 2, N1 / 2, M)
+    x2 = rng.uniform(-N2 / 2, N2 / 2, M)
+    d = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    return f, x1, x2, d
+
+
+def test_adjoint_identity():
+    """<A f, g> = <f, A^H g> with A^H g = conj(type1(conj g)) (the reference's
+    own conj trick, BP_FBP_FFBP/NFFT_Based_Selvas_Approach_for_Signal_Interp_2D.m:28)."""
+    f, x1, x2, g = _case()
+    N1, N2 = f.shape
+    Af = oracle.nfft1_2d(f, x1, x2)
+    AHg = np.conj(oracle.nfft2_2d(np.conj(g), x1, x2, N1, N2))
+    lhs = np.vdot(g, Af)
+    rhs = np.vdot(AHg, f)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs) * 10
+
+
+def test_shift_theorem_and_periodicity():
+    f, x1, x2, d = _case()
+    N1, N2 = f.shape
+    k1 = np.arange(-N1 // 2, N1 // 2)
+    y = oracle.nfft1_2d(f, x1, x2)
+    y_shift = oracle.nfft1_2d(f, x1 + 1.0, x2)
+    y_mod = oracle.nfft1_2d(f * np.exp(-2j * np.pi * k1 / N1)[:, None], x1, x2)
+    assert oracle.rel_l2(y_shift, y_mod) < TOL_FAST
+    assert oracle.rel_l2(oracle.nfft1_2d(f, x1 + N1, x2 - 2 * N2), y) < TOL_FAST
+    F = oracle.nfft2_2d(d, x1, x2, N1, N2)
+    assert oracle.rel_l2(oracle.nfft2_2d(d, x1 - N1, x2 + N2, N1, N2), F) < TOL_FAST
+
+
+def test_single_point_and_single_mode_closed_forms():
+    N1, N2 = 16, 12
+    k1 = np.arange(-N1 // 2, N1 // 2)[:, None]
+    k2 = np.arange(-N2 // 2, N2 // 2)[None, :]
+    x, y = 3.37, -4.81
+    F = oracle.nfft2_2d(np.array([1.0 + 0j]), np.array([x]), np.array([y]), N1, N2)
+    ref = np.exp(-2j * np.pi * (k1 * x / N1 + k2 * y / N2))
+    assert oracle.rel_l2(F, ref) < TOL_FAST
+    f = np.zeros((N1, N2), complex)
+    f[N1 // 2 + 3, N2 // 2 - 5] = 1.0
+    rng = np.random.default_rng(5)
+    x1 = rng.uniform(-8, 8, 50)
+    x2 = rng.uniform(-6, 6, 50)
+    yy = oracle.nfft1_2d(f, x1, x2)
+    assert oracle.rel_l2(yy, np.exp(-2j * np.pi * (3 * x1 / N1 - 5 * x2 / N2))) < TOL_FAST
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3])
+def test_planted_mistakes_are_caught(golden, mutation):
+    g = golden("tiny.npz")
+    f = g["b_f"].astype(np.complex128)
+    N1, N2 = f.shape
+    y = oracle.nfft1_2d(f, g["b_x1"], g["b_x2"], mutation=mutation)
+    assert oracle.rel_l2(y, g["b_y"]) > 1e-2
+    F = oracle.nfft2_2d(g["b_d"].astype(np.complex128), g["b_x1"], g["b_x2"], N1, N2, mutation=mutation)
+    assert oracle.rel_l2(F, g["b_F"]) > 1e-2
