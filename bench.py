@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2D NFFT hot path on one MI355X.
+
+Headline metric: 2D type-2 NFFT throughput in Mpts/s on the polar
+backprojection workload -- 512 x 512 complex64 CN(0,1) modes, M = 1,048,576
+points on 512 equispaced angles in [0, pi) x 2048 radii in [-1/2, 1/2), K = 6,
+c = 2 -- with its rel-L2 against an FP64 direct sum (4096 seeded points).
+
+One step = setpts (tile sort of the M points) + type-2 (window
+pre-compensation + zero padding + 2-D FFT + (2K+1)^2 gather), replayed from a
+HIP graph so that host launch overhead is not measured.  The exec-only rate
+(points already sorted) and the per-kernel times (HIP events on the stream the
+kernels run on) are reported beside it.
+
+Secondary: fwd+adjoint gradient steps/s on 256 x 256 modes, M = 262,144
+uniform points (loss ||A(x) f - y||^2, gradient w.r.t. f and x, gradient
+descent update), also graph-replayed.
+
+cpu_baseline: the CPU restatement of the reference algorithm (FP32,
+OpenMP over the host cores) on the same type-2 workload, timed for a bounded
+number of repetitions.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+
+
+def cn01(gen, shape, device):
+    re = torch.randn(shape, generator=gen, dtype=torch.float32, device="cpu")
+    im = torch.randn(shape, generator=gen, dtype=torch.float32, device="cpu")
+    return (torch.complex(re, im) / np.sqrt(2.0)).to(device)
+
+
+def polar_points(N: int, n_ang: int = 512, n_rad: int = 2048):
+    th = np.arange(n_ang) * np.pi / n_ang
+    r = -0.5 + np.arange(n_rad) / n_rad
+    R, TH = np.meshgrid(r, th)
+    x1 = (R * np.cos(TH)).astype(np.float32).ravel() * np.float32(N)
+    x2 = (R * np.sin(TH)).astype(np.float32).ravel() * np.float32(N)
+    return x1.astype(np.float32), x2.astype(np.float32)
+
+
+def direct_type2_subset(f: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor) -> torch.Tensor:
+    """FP64 direct sum y_l = sum_k f_k e^{-2 pi i (k1 x1_l/N1 + k2 x2_l/N2)} in torch (separable)."""
+    N1, N2 = f.shape
+    k1 = torch.arange(-N1 // 2, N1 // 2, device=f.device, dtype=torch.float64)
+    k2 = torch.arange(-N2 // 2, N2 // 2, device=f.device, dtype=torch.float64)
+    ph1 = torch.outer(x1.double(), k1) / N1
+    ph2 = torch.outer(x2.double(), k2) / N2
+    A = torch.polar(torch.ones_like(ph1), -2 * np.pi * (ph1 - torch.floor(ph1)))
+    B = torch.polar(torch.ones_like(ph2), -2 * np.pi * (ph2 - torch.floor(ph2)))
+    G = B @ f.to(torch.complex128).T.contiguous()
+    return (A * G).sum(dim=1)
+
+
+def capture(fn, warm: int = 3):
+    """Capture fn() into a CUDA/HIP graph after a few eager warm-up calls."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(warm):
+            fn()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    torch.cuda.synchronize()
+    return g
+
+
+def time_graph(g, steps: int, warmup: int) -> float:
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def event_ms(pairs) -> float:
+    return float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+
+
+def bench_type2(args, dev):
+    import miscellaneous_amd as m
+
+    N = 512
+    x1n, x2n = polar_points(N)
+    M = len(x1n)
+    gen = torch.Generator().manual_seed(SEED)
+    f = cn01(gen, (N, N), dev)
+    x1 = torch.from_numpy(x1n).to(dev)
+    x2 = torch.from_numpy(x2n).to(dev)
+    y = torch.empty(M, dtype=torch.complex64, device=dev)
+    plan = m.Plan(N, N, torch.complex64, device=dev)
+    plan.setpts(x1, x2)
+    plan.type2(f, out=y)
+    torch.cuda.synchronize()
+
+    # accuracy on 4096 seeded points against an FP64 direct sum
+    sel = torch.from_numpy(np.random.default_rng(SEED).choice(M, 4096, replace=False)).to(dev)
+    y_ref = direct_type2_subset(f, x1[sel], x2[sel])
+    rel = ((y[sel].to(torch.complex128) - y_ref).norm() / y_ref.norm()).item()
+
+    # per-kernel times with HIP events on the stream the kernels run on
+    ev = []
+    for _ in range(max(args.steps, 20)):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record()
+        plan.setpts(x1, x2)
+        e[1].record()
+        plan.type2_grid(f)
+        e[2].record()
+        plan.type2_interp(y)
+        e[3].record()
+        ev.append(e)
+    torch.cuda.synchronize()
+    setpts_ms = event_ms([(e[0], e[1]) for e in ev])
+    grid_ms = event_ms([(e[1], e[2]) for e in ev])
+    gather_ms = event_ms([(e[2], e[3]) for e in ev])
+
+    def step_full():
+        plan.setpts(x1, x2)
+        plan.type2(f, out=y)
+
+    def step_exec():
+        plan.type2(f, out=y)
+
+    g_full = capture(step_full)
+    g_exec = capture(step_exec)
+    s_full = time_graph(g_full, args.steps, args.warmup)
+    s_exec = time_graph(g_exec, args.steps, args.warmup)
+
+    # graph replays must reproduce the eager result
+    ref = y.clone()
+    g_exec.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(ref, y), "graph replay differs from the eager type-2"
+
+    gather_bytes = 32 * N * N + 16 * M          # grid read once + coordinates + output
+    type2_bytes = 200 * N * N + 16 * M          # stages (a)+(b)+(c)
+    return dict(N=N, M=M, rel_l2=rel, s_full=s_full, s_exec=s_exec, setpts_ms=setpts_ms, grid_ms=grid_ms,
+                gather_ms=gather_ms, gather_bytes=gather_bytes, type2_bytes=type2_bytes, plan=plan, f=f, x1n=x1n,
+                x2n=x2n)
+
+
+def bench_grad(args, dev):
+    import miscellaneous_amd as m
+
+    N, M = 256, 262144
+    gen = torch.Generator().manual_seed(SEED + 1)
+    x1 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    x2 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    f_true = cn01(gen, (N, N), dev)
+    plan = m.Plan(N, N, torch.complex64, device=dev)
+    plan.setpts(x1, x2)
+    y_obs = plan.type2(f_true) + cn01(gen, (M,), dev) * np.sqrt(1e-3)
+    f = torch.zeros(N, N, dtype=torch.complex64, device=dev)
+    y = torch.empty(M, dtype=torch.complex64, device=dev)
+    gf = torch.empty(N, N, dtype=torch.complex64, device=dev)
+    lr = 1.0 / (2 * M)
+
+    def loss_now() -> float:                       # eager, outside the graph
+        return float(((plan.type2(f) - y_obs).abs() ** 2).sum().item())
+
+    def step():
+        plan.type2(f, out=y)
+        r = y - y_obs
+        g = 2 * r                                  # dL/d(conj y), times 2
+        plan.adjoint(g, out=gf)                    # dL/df
+        plan.type2_grad_x(g, f)                    # dL/dx (derivative-window gather)
+        f.sub_(lr * gf)
+
+    g_step = capture(step)
+    f.zero_()
+    torch.cuda.synchronize()
+    loss0 = loss_now()
+    n = max(args.steps, 10)
+    s = time_graph(g_step, n, 2)
+    total_steps = n + 2
+    while total_steps < 200:
+        g_step.replay()
+        total_steps += 1
+    torch.cuda.synchronize()
+    return dict(N=N, M=M, s=s, loss0=loss0, loss_final=loss_now(), steps=total_steps,
+                bytes=408 * N * N + 80 * M)
+
+
+def cpu_baseline(t2, seconds: float):
+    import oracle
+
+    f = t2["f"].cpu().numpy()
+    x1, x2 = t2["x1n"], t2["x2n"]
+    oracle.nfft1_2d(f, x1[:1000], x2[:1000], precision="float")    # warm-up / load
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.nfft1_2d(f, x1, x2, precision="float")
+        reps += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and reps >= 2) or reps >= 50:
+            break
+    return dict(value=len(x1) * reps / el / 1e6, unit="Mpts/s", cores=oracle.num_threads(), kind="port",
+                sample=f"{reps} full type-2 transforms (512x512, M=1048576 polar, FP32) of the CPU restatement "
+                       f"of BP_FBP_FFBP/NFFT1_2D.m, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "gather_polar512_traffic.json"),
+                    help="JSON with rocprofv3-measured HBM bytes per gather launch")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    t2 = bench_type2(args, dev)
+    gr = None if args.no_grad else bench_grad(args, dev)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
+
+    M = t2["M"]
+    achieved = t2["gather_bytes"] / (t2["gather_ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    out = {
+        "metric": "2D type-2 NFFT Mpts/s at rel-L2<=1e-5 (512^2, M=1M polar)",
+        "value": M / t2["s_full"] / 1e6,
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t2["s_full"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded CN(0,1) modes, polar point grid)",
+        "config": {"workload": "type-2 NFFT, 512x512 modes, M=1048576 polar points (512 angles x 2048 radii), "
+                               "K=6, c=2, complex64; step = setpts + type-2",
+                   "N1": 512, "N2": 512, "M": M, "K": 6, "c": 2},
+        "rel_l2": t2["rel_l2"],
+        "exec_only": {"value": M / t2["s_exec"] / 1e6, "unit": "Mpts/s", "ms_per_step": t2["s_exec"] * 1e3},
+        "stage_ms": {"setpts": t2["setpts_ms"], "grid_a_b": t2["grid_ms"], "gather_c": t2["gather_ms"]},
+        "type2_hbm_frac": t2["type2_bytes"] / t2["s_exec"] / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"kernel": "k_gather<float,0> (stage c)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": t2["gather_bytes"]},
+    }
+    if gr is not None:
+        out["grad"] = {"steps_per_s": 1.0 / gr["s"], "ms_per_step": gr["s"] * 1e3,
+                       "config": "256x256 modes, M=262144 uniform, loss ||A(x)f - y||^2, grad f and x, GD update",
+                       "hbm_frac": gr["bytes"] / gr["s"] / (HBM_PEAK_GBS * 1e9),
+                       "loss_first": gr["loss0"], "loss_after_steps": gr["loss_final"], "gd_steps": gr["steps"]}
+    out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

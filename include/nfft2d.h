@@ -125,6 +125,22 @@ int nfft2d_type2_dx(nfft2d_plan *plan, void *y, void *dy1, void *dy2, const void
 int nfft2d_type2_grad_x(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, const void *data,
                         void *stream);
 
+/* The stages of the transforms, for callers that time them or reuse the
+ * oversampled grid (which lives in the plan between the calls):
+ *   type-2  = nfft2d_type2_grid + nfft2d_type2_interp
+ *   type-1  = nfft2d_type1_spread + nfft2d_type1_finish(sign = -1)
+ *   adjoint = nfft2d_type1_spread + nfft2d_type1_finish(sign = +1)
+ * _grid: window pre-compensation, zero padding and the 2-D FFT
+ *        (BP_FBP_FFBP/NFFT1_2D.m:48-55);
+ * _interp: the (2K+1)^2 gather at the plan's points (NFFT1_2D.m:61-67);
+ * _spread: zero the grid and spread the M samples onto it
+ *          (NFFT/NFFT2_2D.cu:163-218);
+ * _finish: 2-D FFT, crop and deconvolution (NFFT/NFFT2_2D.cu:380-402). */
+int nfft2d_type2_grid(nfft2d_plan *plan, const void *data, void *stream);
+int nfft2d_type2_interp(nfft2d_plan *plan, void *result, void *stream);
+int nfft2d_type1_spread(nfft2d_plan *plan, const void *data, void *stream);
+int nfft2d_type1_finish(nfft2d_plan *plan, void *result, int sign, void *stream);
+
 /* Points per subproblem of the spread / gather (default 1024). */
 int nfft2d_plan_set_maxpts(nfft2d_plan *plan, int maxpts);
 /* Number of points set on the plan, or < 0 on error. */

@@ -52,6 +52,10 @@ _SIGS = {
     "nfft2d_adjoint": (_i, [_vp, _vp, _vp, _vp]),
     "nfft2d_type2_dx": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "nfft2d_type2_grad_x": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "nfft2d_type2_grid": (_i, [_vp, _vp, _vp]),
+    "nfft2d_type2_interp": (_i, [_vp, _vp, _vp]),
+    "nfft2d_type1_spread": (_i, [_vp, _vp, _vp]),
+    "nfft2d_type1_finish": (_i, [_vp, _vp, _i, _vp]),
     "nfft2d_plan_set_maxpts": (_i, [_vp, _i]),
     "nfft2d_plan_num_points": (_i, [_vp]),
 }

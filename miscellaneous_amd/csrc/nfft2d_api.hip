@@ -135,16 +135,24 @@ static void make_window_args(WinArgs &wa)
 // ============================================================== plan
 struct nfft2d_plan_s {
     int N1 = 0, N2 = 0, L1 = 0, L2 = 0, dtype = 0, device = 0;
-    int TS = 0, tiles1 = 0, tiles2 = 0, ntiles = 0, maxpts = 1024;
+    int nb1 = 0, nb2 = 0, nbins = 0, maxpts = kGatherThreads;
     size_t esz = 0;                       // sizeof(complex)
     void *tw1 = nullptr, *tw2 = nullptr;  // 2N1 / 2N2 twiddles e^{-2 pi i q/(2N)}
     void *D1 = nullptr, *D2 = nullptr;    // deconvolution S / phi(xi_k)
     void *grid = nullptr;                 // L2 x L1, transposed oversampled grid
     void *tmp = nullptr;                  // N1 x L2 intermediate
-    int M = -1, Mcap = 0, subcap = 0;
+    int M = -1, Mcap = 0, subcap = 0, histcap = 0;
+    int nblk = 0, chunk = 0;              // setpts blocking of the last point set
     void *xy = nullptr;
-    int *idx = nullptr, *rank = nullptr, *counts = nullptr, *offsets = nullptr, *nsub = nullptr;
+    int *idx = nullptr, *rank = nullptr, *hist = nullptr, *tot = nullptr, *binstart = nullptr, *nsub = nullptr;
     int4 *sub = nullptr;
+    // MFMA spread work units (built lazily after setpts)
+    int4 *units = nullptr, *merges = nullptr;
+    int *nunits = nullptr, *nmerge = nullptr;
+    void *scratch = nullptr;
+    int unitcap = 0;
+    bool units_valid = false;
+    BinGeom geo;
     WinArgs wa;
 };
 
@@ -165,6 +173,22 @@ static void free_points(nfft2d_plan *p)
     p->subcap = 0;
 }
 
+static void free_units(nfft2d_plan *p)
+{
+    hfree(p->units); hfree(p->scratch);
+    p->units = nullptr;
+    p->scratch = nullptr;
+    p->unitcap = 0;
+    p->units_valid = false;
+}
+
+static void free_hist(nfft2d_plan *p)
+{
+    hfree(p->hist);
+    p->hist = nullptr;
+    p->histcap = 0;
+}
+
 static void free_plan(nfft2d_plan *p)
 {
     if (!p) return;
@@ -173,8 +197,11 @@ static void free_plan(nfft2d_plan *p)
     (void)hipSetDevice(p->device);
     hfree(p->tw1); hfree(p->tw2); hfree(p->D1); hfree(p->D2);
     hfree(p->grid); hfree(p->tmp);
-    hfree(p->counts); hfree(p->offsets); hfree(p->nsub);
+    hfree(p->tot); hfree(p->binstart); hfree(p->nsub);
     free_points(p);
+    free_hist(p);
+    free_units(p);
+    hfree(p->merges); hfree(p->nunits); hfree(p->nmerge);
     (void)hipSetDevice(cur);
     delete p;
 }
@@ -225,18 +252,25 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
     p->N1 = N1; p->N2 = N2; p->L1 = kC * N1; p->L2 = kC * N2; p->dtype = dtype;
     p->esz = dtype == NFFT2D_C64 ? sizeof(float2) : sizeof(double2);
     NFFT2D_HIP_TRY(hipGetDevice(&p->device));
-    p->TS = dtype == NFFT2D_C64 ? tile_size<float>() : tile_size<double>();
-    p->tiles1 = (p->L1 + p->TS - 1) / p->TS;
-    p->tiles2 = (p->L2 + p->TS - 1) / p->TS;
-    p->ntiles = p->tiles1 * p->tiles2;
+    p->nb1 = (p->L1 + kBin - 1) / kBin;
+    p->nb2 = (p->L2 + kBin - 1) / kBin;
+    p->nbins = p->nb1 * p->nb2;
+    p->geo.L1 = p->L1;
+    p->geo.L2 = p->L2;
+    p->geo.nb1 = p->nb1;
+    p->geo.inv1d = 1.0 / p->L1;
+    p->geo.inv2d = 1.0 / p->L2;
+    p->geo.inv1f = (float)p->geo.inv1d;
+    p->geo.inv2f = (float)p->geo.inv2d;
     make_window_args(p->wa);
     int rc = dtype == NFFT2D_C64 ? upload_tables<float>(p) : upload_tables<double>(p);
     if (rc) { free_plan(p); return rc; }
     if (hipMalloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
-        hipMalloc(&p->counts, sizeof(int) * p->ntiles) != hipSuccess ||
-        hipMalloc(&p->offsets, sizeof(int) * p->ntiles) != hipSuccess ||
-        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess) {
+        hipMalloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||
+        hipMalloc(&p->binstart, sizeof(int) * p->nbins) != hipSuccess ||
+        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess || hipMalloc(&p->merges, sizeof(int4) * p->nbins) != hipSuccess ||
+        hipMalloc(&p->nunits, sizeof(int)) != hipSuccess || hipMalloc(&p->nmerge, sizeof(int)) != hipSuccess) {
         free_plan(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the grid failed");
     }
@@ -253,7 +287,8 @@ extern "C" int nfft2d_plan_destroy(nfft2d_plan *p)
 
 extern "C" int nfft2d_plan_set_maxpts(nfft2d_plan *p, int maxpts)
 {
-    if (!p || maxpts < 32 || maxpts > (1 << 20)) return nfft2d_fail(NFFT2D_ERR_ARG, "set_maxpts: bad argument");
+    if (!p || maxpts < 32 || maxpts > kGatherThreads)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "set_maxpts: maxpts must be in [32, %d]", kGatherThreads);
     p->maxpts = maxpts;
     p->M = -1;   // subproblems must be rebuilt
     return NFFT2D_OK;
@@ -276,7 +311,7 @@ static int zero_fill(void *ptr, long long n, hipStream_t s)
 template <typename T>
 static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_t s)
 {
-    const int subcap = p->ntiles + (M + p->maxpts - 1) / p->maxpts;
+    const int subcap = p->nbins + (M + p->maxpts - 1) / p->maxpts;
     if (M > p->Mcap || subcap > p->subcap) {
         free_points(p);
         const int cap = std::max(M, 1);
@@ -290,22 +325,46 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
         p->Mcap = cap;
         p->subcap = std::max(subcap, 1);
     }
-    int rc = zero_fill<int>(p->counts, p->ntiles, s);
-    if (rc) return rc;
-    if (M > 0) {
-        hipLaunchKernelGGL(k_bin_count<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, p->L1, p->L2, p->TS,
-                           p->tiles1, p->counts, p->rank);
-        NFFT2D_LAUNCH_CHECK();
+    // blocking of pass 1: LDS histograms for up to kMaxLdsBins bins, else one
+    // logical block with global atomics
+    const bool lds = p->nbins <= kMaxLdsBins;
+    const int nblk = lds ? (int)std::max<long long>(1, std::min<long long>(256, ((long long)M + 2047) / 2048)) : 1;
+    const int chunk = std::max(1, (M + nblk - 1) / nblk);
+    const long long hneed = (long long)nblk * p->nbins;
+    if (hneed > p->histcap) {
+        free_hist(p);
+        if (hipMalloc(&p->hist, sizeof(int) * hneed) != hipSuccess) {
+            p->M = -1;
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "setpts: hipMalloc of the bin histogram failed");
+        }
+        p->histcap = (int)hneed;
     }
-    hipLaunchKernelGGL(k_scan_subprob, dim3(1), dim3(kScanThreads), 0, s, p->counts, p->ntiles, p->maxpts, p->offsets,
+    const T inv1 = (T)(1.0 / p->L1), inv2 = (T)(1.0 / p->L2);
+    if (lds) {
+        hipLaunchKernelGGL(k_hist<T>, dim3(nblk), dim3(256), sizeof(int) * p->nbins, s, x, y, M, chunk, p->L1, p->L2,
+                           inv1, inv2, p->nb1, p->nbins, p->hist, p->rank);
+    } else {
+        int rc = zero_fill<int>(p->hist, p->nbins, s);
+        if (rc) return rc;
+        if (M > 0)
+            hipLaunchKernelGGL(k_hist_global<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, p->L1, p->L2, inv1,
+                               inv2, p->nb1, p->hist, p->rank);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colscan, dim3(grid1d(p->nbins, 32)), dim3(256), 0, s, p->hist, nblk, p->nbins, p->tot);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, s, p->tot, p->nbins, p->maxpts, p->binstart,
                        p->sub, p->nsub);
     NFFT2D_LAUNCH_CHECK();
     if (M > 0) {
-        hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, p->L1, p->L2, p->TS,
-                           p->tiles1, p->rank, p->offsets, (cplx<T> *)p->xy, p->idx);
+        hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, chunk, p->L1, p->L2, inv1,
+                           inv2, p->nb1, p->nbins, p->hist, p->binstart, p->rank, (cplx<T> *)p->xy, p->idx);
         NFFT2D_LAUNCH_CHECK();
     }
+    p->nblk = nblk;
+    p->chunk = chunk;
     p->M = M;
+    p->units_valid = false;
     return NFFT2D_OK;
 }
 
@@ -426,25 +485,65 @@ static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, 
 {
     using C = cplx<T>;
     if (p->M == 0) return NFFT2D_OK;
-    hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(256), 0, s, (const C *)p->grid, p->L1, p->L2,
-                       (const C *)p->xy, p->idx, p->sub, p->nsub, p->tiles1, p->wa, (C *)out, (C *)d1, (C *)d2,
-                       (const C *)g, (T *)gx1, (T *)gx2);
+    hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(kGatherThreads), 0, s, (const C *)p->grid, p->geo,
+                       (const C *)p->xy, p->idx, p->sub, p->nsub, p->wa, (C *)out, (C *)d1, (C *)d2, (const C *)g,
+                       (T *)gx1, (T *)gx2);
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }
 
-template <typename T, int SIGN>
-static int type1_impl(nfft2d_plan *p, void *result, const void *data, hipStream_t s)
+static bool mfma_spread_ok(const nfft2d_plan *p)
+{
+    return p->dtype == NFFT2D_C64 && p->nb1 >= 3 && p->nb2 >= 3 && p->L1 >= 44 && p->L2 >= 44;
+}
+
+static int build_units(nfft2d_plan *p, hipStream_t s)
+{
+    const long long cap = (long long)p->nbins + (9LL * p->M + kSpreadCand - 1) / kSpreadCand;
+    if (cap > p->unitcap) {
+        free_units(p);
+        if (hipMalloc(&p->units, sizeof(int4) * cap) != hipSuccess ||
+            hipMalloc(&p->scratch, sizeof(float2) * 256 * cap) != hipSuccess) {
+            free_units(p);
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of the work units failed");
+        }
+        p->unitcap = (int)cap;
+    }
+    hipLaunchKernelGGL(k_spread_units, dim3(1), dim3(kScanThreads), 0, s, p->tot, p->nb1, p->nb2, p->units, p->nunits,
+                       p->merges, p->nmerge);
+    NFFT2D_LAUNCH_CHECK();
+    p->units_valid = true;
+    return NFFT2D_OK;
+}
+
+template <typename T>
+static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
 {
     using C = cplx<T>;
+    if constexpr (std::is_same<T, float>::value) {
+        if (mfma_spread_ok(p)) {
+            if (!p->units_valid) {
+                int rc = build_units(p, s);
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_spread_mfma, dim3(p->unitcap), dim3(kSpreadWaves * 64), 0, s, (const float2 *)p->xy,
+                               p->idx, (const float2 *)data, p->binstart, p->tot, p->units, p->nunits, p->geo, p->nb2,
+                               p->wa, (float2 *)p->grid, (float2 *)p->scratch);
+            NFFT2D_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_spread_merge, dim3(p->nbins), dim3(256), 0, s, p->merges, p->nmerge,
+                               (const float2 *)p->scratch, p->geo, (float2 *)p->grid);
+            NFFT2D_LAUNCH_CHECK();
+            return NFFT2D_OK;
+        }
+    }
     int rc = zero_fill<T>(p->grid, 2LL * p->L1 * p->L2, s);
     if (rc) return rc;
     if (p->M > 0) {
-        hipLaunchKernelGGL((k_spread<T>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->L1, p->L2,
-                           (const C *)p->xy, p->idx, (const C *)data, p->sub, p->nsub, p->tiles1, p->wa);
+        hipLaunchKernelGGL((k_spread<T>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->geo,
+                           (const C *)p->xy, p->idx, (const C *)data, p->sub, p->nsub, p->wa);
         NFFT2D_LAUNCH_CHECK();
     }
-    return t1_passes<T, SIGN>(p, result, s);
+    return NFFT2D_OK;
 }
 
 static int check_exec(const nfft2d_plan *p, const void *a, const void *b)
@@ -455,17 +554,48 @@ static int check_exec(const nfft2d_plan *p, const void *a, const void *b)
     return NFFT2D_OK;
 }
 
+extern "C" int nfft2d_type2_grid(nfft2d_plan *p, const void *data, void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    if (!data) return nfft2d_fail(NFFT2D_ERR_ARG, "null data pointer");
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? t2_grid<float>(p, data, s) : t2_grid<double>(p, data, s);
+}
+
+extern "C" int nfft2d_type2_interp(nfft2d_plan *p, void *result, void *stream)
+{
+    int rc = check_exec(p, result, result);
+    if (rc || p->M == 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? gather<float, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s)
+                                  : gather<double, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+}
+
+extern "C" int nfft2d_type1_spread(nfft2d_plan *p, const void *data, void *stream)
+{
+    int rc = check_exec(p, data, data);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? spread_impl<float>(p, data, s) : spread_impl<double>(p, data, s);
+}
+
+extern "C" int nfft2d_type1_finish(nfft2d_plan *p, void *result, int sign, void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
+    if (sign != -1 && sign != 1) return nfft2d_fail(NFFT2D_ERR_ARG, "sign must be -1 or +1");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64)
+        return sign < 0 ? t1_passes<float, -1>(p, result, s) : t1_passes<float, +1>(p, result, s);
+    return sign < 0 ? t1_passes<double, -1>(p, result, s) : t1_passes<double, +1>(p, result, s);
+}
+
 extern "C" int nfft2d_type2(nfft2d_plan *p, void *result, const void *data, void *stream)
 {
     int rc = check_exec(p, result, data);
     if (rc || p->M == 0) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    if (p->dtype == NFFT2D_C64) {
-        if ((rc = t2_grid<float>(p, data, s))) return rc;
-        return gather<float, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s);
-    }
-    if ((rc = t2_grid<double>(p, data, s))) return rc;
-    return gather<double, 0>(p, result, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+    if ((rc = nfft2d_type2_grid(p, data, stream))) return rc;
+    return nfft2d_type2_interp(p, result, stream);
 }
 
 extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, const void *data, void *stream)
@@ -496,24 +626,23 @@ extern "C" int nfft2d_type2_grad_x(nfft2d_plan *p, void *gx1, void *gx2, const v
     return gather<double, 2>(p, nullptr, nullptr, nullptr, g, gx1, gx2, s);
 }
 
-extern "C" int nfft2d_type1(nfft2d_plan *p, void *result, const void *data, void *stream)
+static int type1_common(nfft2d_plan *p, void *result, const void *data, int sign, void *stream)
 {
-    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
     int rc = check_exec(p, data, data);
     if (rc) return rc;
     if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
-    hipStream_t s = (hipStream_t)stream;
-    return p->dtype == NFFT2D_C64 ? type1_impl<float, -1>(p, result, data, s) : type1_impl<double, -1>(p, result, data, s);
+    if ((rc = nfft2d_type1_spread(p, data, stream))) return rc;
+    return nfft2d_type1_finish(p, result, sign, stream);
+}
+
+extern "C" int nfft2d_type1(nfft2d_plan *p, void *result, const void *data, void *stream)
+{
+    return type1_common(p, result, data, -1, stream);
 }
 
 extern "C" int nfft2d_adjoint(nfft2d_plan *p, void *result, const void *data, void *stream)
 {
-    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
-    int rc = check_exec(p, data, data);
-    if (rc) return rc;
-    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
-    hipStream_t s = (hipStream_t)stream;
-    return p->dtype == NFFT2D_C64 ? type1_impl<float, +1>(p, result, data, s) : type1_impl<double, +1>(p, result, data, s);
+    return type1_common(p, result, data, +1, stream);
 }
 
 // ============================================================== drop-in entry points

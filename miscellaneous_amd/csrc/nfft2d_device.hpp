@@ -116,11 +116,13 @@ __device__ __forceinline__ double window_exact_deriv(double d, const WinExact& w
 
 // ---------------------------------------------------------------- point -> (mu, t)
 // mu = rint(c x) (half to even, NFFT/NFFT2_2D.cu:168), t = c x - mu in
-// [-1/2, 1/2] exactly, and mu wrapped into [0, L).  Non-finite or huge
-// coordinates map to cell 0 with t = NaN so the result is NaN and every index
-// stays in bounds.
+// [-1/2, 1/2] exactly, and mu wrapped into [0, L) (the reference's positive
+// `modulo`, :124-125).  The wrap uses a floating-point quotient estimate and an
+// exact integer fix-up (no integer division).  Non-finite or huge coordinates
+// map to cell 0 with t = NaN so the result is NaN and every index stays in
+// bounds.
 template <typename T>
-__device__ __forceinline__ void locate(T x, int L, int& mu_wrapped, T& t)
+__device__ __forceinline__ void locate(T x, int L, T invL, int& mu_wrapped, T& t)
 {
     T cx = (T)kC * x;
     if (!(fabs(cx) < (T)1073741824)) {   // also catches NaN
@@ -128,11 +130,13 @@ __device__ __forceinline__ void locate(T x, int L, int& mu_wrapped, T& t)
         t = (T)NAN;
         return;
     }
-    T mu = rint(cx);
+    const T mu = rint(cx);
     t = cx - mu;
-    long long m = (long long)mu % L;
-    if (m < 0) m += L;
-    mu_wrapped = (int)m;
+    const int mi = (int)mu;
+    int r = mi - (int)floor(mu * invL) * L;
+    while (r < 0) r += L;      // the estimate is exact or one off except for |c x| >~ 2^24
+    while (r >= L) r -= L;
+    mu_wrapped = r;
 }
 
 // ---------------------------------------------------------------- butterflies

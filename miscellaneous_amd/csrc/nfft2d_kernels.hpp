@@ -318,65 +318,137 @@ __global__ __launch_bounds__(256) void k_t1_rows_direct(
     }
 }
 
-// ================================================================= setpts (tile sort)
-// Bin = TSxTS tile of the oversampled grid that holds mu = rint(c x) mod L.
+// ================================================================= setpts (bin sort)
+// Points are counting-sorted by bin: a bin is a kBin x kBin block of cells of
+// the oversampled grid, holding mu = rint(c x) mod L.  Bins are the tiles of the
+// gather and the output tiles of the spread.
+constexpr int kBin = 16;
+constexpr int kMaxLdsBins = 16384;      // per-block LDS histogram limit (64 KiB)
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_count(const T* __restrict__ x, const T* __restrict__ y, int M,
-                                                   int L1, int L2, int TS, int tiles1, int* __restrict__ counts,
-                                                   int* __restrict__ rank)
+__device__ __forceinline__ int bin_of(T x, T y, int L1, int L2, T inv1, T inv2, int nb1, int& m1, int& m2, T& t1,
+                                      T& t2)
+{
+    locate(x, L1, inv1, m1, t1);
+    locate(y, L2, inv2, m2, t2);
+    return (m2 / kBin) * nb1 + m1 / kBin;
+}
+
+// Pass 1: block b counts the bins of points [b*chunk, (b+1)*chunk) in an LDS
+// histogram (integer LDS atomics); rank[i] is the point's rank inside its
+// (block, bin).  hist is [nblk][nbins].
+template <typename T>
+__global__ __launch_bounds__(256) void k_hist(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+                                              int L1, int L2, T inv1, T inv2, int nb1, int nbins,
+                                              int* __restrict__ hist, int* __restrict__ rank)
+{
+    extern __shared__ int h[];
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int lo = blockIdx.x * chunk, hi = min(lo + chunk, M);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        int m1, m2;
+        T t1, t2;
+        const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+        rank[i] = atomicAdd(&h[bin], 1);
+    }
+    __syncthreads();
+    int* out = hist + (size_t)blockIdx.x * nbins;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) out[i] = h[i];
+}
+
+// Pass 1 for grids with more than kMaxLdsBins bins: one logical block, global
+// integer atomics.
+template <typename T>
+__global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ x, const T* __restrict__ y, int M, int L1,
+                                                     int L2, T inv1, T inv2, int nb1, int* __restrict__ hist,
+                                                     int* __restrict__ rank)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     int m1, m2;
     T t1, t2;
-    locate(x[i], L1, m1, t1);
-    locate(y[i], L2, m2, t2);
-    const int bin = (m2 / TS) * tiles1 + m1 / TS;
-    rank[i] = atomicAdd(&counts[bin], 1);
+    const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+    rank[i] = atomicAdd(&hist[bin], 1);
 }
 
-// One block: exclusive scan of the tile counts and of the per-tile subproblem
-// counts ceil(count / maxpts); writes the subproblem list (tile, start, count).
-constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan_subprob(const int* __restrict__ counts, int ntiles,
-                                                               int maxpts, int* __restrict__ offsets,
-                                                               int4* __restrict__ sub, int* __restrict__ nsub)
+// Pass 2: per bin, turn hist[blk][bin] into the offset of (blk, bin) inside
+// the bin and write the bin total.  A block covers 32 bins x 8 slices of the
+// block range, so the loads of a slice are independent and coalesced.
+__global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int* __restrict__ tot)
 {
-    __shared__ int s_pts[kScanThreads];
-    __shared__ int s_sub[kScanThreads];
-    const int t = threadIdx.x;
-    const int per = (ntiles + kScanThreads - 1) / kScanThreads;
-    const int lo = min(t * per, ntiles), hi = min(lo + per, ntiles);
+    __shared__ int part[8][33];
+    const int bl = threadIdx.x % 32, q = threadIdx.x / 32;
+    const int bin = blockIdx.x * 32 + bl;
+    const int per = (nblk + 7) / 8;
+    const int lo = min(q * per, nblk), hi = min(lo + per, nblk);
+    int sum = 0;
+    if (bin < nbins)
+        for (int k = lo; k < hi; ++k) sum += hist[(size_t)k * nbins + bin];
+    part[q][bl] = sum;
+    __syncthreads();
+    if (q == 0) {
+        int run = 0;
+        for (int j = 0; j < 8; ++j) {
+            const int v = part[j][bl];
+            part[j][bl] = run;
+            run += v;
+        }
+        if (bin < nbins) tot[bin] = run;
+    }
+    __syncthreads();
+    int run = part[q][bl];
+    if (bin < nbins)
+        for (int k = lo; k < hi; ++k) {
+            const int v = hist[(size_t)k * nbins + bin];
+            hist[(size_t)k * nbins + bin] = run;
+            run += v;
+        }
+}
+
+// Pass 3 (one block): exclusive scan of the bin totals -> binstart, and the
+// subproblem list (bin, start, count <= maxpts) in bin order.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_scan_bins(const int* __restrict__ tot, int nbins, int maxpts,
+                                                            int* __restrict__ binstart, int4* __restrict__ sub,
+                                                            int* __restrict__ nsub)
+{
+    __shared__ int w_pts[kScanThreads / 64], w_sub[kScanThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (nbins + kScanThreads - 1) / kScanThreads;
+    const int lo = min(t * per, nbins), hi = min(lo + per, nbins);
     int a = 0, b = 0;
     for (int i = lo; i < hi; ++i) {
-        a += counts[i];
-        b += (counts[i] + maxpts - 1) / maxpts;
+        const int c = tot[i];
+        a += c;
+        b += (c + maxpts - 1) / maxpts;
     }
-    s_pts[t] = a;
-    s_sub[t] = b;
+    int ia = a, ib = b;   // inclusive wave scans
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
+        if (lane >= o) { ia += na; ib += nb; }
+    }
+    if (lane == 63) { w_pts[w] = ia; w_sub[w] = ib; }
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {   // Hillis-Steele inclusive scan
-        int va = t >= off ? s_pts[t - off] : 0;
-        int vb = t >= off ? s_sub[t - off] : 0;
-        __syncthreads();
-        s_pts[t] += va;
-        s_sub[t] += vb;
-        __syncthreads();
-    }
-    int pos = s_pts[t] - a, sp = s_sub[t] - b;
+    int ba = 0, bb = 0;
+    for (int k = 0; k < w; ++k) { ba += w_pts[k]; bb += w_sub[k]; }
+    int pos = ba + ia - a, sp = bb + ib - b;
     for (int i = lo; i < hi; ++i) {
-        const int c = counts[i];
-        offsets[i] = pos;
-        for (int s = 0; s * maxpts < c; ++s) sub[sp++] = make_int4(i, pos + s * maxpts, min(maxpts, c - s * maxpts), 0);
+        const int c = tot[i];
+        binstart[i] = pos;
+        for (int k = 0; k * maxpts < c; ++k) sub[sp++] = make_int4(i, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
         pos += c;
     }
-    if (t == kScanThreads - 1) *nsub = s_sub[t];
+    if (t == kScanThreads - 1) *nsub = sp;
 }
 
+// Pass 5: scatter the points to their sorted positions.
 template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int L1,
-                                                 int L2, int TS, int tiles1, const int* __restrict__ rank,
-                                                 const int* __restrict__ offsets, cplx<T>* __restrict__ xy,
+__global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+                                                 int L1, int L2, T inv1, T inv2, int nb1, int nbins,
+                                                 const int* __restrict__ off, const int* __restrict__ binstart,
+                                                 const int* __restrict__ rank, cplx<T>* __restrict__ xy,
                                                  int* __restrict__ idx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,16 +456,13 @@ __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const 
     const T xi = x[i], yi = y[i];
     int m1, m2;
     T t1, t2;
-    locate(xi, L1, m1, t1);
-    locate(yi, L2, m2, t2);
-    const int bin = (m2 / TS) * tiles1 + m1 / TS;
-    const int p = offsets[bin] + rank[i];
+    const int bin = bin_of(xi, yi, L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+    const int p = binstart[bin] + off[(size_t)(i / chunk) * nbins + bin] + rank[i];
     xy[p] = mk(xi, yi);
     idx[p] = i;
 }
 
-// ================================================================= gather / spread
-// Window weights for one point: w1 along dim 1, w2 along dim 2.
+// ================================================================= window taps
 struct WinArgs {
     WinPoly val;    // phi_hat / S
     WinPoly der;    // phi_hat' / S
@@ -434,124 +503,190 @@ __device__ __forceinline__ void dtaps(double t, const WinArgs& wa, double (&w)[k
     for (int m = 0; m < kW; ++m) w[m] = window_exact_deriv(t - (double)(m - kK), wa.ex);
 }
 
-template <typename T> constexpr int tile_size() { return sizeof(T) == 4 ? 32 : 16; }
+// Geometry shared by the gather and spread kernels.
+struct BinGeom {
+    int L1, L2, nb1;
+    float inv1f, inv2f;
+    double inv1d, inv2d;
+};
+__device__ __forceinline__ float inv1(const BinGeom& g, float) { return g.inv1f; }
+__device__ __forceinline__ float inv2(const BinGeom& g, float) { return g.inv2f; }
+__device__ __forceinline__ double inv1(const BinGeom& g, double) { return g.inv1d; }
+__device__ __forceinline__ double inv2(const BinGeom& g, double) { return g.inv2d; }
 
-// Load the (TS+2K)^2 region of the transposed grid around tile (t1, t2) into LDS.
-template <typename C, int TS>
+// Load the (kBin+2K)^2 region of the transposed grid around bin (b1, b2) into LDS.
+// o1, o2 are in [-K, L); when L >= W one adjustment by L wraps every index.
+__device__ __forceinline__ int wrap1(int g, int L) { return g < 0 ? g + L : (g >= L ? g - L : g); }
+
+template <typename C>
 __device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
 {
-    constexpr int W = TS + 2 * kK;
-    for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
-        const int r = e / W, c = e - r * W;
-        tile[e] = Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
-    }
-}
-
-// (c) type-2 gather, MODE 0: out[l] = y_l;  MODE 1: out = y, out_d1 = dy/dx1,
-// out_d2 = dy/dx2;  MODE 2: gradient Re(conj(g_l) dy_l/dx) into gx1/gx2.
-// y_l = sum_{m2} w2[m2] sum_{m1} w1[m1] G[(mu2+m2-K) mod L2][(mu1+m1-K) mod L1]
-// (BP_FBP_FFBP/NFFT1_2D.m:61-67 on the unshifted grid).
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_gather(
-    const cplx<T>* __restrict__ Gt, int L1, int L2, const cplx<T>* __restrict__ xy,
-    const int* __restrict__ idx, const int4* __restrict__ sub, const int* __restrict__ nsub, int tiles1,
-    WinArgs wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1, cplx<T>* __restrict__ out_d2,
-    const cplx<T>* __restrict__ g, T* __restrict__ gx1, T* __restrict__ gx2)
-{
-    using C = cplx<T>;
-    constexpr int TS = tile_size<T>();
-    constexpr int W = TS + 2 * kK;
-    __shared__ C tile[W * W];
-    const int s = blockIdx.x;
-    if (s >= *nsub) return;
-    const int4 d = sub[s];
-    const int t1 = d.x % tiles1, t2 = d.x / tiles1;
-    load_tile<C, TS>(tile, Gt, L1, L2, t1 * TS - kK, t2 * TS - kK);
-    __syncthreads();
-    for (int p = d.y + threadIdx.x; p < d.y + d.z; p += blockDim.x) {
-        const C q = xy[p];
-        int m1, m2;
-        T f1, f2;
-        locate(q.x, L1, m1, f1);
-        locate(q.y, L2, m2, f2);
-        const int c0 = min(max(m1 - t1 * TS, 0), TS - 1);
-        const int r0 = min(max(m2 - t2 * TS, 0), TS - 1);
-        T w1[kW], w2[kW];
-        taps(f1, wa, w1);
-        taps(f2, wa, w2);
-        const C* base = tile + r0 * W + c0;
-        if constexpr (MODE == 0) {
-            T ax = 0, ay = 0;
-#pragma unroll
-            for (int b = 0; b < kW; ++b) {
-                T sx = 0, sy = 0;
-#pragma unroll
-                for (int a = 0; a < kW; ++a) {
-                    const C u = base[b * W + a];
-                    sx = fma(w1[a], u.x, sx);
-                    sy = fma(w1[a], u.y, sy);
-                }
-                ax = fma(w2[b], sx, ax);
-                ay = fma(w2[b], sy, ay);
-            }
-            out[idx[p]] = mk(ax, ay);
-        } else {
-            T v1[kW], v2[kW];
-            dtaps(f1, wa, v1);
-            dtaps(f2, wa, v2);
-            T ax = 0, ay = 0, bx = 0, by = 0, cx = 0, cy = 0;
-#pragma unroll
-            for (int b = 0; b < kW; ++b) {
-                T sx = 0, sy = 0, dx = 0, dy = 0;
-#pragma unroll
-                for (int a = 0; a < kW; ++a) {
-                    const C u = base[b * W + a];
-                    sx = fma(w1[a], u.x, sx);
-                    sy = fma(w1[a], u.y, sy);
-                    dx = fma(v1[a], u.x, dx);
-                    dy = fma(v1[a], u.y, dy);
-                }
-                ax = fma(w2[b], sx, ax);
-                ay = fma(w2[b], sy, ay);
-                bx = fma(w2[b], dx, bx);   // d/dx1
-                by = fma(w2[b], dy, by);
-                cx = fma(v2[b], sx, cx);   // d/dx2
-                cy = fma(v2[b], sy, cy);
-            }
-            const T cc = (T)kC;            // d(c x - mu)/dx = c
-            const int o = idx[p];
-            if constexpr (MODE == 1) {
-                out[o] = mk(ax, ay);
-                out_d1[o] = mk(cc * bx, cc * by);
-                out_d2[o] = mk(cc * cx, cc * cy);
-            } else {
-                const C gg = g[o];
-                gx1[o] = cc * (gg.x * bx + gg.y * by);
-                gx2[o] = cc * (gg.x * cx + gg.y * cy);
-            }
+    constexpr int W = kBin + 2 * kK;
+    if (L1 >= W && L2 >= W) {
+        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+            const int r = e / W, c = e - r * W;
+            tile[e] = Gt[(size_t)wrap1(o2 + r, L2) * L1 + wrap1(o1 + c, L1)];
+        }
+    } else {
+        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+            const int r = e / W, c = e - r * W;
+            tile[e] = Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
         }
     }
 }
 
-// (d) type-1 spread: each block owns one subproblem (<= maxpts points of one
-// tile), accumulates the tile + K-halo in LDS with LDS atomics, then adds it
-// into the grid (one global atomic per touched cell instead of the
-// reference's 2 (2K+1)^2 per point, NFFT/NFFT2_2D.cu:127-128).
-template <typename T>
-__global__ __launch_bounds__(256) void k_spread(
-    cplx<T>* __restrict__ Gt, int L1, int L2, const cplx<T>* __restrict__ xy, const int* __restrict__ idx,
-    const cplx<T>* __restrict__ data, const int4* __restrict__ sub, const int* __restrict__ nsub, int tiles1,
-    WinArgs wa)
+constexpr int kGatherThreads = 256;     // = points per gather subproblem
+
+// Exclusive scan of kGatherThreads ints held one per thread (block-wide).
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int n = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += n;
+    }
+    if (lane == 63) s_warp[w] = inc;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += s_warp[k];
+    __syncthreads();
+    return base + inc - v;
+}
+
+// (c) type-2 gather over one subproblem (<= 256 points of one bin), MODE 0:
+// out[l] = y_l;  MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;  MODE 2:
+// gradient Re(conj(g_l) dy_l/dx) into gx1/gx2.
+// y_l = sum_{m2} w2[m2] sum_{m1} w1[m1] G[(mu2+m2-K) mod L2][(mu1+m1-K) mod L1]
+// (BP_FBP_FFBP/NFFT1_2D.m:61-67 on the unshifted grid).  The block first
+// counting-sorts its points by cell in LDS so that neighbouring lanes read
+// neighbouring tile addresses.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kGatherThreads) void k_gather(
+    const cplx<T>* __restrict__ Gt, BinGeom geo, const cplx<T>* __restrict__ xy, const int* __restrict__ idx,
+    const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, cplx<T>* __restrict__ out,
+    cplx<T>* __restrict__ out_d1, cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g, T* __restrict__ gx1,
+    T* __restrict__ gx2)
 {
     using C = cplx<T>;
-    constexpr int TS = tile_size<T>();
-    constexpr int W = TS + 2 * kK;
+    constexpr int W = kBin + 2 * kK;
+    __shared__ C tile[W * W];
+    __shared__ int cnt[kBin * kBin];
+    __shared__ int s_warp[kGatherThreads / 64];
+    __shared__ T s_f1[kGatherThreads], s_f2[kGatherThreads];
+    __shared__ int s_cell[kGatherThreads], s_idx[kGatherThreads];
+    const int s = blockIdx.x;
+    if (s >= *nsub) return;
+    const int4 d = sub[s];
+    const int b1 = d.x % geo.nb1, b2 = d.x / geo.nb1;
+    const int O1 = b1 * kBin, O2 = b2 * kBin;
+    const int tid = threadIdx.x;
+    const bool have = tid < d.z;
+    int cell = 0, o = 0;
+    T f1 = 0, f2 = 0;
+    if (have) {
+        const C q = xy[d.y + tid];
+        o = idx[d.y + tid];
+        int m1, m2;
+        locate(q.x, geo.L1, inv1(geo, T()), m1, f1);
+        locate(q.y, geo.L2, inv2(geo, T()), m2, f2);
+        cell = min(max(m2 - O2, 0), kBin - 1) * kBin + min(max(m1 - O1, 0), kBin - 1);
+    }
+    cnt[tid] = 0;
+    load_tile(tile, Gt, geo.L1, geo.L2, O1 - kK, O2 - kK);
+    __syncthreads();
+    const int rk = have ? atomicAdd(&cnt[cell], 1) : 0;
+    __syncthreads();
+    const int start = block_exclusive_scan(cnt[tid], s_warp);
+    __syncthreads();
+    cnt[tid] = start;
+    __syncthreads();
+    if (have) {
+        const int dst = cnt[cell] + rk;
+        s_f1[dst] = f1;
+        s_f2[dst] = f2;
+        s_cell[dst] = cell;
+        s_idx[dst] = o;
+    }
+    __syncthreads();
+    if (!have) return;
+    f1 = s_f1[tid];
+    f2 = s_f2[tid];
+    cell = s_cell[tid];
+    o = s_idx[tid];
+    const int r0 = cell / kBin, c0 = cell % kBin;
+    T w1[kW], w2[kW];
+    taps(f1, wa, w1);
+    taps(f2, wa, w2);
+    const C* base = tile + r0 * W + c0;
+    if constexpr (MODE == 0) {
+        T ax = 0, ay = 0;
+#pragma unroll
+        for (int b = 0; b < kW; ++b) {
+            T sx = 0, sy = 0;
+#pragma unroll
+            for (int a = 0; a < kW; ++a) {
+                const C u = base[b * W + a];
+                sx = fma(w1[a], u.x, sx);
+                sy = fma(w1[a], u.y, sy);
+            }
+            ax = fma(w2[b], sx, ax);
+            ay = fma(w2[b], sy, ay);
+        }
+        out[o] = mk(ax, ay);
+    } else {
+        T v1[kW], v2[kW];
+        dtaps(f1, wa, v1);
+        dtaps(f2, wa, v2);
+        T ax = 0, ay = 0, bx = 0, by = 0, cx = 0, cy = 0;
+#pragma unroll
+        for (int b = 0; b < kW; ++b) {
+            T sx = 0, sy = 0, dx = 0, dy = 0;
+#pragma unroll
+            for (int a = 0; a < kW; ++a) {
+                const C u = base[b * W + a];
+                sx = fma(w1[a], u.x, sx);
+                sy = fma(w1[a], u.y, sy);
+                dx = fma(v1[a], u.x, dx);
+                dy = fma(v1[a], u.y, dy);
+            }
+            ax = fma(w2[b], sx, ax);
+            ay = fma(w2[b], sy, ay);
+            bx = fma(w2[b], dx, bx);   // d/dx1
+            by = fma(w2[b], dy, by);
+            cx = fma(v2[b], sx, cx);   // d/dx2
+            cy = fma(v2[b], sy, cy);
+        }
+        const T cc = (T)kC;            // d(c x - mu)/dx = c
+        if constexpr (MODE == 1) {
+            out[o] = mk(ax, ay);
+            out_d1[o] = mk(cc * bx, cc * by);
+            out_d2[o] = mk(cc * cx, cc * cy);
+        } else {
+            const C gg = g[o];
+            gx1[o] = cc * (gg.x * bx + gg.y * by);
+            gx2[o] = cc * (gg.x * cx + gg.y * cy);
+        }
+    }
+}
+
+// (d) type-1 spread (interim form): each block owns one subproblem, accumulates
+// the bin + K-halo in LDS, then adds it into the grid with global atomics.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeom geo, const cplx<T>* __restrict__ xy,
+                                                const int* __restrict__ idx, const cplx<T>* __restrict__ data,
+                                                const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa)
+{
+    using C = cplx<T>;
+    constexpr int W = kBin + 2 * kK;
     __shared__ T tre[W * W];
     __shared__ T tim[W * W];
     const int s = blockIdx.x;
     if (s >= *nsub) return;
     const int4 d = sub[s];
-    const int t1 = d.x % tiles1, t2 = d.x / tiles1;
+    const int b1 = d.x % geo.nb1, b2 = d.x / geo.nb1;
+    const int O1 = b1 * kBin, O2 = b2 * kBin;
     for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
         tre[e] = 0;
         tim[e] = 0;
@@ -562,10 +697,10 @@ __global__ __launch_bounds__(256) void k_spread(
         const C v = data[idx[p]];
         int m1, m2;
         T f1, f2;
-        locate(q.x, L1, m1, f1);
-        locate(q.y, L2, m2, f2);
-        const int c0 = min(max(m1 - t1 * TS, 0), TS - 1);
-        const int r0 = min(max(m2 - t2 * TS, 0), TS - 1);
+        locate(q.x, geo.L1, inv1(geo, T()), m1, f1);
+        locate(q.y, geo.L2, inv2(geo, T()), m2, f2);
+        const int c0 = min(max(m1 - O1, 0), kBin - 1);
+        const int r0 = min(max(m2 - O2, 0), kBin - 1);
         T w1[kW], w2[kW];
         taps(f1, wa, w1);
         taps(f2, wa, w2);
@@ -582,16 +717,255 @@ __global__ __launch_bounds__(256) void k_spread(
         }
     }
     __syncthreads();
-    const int o1 = t1 * TS - kK, o2 = t2 * TS - kK;
     for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
         const T a = tre[e], b = tim[e];
         if (a != (T)0 || b != (T)0) {
             const int r = e / W, c = e - r * W;
-            C* cell = &Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
-            unsafeAtomicAdd(&cell->x, a);
-            unsafeAtomicAdd(&cell->y, b);
+            C* cellp = &Gt[(size_t)wrap(O2 - kK + r, geo.L2) * geo.L1 + wrap(O1 - kK + c, geo.L1)];
+            unsafeAtomicAdd(&cellp->x, a);
+            unsafeAtomicAdd(&cellp->y, b);
         }
     }
+}
+
+// ================================================================= MFMA spread
+// (d) type-1 spread, output-stationary: one unit per 16x16 output bin (more
+// units for dense bins) computes
+//     G[O2+i][O1+j] = sum_p (d_p w2_p[i + O2 - mu2_p + K]) * w1_p[j + O1 - mu1_p + K]
+// over the points p of the 3x3 neighbouring bins whose footprint meets the
+// bin.  Each point is an outer product of two 16-vectors; v_mfma_f32_16x16x4f32
+// adds four of them per instruction into a 16x16 accumulator held in
+// registers.  No atomics: every cell is written once, by its bin's unit, or by
+// the merge kernel for bins split into several units.  Needs L >= 44 and at
+// least 3 bins per dimension (nfft2d_plan chooses the LDS-atomic k_spread
+// otherwise).
+constexpr int kSpreadWaves = 4;
+constexpr int kSpreadCand = 4096;      // candidates (3x3-bin points) per unit
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int wrap_signed(int v, int L)
+{
+    // v in (-L, L) -> representative in [-L/2, L/2)
+    if (v >= L / 2) v -= L;
+    if (v < -(L / 2)) v += L;
+    return v;
+}
+
+// One block: per output bin, candidates = sum of the 3x3 neighbour bin counts;
+// units = max(1, ceil(cand / kSpreadCand)); bins with several units get
+// partial slots and a merge entry.
+__global__ __launch_bounds__(kScanThreads) void k_spread_units(const int* __restrict__ tot, int nb1, int nb2,
+                                                               int4* __restrict__ units, int* __restrict__ nunits,
+                                                               int4* __restrict__ merges, int* __restrict__ nmerge)
+{
+    __shared__ int w_u[kScanThreads / 64], w_s[kScanThreads / 64], w_m[kScanThreads / 64];
+    const int nbins = nb1 * nb2;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (nbins + kScanThreads - 1) / kScanThreads;
+    const int lo = min(t * per, nbins), hi = min(lo + per, nbins);
+    auto cand_of = [&](int b) {
+        const int b1 = b % nb1, b2 = b / nb1;
+        int c = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int y2 = (b2 + dy + nb2) % nb2, x1 = (b1 + dx + nb1) % nb1;
+                c += tot[y2 * nb1 + x1];
+            }
+        return c;
+    };
+    int cu = 0, cs = 0, cm = 0;
+    for (int b = lo; b < hi; ++b) {
+        const int c = cand_of(b);
+        const int nu = max(1, (c + kSpreadCand - 1) / kSpreadCand);
+        cu += nu;
+        if (nu > 1) { cs += nu; cm += 1; }
+    }
+    int iu = cu, is = cs, im = cm;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int a = __shfl_up(iu, o, 64), b = __shfl_up(is, o, 64), c = __shfl_up(im, o, 64);
+        if (lane >= o) { iu += a; is += b; im += c; }
+    }
+    if (lane == 63) { w_u[w] = iu; w_s[w] = is; w_m[w] = im; }
+    __syncthreads();
+    int bu = 0, bs = 0, bm = 0;
+    for (int k = 0; k < w; ++k) { bu += w_u[k]; bs += w_s[k]; bm += w_m[k]; }
+    int uo = bu + iu - cu, so = bs + is - cs, mo = bm + im - cm;
+    for (int b = lo; b < hi; ++b) {
+        const int c = cand_of(b);
+        const int nu = max(1, (c + kSpreadCand - 1) / kSpreadCand);
+        for (int k = 0; k < nu; ++k)
+            units[uo + k] = make_int4(b, k * kSpreadCand, min(c, (k + 1) * kSpreadCand), nu > 1 ? so + k : -1);
+        if (nu > 1) {
+            merges[mo++] = make_int4(b, so, nu, 0);
+            so += nu;
+        }
+        uo += nu;
+    }
+    if (t == kScanThreads - 1) { *nunits = uo; *nmerge = mo; }
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kSpreadWaves * 64) void k_spread_mfma(
+    const float2* __restrict__ xy, const int* __restrict__ idx, const float2* __restrict__ data,
+    const int* __restrict__ binstart, const int* __restrict__ tot, const int4* __restrict__ units,
+    const int* __restrict__ nunits, BinGeom geo, int nb2, WinArgs wa, float2* __restrict__ Gt,
+    float2* __restrict__ scratch)
+{
+    __shared__ __attribute__((aligned(16))) float stage[kSpreadWaves][64][48];
+    __shared__ int list[kSpreadWaves][128];
+    const int u = blockIdx.x;
+    if (u >= *nunits) return;
+    const int4 U = units[u];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nb1 = geo.nb1;
+    const int b1 = U.x % nb1, b2 = U.x / nb1;
+    const int O1 = b1 * kBin, O2 = b2 * kBin;
+    int nstart[9], npref[10];
+    npref[0] = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int y2 = (b2 + j / 3 - 1 + nb2) % nb2, x1 = (b1 + j % 3 - 1 + nb1) % nb1;
+        const int nb = y2 * nb1 + x1;
+        nstart[j] = binstart[nb];
+        npref[j + 1] = npref[j] + tot[nb];
+    }
+    float* st = &stage[w][0][0];
+    int* lst = list[w];
+    f4v acc_re = {0.f, 0.f, 0.f, 0.f}, acc_im = {0.f, 0.f, 0.f, 0.f};
+
+    // stage k (<= 64) listed points: rows 0-15 d.x*w2, 16-31 d.y*w2, 32-47 w1 at the bin's offsets
+    auto process = [&](int k) {
+        float* row = st + lane * 48;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) *reinterpret_cast<float4*>(row + 4 * q) = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < k) {
+            const int p = lst[lane];
+            const float2 q = xy[p];
+            const float2 d = data[idx[p]];
+            int m1, m2;
+            float f1, f2;
+            locate(q.x, geo.L1, geo.inv1f, m1, f1);
+            locate(q.y, geo.L2, geo.inv2f, m2, f2);
+            const int dc = wrap_signed(m1 - O1, geo.L1), dr = wrap_signed(m2 - O2, geo.L2);
+            float w1[kW], w2[kW];
+            taps(f1, wa, w1);
+            taps(f2, wa, w2);
+#pragma unroll
+            for (int t = 0; t < kW; ++t) {
+                const int r = dr - kK + t;
+                if (r >= 0 && r < 16) {
+                    row[r] = d.x * w2[t];
+                    row[16 + r] = d.y * w2[t];
+                }
+                const int c = dc - kK + t;
+                if (c >= 0 && c < 16) row[32 + c] = w1[t];
+            }
+        }
+        wave_sync_lds();
+        const int i = lane & 15, kk = lane >> 4;
+        for (int s = 0; s < k; s += 4) {
+            const float* pr = st + (s + kk) * 48;
+            const float are = pr[i], aim = pr[16 + i], bb = pr[32 + i];
+            acc_re = __builtin_amdgcn_mfma_f32_16x16x4f32(are, bb, acc_re, 0, 0, 0);
+            acc_im = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, bb, acc_im, 0, 0, 0);
+        }
+        wave_sync_lds();
+    };
+
+    int n = 0;
+    for (int c = U.y + w * 64; c < U.z; c += kSpreadWaves * 64) {
+        const int cc = c + lane;
+        bool rel = false;
+        int p = 0;
+        if (cc < U.z) {
+            int j = 0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q) j += (npref[q] <= cc) ? 1 : 0;
+            p = nstart[j] + (cc - npref[j]);
+            const float2 q = xy[p];
+            int m1, m2;
+            float f1, f2;
+            locate(q.x, geo.L1, geo.inv1f, m1, f1);
+            locate(q.y, geo.L2, geo.inv2f, m2, f2);
+            const int dc = wrap_signed(m1 - O1, geo.L1), dr = wrap_signed(m2 - O2, geo.L2);
+            rel = dr >= -kK && dr < 16 + kK && dc >= -kK && dc < 16 + kK;
+        }
+        const unsigned long long mask = __ballot(rel);
+        if (rel) lst[n + __popcll(mask & ((1ull << lane) - 1ull))] = p;
+        n += __popcll(mask);
+        if (n >= 64) {
+            wave_sync_lds();
+            process(64);
+            n -= 64;
+            const int carry = lane < n ? lst[64 + lane] : 0;
+            wave_sync_lds();
+            if (lane < n) lst[lane] = carry;
+            wave_sync_lds();
+        }
+    }
+    if (n > 0) {
+        wave_sync_lds();
+        process(n);
+    }
+    // reduce the waves' accumulators in a fixed order
+    __syncthreads();
+    float* red = &stage[0][0][0];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        red[(w * 64 + lane) * 8 + r] = acc_re[r];
+        red[(w * 64 + lane) * 8 + 4 + r] = acc_im[r];
+    }
+    __syncthreads();
+    if (w == 0) {
+        float vr[4], vi[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            vr[r] = red[lane * 8 + r];
+            vi[r] = red[lane * 8 + 4 + r];
+        }
+        for (int ww = 1; ww < kSpreadWaves; ++ww)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                vr[r] += red[(ww * 64 + lane) * 8 + r];
+                vi[r] += red[(ww * 64 + lane) * 8 + 4 + r];
+            }
+        const int j = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 4 * (lane >> 4) + r;
+            if (U.w >= 0) {
+                scratch[(size_t)U.w * 256 + i * 16 + j] = make_float2(vr[r], vi[r]);
+            } else if (O2 + i < geo.L2 && O1 + j < geo.L1) {
+                Gt[(size_t)(O2 + i) * geo.L1 + (O1 + j)] = make_float2(vr[r], vi[r]);
+            }
+        }
+    }
+}
+
+// Sum the partial tiles of the bins split into several units, in unit order.
+__global__ __launch_bounds__(256) void k_spread_merge(const int4* __restrict__ merges, const int* __restrict__ nmerge,
+                                                      const float2* __restrict__ scratch, BinGeom geo,
+                                                      float2* __restrict__ Gt)
+{
+    const int m = blockIdx.x;
+    if (m >= *nmerge) return;
+    const int4 M4 = merges[m];
+    const int i = threadIdx.x / 16, j = threadIdx.x % 16;
+    const int O1 = (M4.x % geo.nb1) * kBin, O2 = (M4.x / geo.nb1) * kBin;
+    float sr = 0.f, si = 0.f;
+    for (int k = 0; k < M4.z; ++k) {
+        const float2 v = scratch[(size_t)(M4.y + k) * 256 + threadIdx.x];
+        sr += v.x;
+        si += v.y;
+    }
+    if (O2 + i < geo.L2 && O1 + j < geo.L1) Gt[(size_t)(O2 + i) * geo.L1 + (O1 + j)] = make_float2(sr, si);
 }
 
 }  // namespace nfft2d

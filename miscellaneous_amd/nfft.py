@@ -72,10 +72,10 @@ class Plan:
         if dtype not in _CODE:
             raise TypeError("dtype must be torch.complex64 or torch.complex128")
         self.N1, self.N2, self.dtype, self.rdtype = int(N1), int(N2), dtype, _REAL[dtype]
-        self.device = torch.device(device if device is not None else "cuda", torch.cuda.current_device()
-                                   if device is None else None) if not isinstance(device, torch.device) else device
-        if self.device.index is None:
-            self.device = torch.device("cuda", torch.cuda.current_device())
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        if dev.type != "cuda":
+            raise ValueError(f"Plan needs a ROCm device, got {dev}")
+        self.device = dev if dev.index is not None else torch.device("cuda", torch.cuda.current_device())
         self._lib = _lib.load_library()
         self._h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -155,6 +155,36 @@ class Plan:
         _require(out, self.dtype, self.N1 * self.N2, "out", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_adjoint(self._h, _ptr(out), _ptr(g), _stream(self.device)), "nfft2d_adjoint")
+        return out
+
+    # -------------------------------------------------------------- stages
+    def type2_grid(self, f: torch.Tensor) -> None:
+        """Stages (a)+(b): pre-compensate, zero-pad and FFT f into the plan's grid."""
+        _require(f, self.dtype, self.N1 * self.N2, "f", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type2_grid(self._h, _ptr(f), _stream(self.device)), "nfft2d_type2_grid")
+
+    def type2_interp(self, out: torch.Tensor) -> torch.Tensor:
+        """Stage (c): gather the plan's grid at the plan's points into out (M)."""
+        self._need_points()
+        _require(out, self.dtype, self.M, "out", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type2_interp(self._h, _ptr(out), _stream(self.device)), "nfft2d_type2_interp")
+        return out
+
+    def type1_spread(self, d: torch.Tensor) -> None:
+        """Stage (d): zero the grid and spread the M samples d onto it."""
+        self._need_points()
+        _require(d, self.dtype, self.M, "d", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type1_spread(self._h, _ptr(d), _stream(self.device)), "nfft2d_type1_spread")
+
+    def type1_finish(self, out: torch.Tensor, sign: int = -1) -> torch.Tensor:
+        """FFT + crop + deconvolution of the plan's grid into out (N1 x N2); sign +1 = adjoint."""
+        _require(out, self.dtype, self.N1 * self.N2, "out", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type1_finish(self._h, _ptr(out), int(sign), _stream(self.device)),
+                  "nfft2d_type1_finish")
         return out
 
     def type2_dx(self, f: torch.Tensor):
