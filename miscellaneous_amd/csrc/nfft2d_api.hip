@@ -143,15 +143,12 @@ struct nfft2d_plan_s {
     void *tmp = nullptr;                  // N1 x L2 intermediate
     int M = -1, Mcap = 0, subcap = 0, histcap = 0;
     int nblk = 0, chunk = 0;              // setpts blocking of the last point set
-    void *xy = nullptr;
+    void *pts = nullptr;                  // sorted point records PtRec<T>
     int *idx = nullptr, *rank = nullptr, *hist = nullptr, *tot = nullptr, *binstart = nullptr, *nsub = nullptr;
     int4 *sub = nullptr;
-    // MFMA spread work units (built lazily after setpts)
-    int4 *units = nullptr, *merges = nullptr;
-    int *nunits = nullptr, *nmerge = nullptr;
-    void *scratch = nullptr;
-    int unitcap = 0;
-    bool units_valid = false;
+    int *subbeg = nullptr;                // first subproblem of each bin (nbins + 1)
+    void *scratch = nullptr;              // MFMA spread tiles, 28x28 per subproblem
+    int scratchcap = 0;
     BinGeom geo;
     WinArgs wa;
 };
@@ -167,19 +164,17 @@ static void hfree(void *ptr)
 
 static void free_points(nfft2d_plan *p)
 {
-    hfree(p->xy); hfree(p->idx); hfree(p->rank); hfree(p->sub);
-    p->xy = nullptr; p->idx = p->rank = nullptr; p->sub = nullptr;
+    hfree(p->pts); hfree(p->rank); hfree(p->sub);
+    p->pts = nullptr; p->rank = nullptr; p->sub = nullptr;
     p->Mcap = 0;
     p->subcap = 0;
 }
 
-static void free_units(nfft2d_plan *p)
+static void free_scratch(nfft2d_plan *p)
 {
-    hfree(p->units); hfree(p->scratch);
-    p->units = nullptr;
+    hfree(p->scratch);
     p->scratch = nullptr;
-    p->unitcap = 0;
-    p->units_valid = false;
+    p->scratchcap = 0;
 }
 
 static void free_hist(nfft2d_plan *p)
@@ -200,8 +195,8 @@ static void free_plan(nfft2d_plan *p)
     hfree(p->tot); hfree(p->binstart); hfree(p->nsub);
     free_points(p);
     free_hist(p);
-    free_units(p);
-    hfree(p->merges); hfree(p->nunits); hfree(p->nmerge);
+    free_scratch(p);
+    hfree(p->subbeg);
     (void)hipSetDevice(cur);
     delete p;
 }
@@ -269,8 +264,8 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
         hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||
         hipMalloc(&p->binstart, sizeof(int) * p->nbins) != hipSuccess ||
-        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess || hipMalloc(&p->merges, sizeof(int4) * p->nbins) != hipSuccess ||
-        hipMalloc(&p->nunits, sizeof(int)) != hipSuccess || hipMalloc(&p->nmerge, sizeof(int)) != hipSuccess) {
+        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess ||
+        hipMalloc(&p->subbeg, sizeof(int) * (p->nbins + 1)) != hipSuccess) {
         free_plan(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the grid failed");
     }
@@ -315,7 +310,7 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
     if (M > p->Mcap || subcap > p->subcap) {
         free_points(p);
         const int cap = std::max(M, 1);
-        if (hipMalloc(&p->xy, sizeof(cplx<T>) * cap) != hipSuccess || hipMalloc(&p->idx, sizeof(int) * cap) != hipSuccess ||
+        if (hipMalloc(&p->pts, sizeof(PtRec<T>) * cap) != hipSuccess ||
             hipMalloc(&p->rank, sizeof(int) * cap) != hipSuccess ||
             hipMalloc(&p->sub, sizeof(int4) * std::max(subcap, 1)) != hipSuccess) {
             free_points(p);
@@ -351,20 +346,19 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
                                inv2, p->nb1, p->hist, p->rank);
     }
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colscan, dim3(grid1d(p->nbins, 32)), dim3(256), 0, s, p->hist, nblk, p->nbins, p->tot);
+    hipLaunchKernelGGL(k_colscan, dim3(grid1d(p->nbins, kColBins)), dim3(256), 0, s, p->hist, nblk, p->nbins, p->tot);
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, s, p->tot, p->nbins, p->maxpts, p->binstart,
-                       p->sub, p->nsub);
+                       p->subbeg, p->sub, p->nsub);
     NFFT2D_LAUNCH_CHECK();
     if (M > 0) {
         hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, chunk, p->L1, p->L2, inv1,
-                           inv2, p->nb1, p->nbins, p->hist, p->binstart, p->rank, (cplx<T> *)p->xy, p->idx);
+                           inv2, p->nb1, p->nbins, p->hist, p->binstart, p->rank, (PtRec<T> *)p->pts);
         NFFT2D_LAUNCH_CHECK();
     }
     p->nblk = nblk;
     p->chunk = chunk;
     p->M = M;
-    p->units_valid = false;
     return NFFT2D_OK;
 }
 
@@ -486,7 +480,7 @@ static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, 
     using C = cplx<T>;
     if (p->M == 0) return NFFT2D_OK;
     hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(kGatherThreads), 0, s, (const C *)p->grid, p->geo,
-                       (const C *)p->xy, p->idx, p->sub, p->nsub, p->wa, (C *)out, (C *)d1, (C *)d2, (const C *)g,
+                       (const PtRec<T> *)p->pts, p->sub, p->nsub, p->wa, (C *)out, (C *)d1, (C *)d2, (const C *)g,
                        (T *)gx1, (T *)gx2);
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
@@ -497,41 +491,26 @@ static bool mfma_spread_ok(const nfft2d_plan *p)
     return p->dtype == NFFT2D_C64 && p->nb1 >= 3 && p->nb2 >= 3 && p->L1 >= 44 && p->L2 >= 44;
 }
 
-static int build_units(nfft2d_plan *p, hipStream_t s)
-{
-    const long long cap = (long long)p->nbins + (9LL * p->M + kSpreadCand - 1) / kSpreadCand;
-    if (cap > p->unitcap) {
-        free_units(p);
-        if (hipMalloc(&p->units, sizeof(int4) * cap) != hipSuccess ||
-            hipMalloc(&p->scratch, sizeof(float2) * 256 * cap) != hipSuccess) {
-            free_units(p);
-            return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of the work units failed");
-        }
-        p->unitcap = (int)cap;
-    }
-    hipLaunchKernelGGL(k_spread_units, dim3(1), dim3(kScanThreads), 0, s, p->tot, p->nb1, p->nb2, p->units, p->nunits,
-                       p->merges, p->nmerge);
-    NFFT2D_LAUNCH_CHECK();
-    p->units_valid = true;
-    return NFFT2D_OK;
-}
-
 template <typename T>
 static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
 {
     using C = cplx<T>;
     if constexpr (std::is_same<T, float>::value) {
         if (mfma_spread_ok(p)) {
-            if (!p->units_valid) {
-                int rc = build_units(p, s);
-                if (rc) return rc;
+            if (p->subcap > p->scratchcap) {
+                free_scratch(p);
+                if (hipMalloc(&p->scratch, sizeof(float2) * kTileCells * (size_t)p->subcap) != hipSuccess)
+                    return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of %d tiles failed", p->subcap);
+                p->scratchcap = p->subcap;
             }
-            hipLaunchKernelGGL(k_spread_mfma, dim3(p->unitcap), dim3(kSpreadWaves * 64), 0, s, (const float2 *)p->xy,
-                               p->idx, (const float2 *)data, p->binstart, p->tot, p->units, p->nunits, p->geo, p->nb2,
-                               p->wa, (float2 *)p->grid, (float2 *)p->scratch);
-            NFFT2D_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_spread_merge, dim3(p->nbins), dim3(256), 0, s, p->merges, p->nmerge,
-                               (const float2 *)p->scratch, p->geo, (float2 *)p->grid);
+            if (p->M > 0) {
+                hipLaunchKernelGGL(k_spread_halo, dim3(sub_blocks(p)), dim3(256), 0, s, (const PtRec<float> *)p->pts,
+                                   (const float2 *)data, p->sub, p->nsub, p->geo, p->wa,
+                                   (float2 *)p->scratch);
+                NFFT2D_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_spread_merge, dim3(p->nbins), dim3(256), 0, s, (const float2 *)p->scratch, p->subbeg,
+                               p->geo, p->nb2, (float2 *)p->grid);
             NFFT2D_LAUNCH_CHECK();
             return NFFT2D_OK;
         }
@@ -540,7 +519,7 @@ static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
     if (rc) return rc;
     if (p->M > 0) {
         hipLaunchKernelGGL((k_spread<T>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->geo,
-                           (const C *)p->xy, p->idx, (const C *)data, p->sub, p->nsub, p->wa);
+                           (const PtRec<T> *)p->pts, (const C *)data, p->sub, p->nsub, p->wa);
         NFFT2D_LAUNCH_CHECK();
     }
     return NFFT2D_OK;

@@ -48,6 +48,16 @@ __device__ __forceinline__ int wrap(int g, int L)
     return g < 0 ? g + L : g;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2.  Give every XCD a contiguous range of logical
+// blocks so that neighbouring column blocks, which share cache lines, meet in
+// the same L2.  Identity when the grid is not a multiple of 8.
+__device__ __forceinline__ int xcd_block(int b, int n)
+{
+    if (n & 7) return b;
+    return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // Row-group / column-block sizes of the FFT passes.
 template <int N> constexpr int rows_per_block() { return N >= 1024 ? 1 : 1024 / N; }
 template <int N> constexpr int cols_per_block() { return N >= 2048 ? 1 : 2048 / N; }
@@ -122,7 +132,8 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
     const int t = threadIdx.x;
     const int bF = t % B, jF = t / B;      // first stage: B columns side by side
     const int bL = t / TPL, jL = t % TPL;  // last stage: whole lines
-    const int n2F = min(blockIdx.x * B + bF, L2 - 1);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int n2F = min(blk * B + bF, L2 - 1);
     C v[V][8];
 #pragma unroll
     for (int b = 0; b < 8 / R0; ++b)
@@ -135,7 +146,7 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
             v[1][b * R0 + r] = cmul(a, twid<SIGN>(tw, k1 < 0 ? k1 + 2 * N : k1));
         }
     fft_lines<N, V, SIGN>(v, lds + bF * V * LL, jF, lds + bL * V * LL, jL, tw, 2 * N);
-    const int n2L = blockIdx.x * B + bL;
+    const int n2L = blk * B + bL;
     if (n2L < L2) {
         C* out = Gt + (size_t)n2L * (2 * N);
 #pragma unroll
@@ -164,7 +175,8 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t1_cols(
     const int t = threadIdx.x;
     const int bF = t / TPL, jF = t % TPL;  // first stage: whole grid rows (contiguous)
     const int bL = t % B, jL = t / B;      // last stage: B outputs side by side
-    const int n2F = min(blockIdx.x * B + bF, L2 - 1);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int n2F = min(blk * B + bF, L2 - 1);
     const C* in = Gt + (size_t)n2F * (2 * N);
     C v[V][8];
 #pragma unroll
@@ -175,7 +187,7 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t1_cols(
             load_pair(&in[2 * p], v[0][b * R0 + r], v[1][b * R0 + r]);
         }
     fft_lines<N, V, SIGN>(v, lds + bF * V * LL, jF, lds + bL * V * LL, jL, tw, 2 * N);
-    const int n2L = blockIdx.x * B + bL;
+    const int n2L = blk * B + bL;
     if (n2L < L2) {
 #pragma unroll
         for (int b = 0; b < 8 / RL; ++b)
@@ -325,6 +337,19 @@ __global__ __launch_bounds__(256) void k_t1_rows_direct(
 constexpr int kBin = 16;
 constexpr int kMaxLdsBins = 16384;      // per-block LDS histogram limit (64 KiB)
 
+// A sorted point: fractional offsets t = c x - rint(c x) (exact), the packed
+// cell mu2 << 16 | mu1 and the point's original index.  One 16-byte (FP32) or
+// 32-byte (FP64) load per point.
+template <typename T> struct PtRec;
+template <> struct __attribute__((aligned(16))) PtRec<float> {
+    float t1, t2;
+    int cell, idx;
+};
+template <> struct __attribute__((aligned(16))) PtRec<double> {
+    double t1, t2;
+    int cell, idx, pad0, pad1;
+};
+
 template <typename T>
 __device__ __forceinline__ int bin_of(T x, T y, int L1, int L2, T inv1, T inv2, int nb1, int& m1, int& m2, T& t1,
                                       T& t2)
@@ -373,93 +398,119 @@ __global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ x, co
 }
 
 // Pass 2: per bin, turn hist[blk][bin] into the offset of (blk, bin) inside
-// the bin and write the bin total.  A block covers 32 bins x 8 slices of the
-// block range, so the loads of a slice are independent and coalesced.
+// the bin and write the bin total.  A block covers 16 bins x 16 slices of the
+// block range; each thread's loads are independent and coalesced across bins.
+constexpr int kColBins = 16, kColSlices = 16;
 __global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int* __restrict__ tot)
 {
-    __shared__ int part[8][33];
-    const int bl = threadIdx.x % 32, q = threadIdx.x / 32;
-    const int bin = blockIdx.x * 32 + bl;
-    const int per = (nblk + 7) / 8;
+    __shared__ int part[kColSlices][kColBins + 1];
+    const int bl = threadIdx.x % kColBins, q = threadIdx.x / kColBins;
+    const int bin = blockIdx.x * kColBins + bl;
+    const int per = (nblk + kColSlices - 1) / kColSlices;
     const int lo = min(q * per, nblk), hi = min(lo + per, nblk);
+    int v[16];
     int sum = 0;
-    if (bin < nbins)
-        for (int k = lo; k < hi; ++k) sum += hist[(size_t)k * nbins + bin];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int kb = lo + k;
+        v[k] = (bin < nbins && kb < hi) ? hist[(size_t)kb * nbins + bin] : 0;
+        sum += v[k];
+    }
+    for (int kb = lo + 16; kb < hi; ++kb) sum += bin < nbins ? hist[(size_t)kb * nbins + bin] : 0;
     part[q][bl] = sum;
     __syncthreads();
     if (q == 0) {
         int run = 0;
-        for (int j = 0; j < 8; ++j) {
-            const int v = part[j][bl];
+        for (int j = 0; j < kColSlices; ++j) {
+            const int t = part[j][bl];
             part[j][bl] = run;
-            run += v;
+            run += t;
         }
         if (bin < nbins) tot[bin] = run;
     }
     __syncthreads();
+    if (bin >= nbins) return;
     int run = part[q][bl];
-    if (bin < nbins)
-        for (int k = lo; k < hi; ++k) {
-            const int v = hist[(size_t)k * nbins + bin];
-            hist[(size_t)k * nbins + bin] = run;
-            run += v;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int kb = lo + k;
+        if (kb < hi) {
+            hist[(size_t)kb * nbins + bin] = run;
+            run += v[k];
         }
+    }
+    for (int kb = lo + 16; kb < hi; ++kb) {
+        const int t = hist[(size_t)kb * nbins + bin];
+        hist[(size_t)kb * nbins + bin] = run;
+        run += t;
+    }
 }
 
-// Pass 3 (one block): exclusive scan of the bin totals -> binstart, and the
-// subproblem list (bin, start, count <= maxpts) in bin order.
+// Pass 3 (one block): exclusive scan of the bin totals -> binstart, the first
+// subproblem of every bin (subbeg), and the subproblem list (bin, start,
+// count <= maxpts) in bin order.  Bins are scanned in rounds of 1024 with
+// coalesced loads.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_scan_bins(const int* __restrict__ tot, int nbins, int maxpts,
-                                                            int* __restrict__ binstart, int4* __restrict__ sub,
-                                                            int* __restrict__ nsub)
+                                                            int* __restrict__ binstart, int* __restrict__ subbeg,
+                                                            int4* __restrict__ sub, int* __restrict__ nsub)
 {
     __shared__ int w_pts[kScanThreads / 64], w_sub[kScanThreads / 64];
+    __shared__ int carry[2];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int per = (nbins + kScanThreads - 1) / kScanThreads;
-    const int lo = min(t * per, nbins), hi = min(lo + per, nbins);
-    int a = 0, b = 0;
-    for (int i = lo; i < hi; ++i) {
-        const int c = tot[i];
-        a += c;
-        b += (c + maxpts - 1) / maxpts;
-    }
-    int ia = a, ib = b;   // inclusive wave scans
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
-        if (lane >= o) { ia += na; ib += nb; }
-    }
-    if (lane == 63) { w_pts[w] = ia; w_sub[w] = ib; }
+    if (t == 0) { carry[0] = 0; carry[1] = 0; }
     __syncthreads();
-    int ba = 0, bb = 0;
-    for (int k = 0; k < w; ++k) { ba += w_pts[k]; bb += w_sub[k]; }
-    int pos = ba + ia - a, sp = bb + ib - b;
-    for (int i = lo; i < hi; ++i) {
-        const int c = tot[i];
-        binstart[i] = pos;
-        for (int k = 0; k * maxpts < c; ++k) sub[sp++] = make_int4(i, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
-        pos += c;
+    for (int base = 0; base < nbins; base += kScanThreads) {
+        const int i = base + t;
+        const int c = i < nbins ? tot[i] : 0;
+        const int ns = (c + maxpts - 1) / maxpts;
+        int ia = c, ib = ns;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
+            if (lane >= o) { ia += na; ib += nb; }
+        }
+        if (lane == 63) { w_pts[w] = ia; w_sub[w] = ib; }
+        __syncthreads();
+        int ba = carry[0], bb = carry[1];
+        for (int k = 0; k < w; ++k) { ba += w_pts[k]; bb += w_sub[k]; }
+        const int pos = ba + ia - c, sp = bb + ib - ns;
+        if (i < nbins) {
+            binstart[i] = pos;
+            subbeg[i] = sp;
+            for (int k = 0; k < ns; ++k) sub[sp + k] = make_int4(i, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
+        }
+        __syncthreads();
+        if (t == kScanThreads - 1) { carry[0] = pos + c; carry[1] = sp + ns; }
+        __syncthreads();
     }
-    if (t == kScanThreads - 1) *nsub = sp;
+    if (t == 0) {
+        *nsub = carry[1];
+        subbeg[nbins] = carry[1];
+    }
 }
 
-// Pass 5: scatter the points to their sorted positions.
+// Pass 5: scatter the points to their sorted positions, storing what the
+// gather and spread need: the fractional offsets t = c x - rint(c x) (exact),
+// the packed cell (mu2 << 16 | mu1) and the original index.
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
                                                  int L1, int L2, T inv1, T inv2, int nb1, int nbins,
                                                  const int* __restrict__ off, const int* __restrict__ binstart,
-                                                 const int* __restrict__ rank, cplx<T>* __restrict__ xy,
-                                                 int* __restrict__ idx)
+                                                 const int* __restrict__ rank, PtRec<T>* __restrict__ pts)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    const T xi = x[i], yi = y[i];
     int m1, m2;
     T t1, t2;
-    const int bin = bin_of(xi, yi, L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+    const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
     const int p = binstart[bin] + off[(size_t)(i / chunk) * nbins + bin] + rank[i];
-    xy[p] = mk(xi, yi);
-    idx[p] = i;
+    PtRec<T> rec;
+    rec.t1 = t1;
+    rec.t2 = t2;
+    rec.cell = (m2 << 16) | m1;
+    rec.idx = i;
+    pts[p] = rec;
 }
 
 // ================================================================= window taps
@@ -564,7 +615,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
 // neighbouring tile addresses.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(
-    const cplx<T>* __restrict__ Gt, BinGeom geo, const cplx<T>* __restrict__ xy, const int* __restrict__ idx,
+    const cplx<T>* __restrict__ Gt, BinGeom geo, const PtRec<T>* __restrict__ pts,
     const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, cplx<T>* __restrict__ out,
     cplx<T>* __restrict__ out_d1, cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g, T* __restrict__ gx1,
     T* __restrict__ gx2)
@@ -586,12 +637,12 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
     int cell = 0, o = 0;
     T f1 = 0, f2 = 0;
     if (have) {
-        const C q = xy[d.y + tid];
-        o = idx[d.y + tid];
-        int m1, m2;
-        locate(q.x, geo.L1, inv1(geo, T()), m1, f1);
-        locate(q.y, geo.L2, inv2(geo, T()), m2, f2);
-        cell = min(max(m2 - O2, 0), kBin - 1) * kBin + min(max(m1 - O1, 0), kBin - 1);
+        const PtRec<T> q = pts[d.y + tid];
+        const int pc = q.cell;
+        o = q.idx;
+        f1 = q.t1;
+        f2 = q.t2;
+        cell = min(max((pc >> 16) - O2, 0), kBin - 1) * kBin + min(max((pc & 0xffff) - O1, 0), kBin - 1);
     }
     cnt[tid] = 0;
     load_tile(tile, Gt, geo.L1, geo.L2, O1 - kK, O2 - kK);
@@ -674,8 +725,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates
 // the bin + K-halo in LDS, then adds it into the grid with global atomics.
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeom geo, const cplx<T>* __restrict__ xy,
-                                                const int* __restrict__ idx, const cplx<T>* __restrict__ data,
+__global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeom geo, const PtRec<T>* __restrict__ pts,
+                                                const cplx<T>* __restrict__ data,
                                                 const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa)
 {
     using C = cplx<T>;
@@ -693,14 +744,12 @@ __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeo
     }
     __syncthreads();
     for (int p = d.y + threadIdx.x; p < d.y + d.z; p += blockDim.x) {
-        const C q = xy[p];
-        const C v = data[idx[p]];
-        int m1, m2;
-        T f1, f2;
-        locate(q.x, geo.L1, inv1(geo, T()), m1, f1);
-        locate(q.y, geo.L2, inv2(geo, T()), m2, f2);
-        const int c0 = min(max(m1 - O1, 0), kBin - 1);
-        const int r0 = min(max(m2 - O2, 0), kBin - 1);
+        const PtRec<T> q = pts[p];
+        const int pc = q.cell;
+        const C v = data[q.idx];
+        const T f1 = q.t1, f2 = q.t2;
+        const int c0 = min(max((pc & 0xffff) - O1, 0), kBin - 1);
+        const int r0 = min(max((pc >> 16) - O2, 0), kBin - 1);
         T w1[kW], w2[kW];
         taps(f1, wa, w1);
         taps(f2, wa, w2);
@@ -729,81 +778,19 @@ __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeo
 }
 
 // ================================================================= MFMA spread
-// (d) type-1 spread, output-stationary: one unit per 16x16 output bin (more
-// units for dense bins) computes
-//     G[O2+i][O1+j] = sum_p (d_p w2_p[i + O2 - mu2_p + K]) * w1_p[j + O1 - mu1_p + K]
-// over the points p of the 3x3 neighbouring bins whose footprint meets the
-// bin.  Each point is an outer product of two 16-vectors; v_mfma_f32_16x16x4f32
-// adds four of them per instruction into a 16x16 accumulator held in
-// registers.  No atomics: every cell is written once, by its bin's unit, or by
-// the merge kernel for bins split into several units.  Needs L >= 44 and at
-// least 3 bins per dimension (nfft2d_plan chooses the LDS-atomic k_spread
-// otherwise).
-constexpr int kSpreadWaves = 4;
-constexpr int kSpreadCand = 4096;      // candidates (3x3-bin points) per unit
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ int wrap_signed(int v, int L)
-{
-    // v in (-L, L) -> representative in [-L/2, L/2)
-    if (v >= L / 2) v -= L;
-    if (v < -(L / 2)) v += L;
-    return v;
-}
-
-// One block: per output bin, candidates = sum of the 3x3 neighbour bin counts;
-// units = max(1, ceil(cand / kSpreadCand)); bins with several units get
-// partial slots and a merge entry.
-__global__ __launch_bounds__(kScanThreads) void k_spread_units(const int* __restrict__ tot, int nb1, int nb2,
-                                                               int4* __restrict__ units, int* __restrict__ nunits,
-                                                               int4* __restrict__ merges, int* __restrict__ nmerge)
-{
-    __shared__ int w_u[kScanThreads / 64], w_s[kScanThreads / 64], w_m[kScanThreads / 64];
-    const int nbins = nb1 * nb2;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int per = (nbins + kScanThreads - 1) / kScanThreads;
-    const int lo = min(t * per, nbins), hi = min(lo + per, nbins);
-    auto cand_of = [&](int b) {
-        const int b1 = b % nb1, b2 = b / nb1;
-        int c = 0;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int y2 = (b2 + dy + nb2) % nb2, x1 = (b1 + dx + nb1) % nb1;
-                c += tot[y2 * nb1 + x1];
-            }
-        return c;
-    };
-    int cu = 0, cs = 0, cm = 0;
-    for (int b = lo; b < hi; ++b) {
-        const int c = cand_of(b);
-        const int nu = max(1, (c + kSpreadCand - 1) / kSpreadCand);
-        cu += nu;
-        if (nu > 1) { cs += nu; cm += 1; }
-    }
-    int iu = cu, is = cs, im = cm;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int a = __shfl_up(iu, o, 64), b = __shfl_up(is, o, 64), c = __shfl_up(im, o, 64);
-        if (lane >= o) { iu += a; is += b; im += c; }
-    }
-    if (lane == 63) { w_u[w] = iu; w_s[w] = is; w_m[w] = im; }
-    __syncthreads();
-    int bu = 0, bs = 0, bm = 0;
-    for (int k = 0; k < w; ++k) { bu += w_u[k]; bs += w_s[k]; bm += w_m[k]; }
-    int uo = bu + iu - cu, so = bs + is - cs, mo = bm + im - cm;
-    for (int b = lo; b < hi; ++b) {
-        const int c = cand_of(b);
-        const int nu = max(1, (c + kSpreadCand - 1) / kSpreadCand);
-        for (int k = 0; k < nu; ++k)
-            units[uo + k] = make_int4(b, k * kSpreadCand, min(c, (k + 1) * kSpreadCand), nu > 1 ? so + k : -1);
-        if (nu > 1) {
-            merges[mo++] = make_int4(b, so, nu, 0);
-            so += nu;
-        }
-        uo += nu;
-    }
-    if (t == kScanThreads - 1) { *nunits = uo; *nmerge = mo; }
-}
+// (d) type-1 spread, point-stationary.  Each subproblem (<= 256 points of one
+// 16x16 bin) accumulates its points' footprints into a 32x32 tile whose origin
+// is the bin origin minus the K = 6 halo (rows / columns 0..27 are used):
+//     T[i][j] = sum_p (d_p w2_p[i - r_p]) * w1_p[j - c_p],   r_p, c_p = cell - bin origin.
+// Every point is an outer product of two 32-vectors; v_mfma_f32_32x32x2f32
+// adds two per instruction into a 32x32 accumulator held in registers.  The
+// tiles go to scratch, and k_spread_merge sums, for every grid cell, the tiles
+// of the (up to 9) neighbouring bins that cover it, in a fixed order.  No
+// atomics; needs at least 3 bins and L >= 44 per dimension (the plan falls back
+// to the LDS-atomic k_spread otherwise).
+constexpr int kHalo = kBin + 2 * kK;          // 28 used rows / columns of a tile
+constexpr int kTileCells = kHalo * kHalo;     // 784
+typedef float f16v __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ void wave_sync_lds()
 {
@@ -812,160 +799,119 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(kSpreadWaves * 64) void k_spread_mfma(
-    const float2* __restrict__ xy, const int* __restrict__ idx, const float2* __restrict__ data,
-    const int* __restrict__ binstart, const int* __restrict__ tot, const int4* __restrict__ units,
-    const int* __restrict__ nunits, BinGeom geo, int nb2, WinArgs wa, float2* __restrict__ Gt,
-    float2* __restrict__ scratch)
+// Per-wave staging of 64 points, one column per point (odd stride 65):
+// rows 0-12 w2 taps, 13-25 w1 taps, 26 zeros.
+constexpr int kStgLd = 65, kStgRows = 27;
+
+__global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restrict__ pts, const float2* __restrict__ data,
+                                                     const int4* __restrict__ sub, const int* __restrict__ nsub,
+                                                     BinGeom geo, WinArgs wa, float2* __restrict__ scratch)
 {
-    __shared__ __attribute__((aligned(16))) float stage[kSpreadWaves][64][48];
-    __shared__ int list[kSpreadWaves][128];
-    const int u = blockIdx.x;
-    if (u >= *nunits) return;
-    const int4 U = units[u];
+    constexpr int kStageFloats = 4 * kStgRows * kStgLd;                 // 7020
+    constexpr int kRedFloats = 4 * 32 * 32 * 2;                         // 8192
+    __shared__ __attribute__((aligned(16))) float smem[kRedFloats > kStageFloats ? kRedFloats : kStageFloats];
+    __shared__ float4 pinfo[4][64];                                     // (r_p, c_p, d.x, d.y)
+    const int s = blockIdx.x;
+    if (s >= *nsub) return;
+    const int4 d = sub[s];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nb1 = geo.nb1;
-    const int b1 = U.x % nb1, b2 = U.x / nb1;
-    const int O1 = b1 * kBin, O2 = b2 * kBin;
-    int nstart[9], npref[10];
-    npref[0] = 0;
+    const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
+    float* st = smem + w * kStgRows * kStgLd;
+    // stage this wave's points (64 per wave, zeros beyond the subproblem)
+    const int q = 64 * w + lane;
+    float w1[kW], w2[kW];
+    float4 info = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < d.z) {
+        const int p = d.y + q;
+        const PtRec<float> rec = pts[p];
+        const int pc = rec.cell;
+        const float2 v = data[rec.idx];
+        taps(rec.t1, wa, w1);
+        taps(rec.t2, wa, w2);
+        const int rl = min(max((pc >> 16) - O2, 0), kBin - 1), cl = min(max((pc & 0xffff) - O1, 0), kBin - 1);
+        info = make_float4((float)rl, (float)cl, v.x, v.y);
+    } else {
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        const int y2 = (b2 + j / 3 - 1 + nb2) % nb2, x1 = (b1 + j % 3 - 1 + nb1) % nb1;
-        const int nb = y2 * nb1 + x1;
-        nstart[j] = binstart[nb];
-        npref[j + 1] = npref[j] + tot[nb];
+        for (int m = 0; m < kW; ++m) w1[m] = w2[m] = 0.f;
     }
-    float* st = &stage[w][0][0];
-    int* lst = list[w];
-    f4v acc_re = {0.f, 0.f, 0.f, 0.f}, acc_im = {0.f, 0.f, 0.f, 0.f};
-
-    // stage k (<= 64) listed points: rows 0-15 d.x*w2, 16-31 d.y*w2, 32-47 w1 at the bin's offsets
-    auto process = [&](int k) {
-        float* row = st + lane * 48;
 #pragma unroll
-        for (int q = 0; q < 12; ++q) *reinterpret_cast<float4*>(row + 4 * q) = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < k) {
-            const int p = lst[lane];
-            const float2 q = xy[p];
-            const float2 d = data[idx[p]];
-            int m1, m2;
-            float f1, f2;
-            locate(q.x, geo.L1, geo.inv1f, m1, f1);
-            locate(q.y, geo.L2, geo.inv2f, m2, f2);
-            const int dc = wrap_signed(m1 - O1, geo.L1), dr = wrap_signed(m2 - O2, geo.L2);
-            float w1[kW], w2[kW];
-            taps(f1, wa, w1);
-            taps(f2, wa, w2);
-#pragma unroll
-            for (int t = 0; t < kW; ++t) {
-                const int r = dr - kK + t;
-                if (r >= 0 && r < 16) {
-                    row[r] = d.x * w2[t];
-                    row[16 + r] = d.y * w2[t];
-                }
-                const int c = dc - kK + t;
-                if (c >= 0 && c < 16) row[32 + c] = w1[t];
-            }
-        }
-        wave_sync_lds();
-        const int i = lane & 15, kk = lane >> 4;
-        for (int s = 0; s < k; s += 4) {
-            const float* pr = st + (s + kk) * 48;
-            const float are = pr[i], aim = pr[16 + i], bb = pr[32 + i];
-            acc_re = __builtin_amdgcn_mfma_f32_16x16x4f32(are, bb, acc_re, 0, 0, 0);
-            acc_im = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, bb, acc_im, 0, 0, 0);
-        }
-        wave_sync_lds();
-    };
-
-    int n = 0;
-    for (int c = U.y + w * 64; c < U.z; c += kSpreadWaves * 64) {
-        const int cc = c + lane;
-        bool rel = false;
-        int p = 0;
-        if (cc < U.z) {
-            int j = 0;
-#pragma unroll
-            for (int q = 1; q < 9; ++q) j += (npref[q] <= cc) ? 1 : 0;
-            p = nstart[j] + (cc - npref[j]);
-            const float2 q = xy[p];
-            int m1, m2;
-            float f1, f2;
-            locate(q.x, geo.L1, geo.inv1f, m1, f1);
-            locate(q.y, geo.L2, geo.inv2f, m2, f2);
-            const int dc = wrap_signed(m1 - O1, geo.L1), dr = wrap_signed(m2 - O2, geo.L2);
-            rel = dr >= -kK && dr < 16 + kK && dc >= -kK && dc < 16 + kK;
-        }
-        const unsigned long long mask = __ballot(rel);
-        if (rel) lst[n + __popcll(mask & ((1ull << lane) - 1ull))] = p;
-        n += __popcll(mask);
-        if (n >= 64) {
-            wave_sync_lds();
-            process(64);
-            n -= 64;
-            const int carry = lane < n ? lst[64 + lane] : 0;
-            wave_sync_lds();
-            if (lane < n) lst[lane] = carry;
-            wave_sync_lds();
-        }
+    for (int m = 0; m < kW; ++m) {
+        st[m * kStgLd + lane] = w2[m];
+        st[(kW + m) * kStgLd + lane] = w1[m];
     }
-    if (n > 0) {
-        wave_sync_lds();
-        process(n);
+    st[2 * kW * kStgLd + lane] = 0.f;
+    pinfo[w][lane] = info;
+    wave_sync_lds();
+    f16v acc_re = {}, acc_im = {};
+    const int nw = min(64, max(0, d.z - 64 * w));     // points of this wave
+    const int i = lane & 31, kk = lane >> 5;
+    for (int s2 = 0; s2 < nw; s2 += 2) {
+        const int pt = s2 + kk;
+        const float4 pi = pinfo[w][pt];
+        const int ra = i - (int)pi.x, ca = i - (int)pi.y;
+        const float a = st[((unsigned)ra < (unsigned)kW ? ra : 2 * kW) * kStgLd + pt];
+        const float b = st[((unsigned)ca < (unsigned)kW ? kW + ca : 2 * kW) * kStgLd + pt];
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a * pi.z, b, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(a * pi.w, b, acc_im, 0, 0, 0);
     }
-    // reduce the waves' accumulators in a fixed order
+    // reduce the 4 waves' tiles in a fixed order and store the 28x28 used part
     __syncthreads();
-    float* red = &stage[0][0][0];
+    float* red = smem;                 // [w][cell(32x32)][re, im]
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        red[(w * 64 + lane) * 8 + r] = acc_re[r];
-        red[(w * 64 + lane) * 8 + 4 + r] = acc_im[r];
+    for (int r = 0; r < 16; ++r) {
+        const int ti = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), tj = lane & 31;
+        red[(w * 1024 + ti * 32 + tj) * 2] = acc_re[r];
+        red[(w * 1024 + ti * 32 + tj) * 2 + 1] = acc_im[r];
     }
     __syncthreads();
-    if (w == 0) {
-        float vr[4], vi[4];
+    float2* out = scratch + (size_t)s * kTileCells;
+    for (int e = threadIdx.x; e < kTileCells; e += 256) {
+        const int ti = e / kHalo, tj = e - ti * kHalo;
+        const int c = ti * 32 + tj;
+        float sr = red[c * 2], si = red[c * 2 + 1];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            vr[r] = red[lane * 8 + r];
-            vi[r] = red[lane * 8 + 4 + r];
+        for (int ww = 1; ww < 4; ++ww) {
+            sr += red[(ww * 1024 + c) * 2];
+            si += red[(ww * 1024 + c) * 2 + 1];
         }
-        for (int ww = 1; ww < kSpreadWaves; ++ww)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                vr[r] += red[(ww * 64 + lane) * 8 + r];
-                vi[r] += red[(ww * 64 + lane) * 8 + 4 + r];
-            }
-        const int j = lane & 15;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 4 * (lane >> 4) + r;
-            if (U.w >= 0) {
-                scratch[(size_t)U.w * 256 + i * 16 + j] = make_float2(vr[r], vi[r]);
-            } else if (O2 + i < geo.L2 && O1 + j < geo.L1) {
-                Gt[(size_t)(O2 + i) * geo.L1 + (O1 + j)] = make_float2(vr[r], vi[r]);
-            }
-        }
+        out[e] = make_float2(sr, si);
     }
 }
 
-// Sum the partial tiles of the bins split into several units, in unit order.
-__global__ __launch_bounds__(256) void k_spread_merge(const int4* __restrict__ merges, const int* __restrict__ nmerge,
-                                                      const float2* __restrict__ scratch, BinGeom geo,
+// Grid cell (n2, n1) = sum over the neighbouring bins b' (3x3, fixed order) and
+// their subproblems (in order) of tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1]
+// where that index lies in [0, 28).  One block per bin writes the bin's cells.
+__global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__ scratch,
+                                                      const int* __restrict__ subbeg, BinGeom geo, int nb2,
                                                       float2* __restrict__ Gt)
 {
-    const int m = blockIdx.x;
-    if (m >= *nmerge) return;
-    const int4 M4 = merges[m];
-    const int i = threadIdx.x / 16, j = threadIdx.x % 16;
-    const int O1 = (M4.x % geo.nb1) * kBin, O2 = (M4.x / geo.nb1) * kBin;
-    float sr = 0.f, si = 0.f;
-    for (int k = 0; k < M4.z; ++k) {
-        const float2 v = scratch[(size_t)(M4.y + k) * 256 + threadIdx.x];
-        sr += v.x;
-        si += v.y;
+    const int nb1 = geo.nb1;
+    const int bin = blockIdx.x;
+    const int b1 = bin % nb1, b2 = bin / nb1;
+    const int n2 = b2 * kBin + threadIdx.x / kBin, n1 = b1 * kBin + threadIdx.x % kBin;
+    if (n2 >= geo.L2 || n1 >= geo.L1) return;
+    int beg[9], end[9], off[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int y2 = (b2 + j / 3 - 1 + nb2) % nb2, x1 = (b1 + j % 3 - 1 + nb1) % nb1;
+        int r = n2 - (y2 * kBin - kK), c = n1 - (x1 * kBin - kK);
+        r = r < 0 ? r + geo.L2 : (r >= geo.L2 ? r - geo.L2 : r);
+        c = c < 0 ? c + geo.L1 : (c >= geo.L1 ? c - geo.L1 : c);
+        const int nb = y2 * nb1 + x1;
+        const bool in = r < kHalo && c < kHalo;
+        beg[j] = in ? subbeg[nb] : 0;
+        end[j] = in ? subbeg[nb + 1] : 0;
+        off[j] = r * kHalo + c;
     }
-    if (O2 + i < geo.L2 && O1 + j < geo.L1) Gt[(size_t)(O2 + i) * geo.L1 + (O1 + j)] = make_float2(sr, si);
+    float sr = 0.f, si = 0.f;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        for (int s = beg[j]; s < end[j]; ++s) {
+            const float2 v = scratch[(size_t)s * kTileCells + off[j]];
+            sr += v.x;
+            si += v.y;
+        }
+    Gt[(size_t)n2 * geo.L1 + n1] = make_float2(sr, si);
 }
 
 }  // namespace nfft2d
