@@ -176,7 +176,9 @@ def bench_grad(args, dev):
     f = torch.zeros(N, N, dtype=torch.complex64, device=dev)
     y = torch.empty(M, dtype=torch.complex64, device=dev)
     gf = torch.empty(N, N, dtype=torch.complex64, device=dev)
-    lr = 1.0 / (2 * M)
+    # GD is stable for lr < 1 / lambda_max(A^H A); for M/N^2 = 4 uniform points
+    # lambda_max is about M (1 + sqrt(N^2/M))^2 = 2.25 M
+    lr = 1.0 / (4 * M)
 
     def loss_now() -> float:                       # eager, outside the graph
         return float(((plan.type2(f) - y_obs).abs() ** 2).sum().item())
@@ -200,7 +202,7 @@ def bench_grad(args, dev):
         g_step.replay()
         total_steps += 1
     torch.cuda.synchronize()
-    return dict(N=N, M=M, s=s, loss0=loss0, loss_final=loss_now(), steps=total_steps,
+    return dict(N=N, M=M, s=s, loss0=loss0, loss_final=loss_now(), steps=total_steps, lr=lr,
                 bytes=408 * N * N + 80 * M)
 
 
@@ -215,7 +217,7 @@ def cpu_baseline(t2, seconds: float):
         oracle.nfft1_2d(f, x1, x2, precision="float")
         reps += 1
         el = time.perf_counter() - t0
-        if (el >= seconds and reps >= 2) or reps >= 50:
+        if (el >= seconds and reps >= 2) or reps >= 1000:
             break
     return dict(value=len(x1) * reps / el / 1e6, unit="Mpts/s", cores=oracle.num_threads(), kind="port",
                 sample=f"{reps} full type-2 transforms (512x512, M=1048576 polar, FP32) of the CPU restatement "
@@ -232,6 +234,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "gather_polar512_traffic.json"),
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
@@ -240,6 +244,13 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
 
+    if args.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t2 = bench_type2(args, dev)
     gr = None if args.no_grad else bench_grad(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
@@ -278,7 +289,8 @@ def main():
         out["grad"] = {"steps_per_s": 1.0 / gr["s"], "ms_per_step": gr["s"] * 1e3,
                        "config": "256x256 modes, M=262144 uniform, loss ||A(x)f - y||^2, grad f and x, GD update",
                        "hbm_frac": gr["bytes"] / gr["s"] / (HBM_PEAK_GBS * 1e9),
-                       "loss_first": gr["loss0"], "loss_after_steps": gr["loss_final"], "gd_steps": gr["steps"]}
+                       "loss_first": gr["loss0"], "loss_after_steps": gr["loss_final"], "gd_steps": gr["steps"],
+                       "gd_lr": gr["lr"]}
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
