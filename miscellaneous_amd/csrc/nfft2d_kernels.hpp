@@ -569,6 +569,11 @@ __device__ __forceinline__ double inv2(const BinGeom& g, double) { return g.inv2
 // o1, o2 are in [-K, L); when L >= W one adjustment by L wraps every index.
 __device__ __forceinline__ int wrap1(int g, int L) { return g < 0 ? g + L : (g >= L ? g - L : g); }
 
+#ifndef NFFT2D_TILE_LD
+#define NFFT2D_TILE_LD 32
+#endif
+constexpr int kTileLd = NFFT2D_TILE_LD;     // LDS row stride of a gather tile (complex elements)
+
 template <typename C>
 __device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
 {
@@ -576,12 +581,12 @@ __device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int
     if (L1 >= W && L2 >= W) {
         for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
             const int r = e / W, c = e - r * W;
-            tile[e] = Gt[(size_t)wrap1(o2 + r, L2) * L1 + wrap1(o1 + c, L1)];
+            tile[r * kTileLd + c] = Gt[(size_t)wrap1(o2 + r, L2) * L1 + wrap1(o1 + c, L1)];
         }
     } else {
         for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
             const int r = e / W, c = e - r * W;
-            tile[e] = Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
+            tile[r * kTileLd + c] = Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
         }
     }
 }
@@ -621,8 +626,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
     T* __restrict__ gx2)
 {
     using C = cplx<T>;
-    constexpr int W = kBin + 2 * kK;
-    __shared__ C tile[W * W];
+    constexpr int W = kTileLd;
+    __shared__ C tile[(kBin + 2 * kK) * kTileLd];
     __shared__ int cnt[kBin * kBin];
     __shared__ int s_warp[kGatherThreads / 64];
     __shared__ T s_f1[kGatherThreads], s_f2[kGatherThreads];
