@@ -141,10 +141,39 @@ int nfft2d_type2_interp(nfft2d_plan *plan, void *result, void *stream);
 int nfft2d_type1_spread(nfft2d_plan *plan, const void *data, void *stream);
 int nfft2d_type1_finish(nfft2d_plan *plan, void *result, int sign, void *stream);
 
-/* Points per subproblem of the spread / gather (default 1024). */
+/* Points per subproblem of the spread / gather: 32 to 256 (default 256). */
 int nfft2d_plan_set_maxpts(nfft2d_plan *plan, int maxpts);
 /* Number of points set on the plan, or < 0 on error. */
 int nfft2d_plan_num_points(const nfft2d_plan *plan);
+
+/* ------------------------------------------------ NDFT as an MFMA contraction */
+
+/* Type-2 direct sum over the equispaced modes, with no window and no grid:
+ *   result[m] = sum_{k1,k2} data[k1][k2] e^{-2 pi i (k1 x[m] / N1 + k2 y[m] / N2)},
+ * i.e. the sum of NFFT/NDFT2_2D.cu:15-32,67 and NFFT/NFFT2/Matlab/NDFT2_2D.m:9-14
+ * in the modes -> points direction, evaluated as the separable contraction
+ * y = rowsum(A o (B data^T)), A[m][k1] = e^{-2 pi i k1 x_m / N1},
+ * B[m][k2] = e^{-2 pi i k2 y_m / N2}, on the matrix cores.  The product
+ * B data^T runs in `precision`, always with FP32 accumulation:
+ *   NFFT2D_NDFT_FP32    FP32 operands;
+ *   NFFT2D_NDFT_BF16X3  each operand split x = hi + lo into two BF16 parts,
+ *                       product hi*hi + hi*lo + lo*hi;
+ *   NFFT2D_NDFT_FP16    FP16 operands, each k1 row of `data` scaled by a power
+ *                       of two into the FP16 range (undone after the product).
+ * The phasors A and B are evaluated in FP32 from an FP64 phase reduction.
+ * Layouts and units are those of NFFT1_2D_GPU_c64.  The plan holds the mode
+ * operand in the matrix cores' operand order (about 4 N1 N2 bytes for BF16X3);
+ * nfft2d_ndft_set_modes fills it, nfft2d_ndft_eval contracts it with M
+ * points, nfft2d_ndft_type2 does both.  N1, N2 even, at most 8192. */
+typedef struct nfft2d_ndft_plan_s nfft2d_ndft_plan;
+enum { NFFT2D_NDFT_FP32 = 0, NFFT2D_NDFT_BF16X3 = 1, NFFT2D_NDFT_FP16 = 2 };
+int nfft2d_ndft_plan_create(nfft2d_ndft_plan **plan, int N1, int N2, int precision);
+int nfft2d_ndft_plan_destroy(nfft2d_ndft_plan *plan);
+int nfft2d_ndft_set_modes(nfft2d_ndft_plan *plan, const nfft2d_c64 *data, void *stream);
+int nfft2d_ndft_eval(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const float *x, const float *y, int M,
+                     void *stream);
+int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x,
+                      const float *y, int M, void *stream);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,9 @@ HEADER = os.path.join(ROOT, "include", "nfft2d.h")
 NFFT2D_OK = 0
 NFFT2D_C64 = 0
 NFFT2D_C128 = 1
+NFFT2D_NDFT_FP32 = 0
+NFFT2D_NDFT_BF16X3 = 1
+NFFT2D_NDFT_FP16 = 2
 
 _ERR_NAMES = {-1: "NFFT2D_ERR_ARG", -2: "NFFT2D_ERR_SIZE", -3: "NFFT2D_ERR_HIP", -4: "NFFT2D_ERR_ALLOC",
               -5: "NFFT2D_ERR_STATE", -6: "NFFT2D_ERR_UNSUPPORTED"}
@@ -58,6 +61,11 @@ _SIGS = {
     "nfft2d_type1_finish": (_i, [_vp, _vp, _i, _vp]),
     "nfft2d_plan_set_maxpts": (_i, [_vp, _i]),
     "nfft2d_plan_num_points": (_i, [_vp]),
+    "nfft2d_ndft_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i]),
+    "nfft2d_ndft_plan_destroy": (_i, [_vp]),
+    "nfft2d_ndft_set_modes": (_i, [_vp, _vp, _vp]),
+    "nfft2d_ndft_eval": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "nfft2d_ndft_type2": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
 }
 
 _lib = None

@@ -1,5 +1,5 @@
-// ndft2d.hip -- direct 2D NDFT in FP64, the reference's NDFT2_2D_GPU
-// (NFFT/NDFT2_2D.cu:50-85):
+// ndft2d.hip -- the 2D NDFT.  Part 1: NDFT2_2D_GPU, the reference's direct
+// sum in FP64 (NFFT/NDFT2_2D.cu:50-85):
 //   out[n] = sum_{m<M} in[m] exp(-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)).
 // The reference fills an explicit M x N FP64 kernel matrix (:57-63; 64 GiB at
 // M = N = 65536), writes it to a text file (:69) and calls cublasZgemv (:80).
@@ -7,7 +7,10 @@
 // streams the points through LDS in chunks, accumulating in FP64.
 #include <hip/hip_runtime.h>
 
+#include <new>
+
 #include "../../include/nfft2d.h"
+#include "ndft2d_mfma.hpp"
 #include "nfft2d_internal.hpp"
 
 namespace {
@@ -64,4 +67,136 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
                        N);
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
+}
+
+// ============================================================================
+// Part 2: the type-2 direct sum as a split-precision MFMA contraction
+// (kernels in ndft2d_mfma.hpp).  A plan holds the mode operand.
+namespace {
+
+constexpr int kNdftMaxN = 8192;
+constexpr int kNdftC = ndft::kChunkTiles;  // MFMA column tiles per chunk
+
+size_t ndft_slice_bytes(int prec)
+{
+    switch (prec) {
+    case NFFT2D_NDFT_FP32: return sizeof(ndft::Slice<ndft::kNdftFp32>);
+    case NFFT2D_NDFT_BF16X3: return sizeof(ndft::Slice<ndft::kNdftBf16x3>);
+    default: return sizeof(ndft::Slice<ndft::kNdftFp16>);
+    }
+}
+
+}  // namespace
+
+struct nfft2d_ndft_plan_s {
+    int N1, N2, prec, device;
+    int nch, nks;
+    void *G;              // mode operand, nch * nks * C * 64 slices
+    float *scale;         // N1 per-row scales and their inverses
+    float *inv_scale;
+};
+
+extern "C" int nfft2d_ndft_plan_create(nfft2d_ndft_plan **plan, int N1, int N2, int precision)
+{
+    if (!plan) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_plan_create: null plan pointer");
+    *plan = nullptr;
+    if (N1 <= 0 || N2 <= 0 || (N1 & 1) || (N2 & 1) || N1 > kNdftMaxN || N2 > kNdftMaxN)
+        return nfft2d_fail(NFFT2D_ERR_SIZE, "nfft2d_ndft_plan_create: N1=%d N2=%d must be even, in [2, %d]", N1, N2,
+                           kNdftMaxN);
+    if (precision != NFFT2D_NDFT_FP32 && precision != NFFT2D_NDFT_BF16X3 && precision != NFFT2D_NDFT_FP16)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_plan_create: unknown precision %d", precision);
+    nfft2d_ndft_plan_s *p = new (std::nothrow) nfft2d_ndft_plan_s();
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft2d_ndft_plan_create: host allocation failed");
+    p->N1 = N1;
+    p->N2 = N2;
+    p->prec = precision;
+    NFFT2D_HIP_TRY(hipGetDevice(&p->device));
+    p->nch = (N1 + ndft::kChunkK1 - 1) / ndft::kChunkK1;
+    p->nks = (N2 + 8 * ndft::kKsPad - 1) / (8 * ndft::kKsPad) * ndft::kKsPad;
+    const size_t gbytes = (size_t)p->nch * p->nks * kNdftC * 64 * ndft_slice_bytes(precision);
+    if (hipMalloc(&p->G, gbytes) != hipSuccess || hipMalloc((void **)&p->scale, sizeof(float) * N1) != hipSuccess ||
+        hipMalloc((void **)&p->inv_scale, sizeof(float) * N1) != hipSuccess) {
+        (void)hipGetLastError();
+        nfft2d_ndft_plan_destroy(p);
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft2d_ndft_plan_create: device allocation of %zu bytes failed", gbytes);
+    }
+    *plan = p;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_ndft_plan_destroy(nfft2d_ndft_plan *p)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_plan_destroy: null plan");
+    if (p->G) (void)hipFree(p->G);
+    if (p->scale) (void)hipFree(p->scale);
+    if (p->inv_scale) (void)hipFree(p->inv_scale);
+    delete p;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_ndft_set_modes(nfft2d_ndft_plan *p, const nfft2d_c64 *data, void *stream)
+{
+    if (!p || !data) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_set_modes: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const float2 *F = reinterpret_cast<const float2 *>(data);
+    hipLaunchKernelGGL(ndft::k_ndft_rowscale, dim3(p->N1), dim3(256), 0, s, F, p->scale, p->inv_scale, p->N2,
+                       p->prec == NFFT2D_NDFT_FP16 ? 1 : 0);
+    NFFT2D_LAUNCH_CHECK();
+    const long long total = (long long)p->nch * p->nks * kNdftC * 64;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    switch (p->prec) {
+    case NFFT2D_NDFT_FP32:
+        hipLaunchKernelGGL((ndft::k_ndft_prep<ndft::kNdftFp32, kNdftC>), grid, dim3(256), 0, s, F, p->scale,
+                           (ndft::Slice<ndft::kNdftFp32> *)p->G, p->N1, p->N2, p->nch, p->nks);
+        break;
+    case NFFT2D_NDFT_BF16X3:
+        hipLaunchKernelGGL((ndft::k_ndft_prep<ndft::kNdftBf16x3, kNdftC>), grid, dim3(256), 0, s, F, p->scale,
+                           (ndft::Slice<ndft::kNdftBf16x3> *)p->G, p->N1, p->N2, p->nch, p->nks);
+        break;
+    default:
+        hipLaunchKernelGGL((ndft::k_ndft_prep<ndft::kNdftFp16, kNdftC>), grid, dim3(256), 0, s, F, p->scale,
+                           (ndft::Slice<ndft::kNdftFp16> *)p->G, p->N1, p->N2, p->nch, p->nks);
+        break;
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_ndft_eval(nfft2d_ndft_plan *p, nfft2d_c64 *result, const float *x, const float *y, int M,
+                                void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_eval: null plan");
+    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_eval: negative M");
+    if (M == 0) return NFFT2D_OK;
+    if (!result || !x || !y) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_eval: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((M + ndft::kPtsPerGroup - 1) / ndft::kPtsPerGroup));
+    float2 *out = reinterpret_cast<float2 *>(result);
+    switch (p->prec) {
+    case NFFT2D_NDFT_FP32:
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp32, 4>), grid, dim3(256), 0, s,
+                           (const ndft::Slice<ndft::kNdftFp32> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
+                           p->nch, p->nks);
+        break;
+    case NFFT2D_NDFT_BF16X3:
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftBf16x3, 16>), grid, dim3(256), 0, s,
+                           (const ndft::Slice<ndft::kNdftBf16x3> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
+                           p->nch, p->nks);
+        break;
+    default:
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp16, 16>), grid, dim3(256), 0, s,
+                           (const ndft::Slice<ndft::kNdftFp16> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
+                           p->nch, p->nks);
+        break;
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_ndft_type2(nfft2d_ndft_plan *p, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x,
+                                 const float *y, int M, void *stream)
+{
+    int rc = nfft2d_ndft_set_modes(p, data, stream);
+    if (rc != NFFT2D_OK) return rc;
+    return nfft2d_ndft_eval(p, result, x, y, M, stream);
 }

@@ -70,6 +70,29 @@ def test_null_arguments_rejected(lib):
     assert lib.NDFT2_2D_GPU(None, None, None, None, None, None, None, 0, 4, 3, 4) == -1
 
 
+@pytest.mark.parametrize("args,code,needle", [
+    ((63, 64, 1), -2, "even"),
+    ((64, 0, 1), -2, "even"),
+    ((16384, 64, 1), -2, "8192"),
+    ((64, 64, 3), -1, "precision"),
+    ((64, 64, -1), -1, "precision"),
+])
+def test_ndft_plan_create_validation(lib, args, code, needle):
+    h = ctypes.c_void_p()
+    rc = lib.nfft2d_ndft_plan_create(ctypes.byref(h), *args)
+    assert rc == code
+    assert needle in lib.nfft2d_last_error().decode()
+    assert not h.value
+
+
+def test_ndft_null_arguments_rejected(lib):
+    assert lib.nfft2d_ndft_plan_create(None, 64, 64, 1) == -1
+    assert lib.nfft2d_ndft_plan_destroy(None) == -1
+    assert lib.nfft2d_ndft_set_modes(None, None, None) == -1
+    assert lib.nfft2d_ndft_eval(None, None, None, None, 10, None) == -1
+    assert lib.nfft2d_ndft_type2(None, None, None, None, None, 10, None) == -1
+
+
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         _lib.load_library(str(tmp_path / "libnfft2d.so"))

@@ -1,0 +1,131 @@
+"""GPU parity of the type-2 NDFT contraction (NdftPlan) against the FP64 oracle.
+
+The three precisions are held to rel-L2 bounds set from their operand
+rounding (unit roundoff u of the operand type, errors averaging over the
+2 N2-long sums):
+  fp32    2e-6   (u = 6e-8;   measured by emulation ~3e-7)
+  bf16x3  2e-5   (hi + lo carries ~16 bits; emulation ~4e-6)
+  fp16    2e-3   (u = 4.9e-4; emulation ~3e-4)
+"""
+import numpy as np
+import pytest
+import torch
+
+import miscellaneous_amd as m
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+TOL = {"fp32": 2e-6, "bf16x3": 2e-5, "fp16": 2e-3}
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def cn01(rng, shape):
+    return ((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)).astype(np.complex64)
+
+
+def points(rng, M, N1, N2):
+    x1 = ((rng.random(M) - 0.5) * N1).astype(np.float32)
+    x2 = ((rng.random(M) - 0.5) * N2).astype(np.float32)
+    return x1, x2
+
+
+def ndft(prec, f, x1, x2):
+    N1, N2 = f.shape
+    plan = m.NdftPlan(N1, N2, prec, device=DEV)
+    return plan.type2(T(f), T(x1), T(x2)).cpu().numpy()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16"])
+@pytest.mark.parametrize("N1,N2,M", [(16, 12, 300), (64, 64, 5000), (6, 10, 129), (128, 40, 1000), (256, 256, 2048)])
+def test_ndft_type2_vs_oracle(prec, N1, N2, M):
+    rng = np.random.default_rng(1000 + N1 * 7 + N2 + M)
+    f = cn01(rng, (N1, N2))
+    x1, x2 = points(rng, M, N1, N2)
+    y = ndft(prec, f, x1, x2)
+    ref = oracle.ndft_type2(f.astype(np.complex128), x1, x2)
+    err = oracle.rel_l2(y, ref)
+    print(f"ndft {prec} {N1}x{N2} M={M}: rel-L2 {err:.3e}")
+    assert err <= TOL[prec], err
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16"])
+def test_ndft_c5a_subset(prec):
+    """256 x 256 modes, M = 65536 uniform points; FP64 check on 4096 seeded outputs."""
+    rng = np.random.default_rng(20261202)
+    N = 256
+    M = 65536
+    f = cn01(rng, (N, N))
+    x1, x2 = points(rng, M, N, N)
+    y = ndft(prec, f, x1, x2)
+    sel = np.sort(rng.choice(M, 4096, replace=False))
+    ref = oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel])
+    err = oracle.rel_l2(y[sel], ref)
+    print(f"ndft C5a {prec}: rel-L2 {err:.3e}")
+    assert err <= TOL[prec], err
+
+
+def test_ndft_matches_nfft():
+    """The NDFT and the K = 6 NFFT compute the same sum: they agree to the NFFT's accuracy."""
+    rng = np.random.default_rng(7)
+    N, M = 128, 16384
+    f = cn01(rng, (N, N))
+    x1, x2 = points(rng, M, N, N)
+    y_ndft = ndft("fp32", f, x1, x2)
+    plan = m.Plan(N, N, torch.complex64, device=DEV)
+    plan.setpts(T(x1), T(x2))
+    y_nfft = plan.type2(T(f)).cpu().numpy()
+    assert oracle.rel_l2(y_ndft, y_nfft) <= 2e-6
+
+
+def test_ndft_periodic_and_isolated_nan():
+    """x -> x + N1 leaves every output unchanged; a NaN point poisons only its own output."""
+    rng = np.random.default_rng(11)
+    N1, N2, M = 32, 48, 700
+    f = cn01(rng, (N1, N2))
+    x1, x2 = points(rng, M, N1, N2)
+    # multiples of 2^-12, so that the shifted coordinates are exact in fp32
+    x1 = (np.round(x1 * 4096) / 4096).astype(np.float32)
+    x2 = (np.round(x2 * 4096) / 4096).astype(np.float32)
+    y0 = ndft("fp32", f, x1, x2)
+    y1 = ndft("fp32", f, (x1 + np.float32(N1)).astype(np.float32), (x2 - np.float32(2 * N2)).astype(np.float32))
+    assert oracle.rel_l2(y1, y0) <= 2e-6
+    x1n = x1.copy()
+    x1n[5] = np.nan
+    y2 = ndft("bf16x3", f, x1n, x2)
+    assert np.isnan(y2[5])
+    keep = np.arange(M) != 5
+    assert np.all(np.isfinite(y2[keep]))
+    ref = oracle.ndft_type2(f.astype(np.complex128), x1[keep], x2[keep])
+    assert oracle.rel_l2(y2[keep], ref) <= TOL["bf16x3"]
+
+
+def test_ndft_fp16_row_scaling_range():
+    """Rows of very different magnitude (1e-3 .. 1e3) keep FP16 accuracy thanks to the per-row scale."""
+    rng = np.random.default_rng(12)
+    N1, N2, M = 64, 64, 3000
+    f = cn01(rng, (N1, N2)) * (10.0 ** rng.uniform(-3, 3, size=(N1, 1))).astype(np.float32)
+    f = f.astype(np.complex64)
+    x1, x2 = points(rng, M, N1, N2)
+    y = ndft("fp16", f, x1, x2)
+    ref = oracle.ndft_type2(f.astype(np.complex128), x1, x2)
+    assert oracle.rel_l2(y, ref) <= TOL["fp16"]
+
+
+def test_ndft_empty_and_plan_reuse():
+    rng = np.random.default_rng(13)
+    N1, N2 = 16, 16
+    plan = m.NdftPlan(N1, N2, "bf16x3", device=DEV)
+    f = cn01(rng, (N1, N2))
+    plan.set_modes(T(f))
+    e = torch.empty(0, dtype=torch.float32, device=DEV)
+    assert plan.eval(e, e).numel() == 0
+    for M in (1, 130, 64):
+        x1, x2 = points(rng, M, N1, N2)
+        y = plan.eval(T(x1), T(x2)).cpu().numpy()
+        ref = oracle.ndft_type2(f.astype(np.complex128), x1, x2)
+        assert oracle.rel_l2(y, ref) <= TOL["bf16x3"]
