@@ -144,9 +144,10 @@ struct nfft2d_plan_s {
     int M = -1, Mcap = 0, subcap = 0, histcap = 0;
     int nblk = 0, chunk = 0;              // setpts blocking of the last point set
     void *pts = nullptr;                  // sorted point records PtRec<T>
-    int *idx = nullptr, *rank = nullptr, *hist = nullptr, *tot = nullptr, *binstart = nullptr, *nsub = nullptr;
+    int *idx = nullptr, *rank = nullptr, *hist = nullptr, *tot = nullptr, *nsub = nullptr;
     int4 *sub = nullptr;
     int *subbeg = nullptr;                // first subproblem of each bin (nbins + 1)
+    int2 *gexcl = nullptr, *gsum = nullptr;  // bin-scan prefixes inside / of groups of kColBins bins
     void *scratch = nullptr;              // MFMA spread tiles, 28x28 per subproblem
     int scratchcap = 0;
     BinGeom geo;
@@ -192,11 +193,12 @@ static void free_plan(nfft2d_plan *p)
     (void)hipSetDevice(p->device);
     hfree(p->tw1); hfree(p->tw2); hfree(p->D1); hfree(p->D2);
     hfree(p->grid); hfree(p->tmp);
-    hfree(p->tot); hfree(p->binstart); hfree(p->nsub);
+    hfree(p->tot); hfree(p->nsub);
     free_points(p);
     free_hist(p);
     free_scratch(p);
     hfree(p->subbeg);
+    hfree(p->gexcl); hfree(p->gsum);
     (void)hipSetDevice(cur);
     delete p;
 }
@@ -263,9 +265,10 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
     if (hipMalloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||
-        hipMalloc(&p->binstart, sizeof(int) * p->nbins) != hipSuccess ||
         hipMalloc(&p->nsub, sizeof(int)) != hipSuccess ||
-        hipMalloc(&p->subbeg, sizeof(int) * (p->nbins + 1)) != hipSuccess) {
+        hipMalloc(&p->subbeg, sizeof(int) * (p->nbins + 1)) != hipSuccess ||
+        hipMalloc(&p->gexcl, sizeof(int2) * p->nbins) != hipSuccess ||
+        hipMalloc(&p->gsum, sizeof(int2) * ((p->nbins + kColBins - 1) / kColBins)) != hipSuccess) {
         free_plan(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the grid failed");
     }
@@ -336,8 +339,8 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
     }
     const T inv1 = (T)(1.0 / p->L1), inv2 = (T)(1.0 / p->L2);
     if (lds) {
-        hipLaunchKernelGGL(k_hist<T>, dim3(nblk), dim3(256), sizeof(int) * p->nbins, s, x, y, M, chunk, p->L1, p->L2,
-                           inv1, inv2, p->nb1, p->nbins, p->hist, p->rank);
+        hipLaunchKernelGGL(k_hist<T>, dim3(nblk), dim3(kHistThreads), sizeof(int) * p->nbins, s, x, y, M, chunk, p->L1,
+                           p->L2, inv1, inv2, p->nb1, p->nbins, p->hist, p->rank);
     } else {
         int rc = zero_fill<int>(p->hist, p->nbins, s);
         if (rc) return rc;
@@ -346,16 +349,16 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
                                inv2, p->nb1, p->hist, p->rank);
     }
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colscan, dim3(grid1d(p->nbins, kColBins)), dim3(256), 0, s, p->hist, nblk, p->nbins, p->tot);
+    const int ngroups = (int)grid1d(p->nbins, kColBins);
+    hipLaunchKernelGGL(k_colscan, dim3(ngroups), dim3(256), 0, s, p->hist, nblk, p->nbins, p->maxpts, p->tot, p->gexcl,
+                       p->gsum);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, s, p->tot, p->nbins, p->maxpts, p->binstart,
-                       p->subbeg, p->sub, p->nsub);
+    hipLaunchKernelGGL(k_scan_groups, dim3(1), dim3(kScanThreads), 0, s, p->gsum, ngroups, p->nbins, p->subbeg, p->nsub);
     NFFT2D_LAUNCH_CHECK();
-    if (M > 0) {
-        hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(M, 256)), dim3(256), 0, s, x, y, M, chunk, p->L1, p->L2, inv1,
-                           inv2, p->nb1, p->nbins, p->hist, p->binstart, p->rank, (PtRec<T> *)p->pts);
-        NFFT2D_LAUNCH_CHECK();
-    }
+    hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(std::max(M, p->nbins), 256)), dim3(256), 0, s, x, y, M, chunk, p->L1,
+                       p->L2, inv1, inv2, p->nb1, p->nbins, p->maxpts, p->hist, p->tot, p->gexcl, p->gsum, p->rank,
+                       (PtRec<T> *)p->pts, p->subbeg, p->sub);
+    NFFT2D_LAUNCH_CHECK();
     p->nblk = nblk;
     p->chunk = chunk;
     p->M = M;

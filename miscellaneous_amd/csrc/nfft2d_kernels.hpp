@@ -359,23 +359,65 @@ __device__ __forceinline__ int bin_of(T x, T y, int L1, int L2, T inv1, T inv2, 
     return (m2 / kBin) * nb1 + m1 / kBin;
 }
 
+// Rank of this lane's point among the wave's points of the same bin, with one
+// counter increment per run of equal bins in consecutive lanes (points that
+// arrive in coordinate order, as along the rays of a polar grid, share bins
+// over long runs).  Inactive lanes pass act = false.  Returns the point's rank
+// in counter[bin].
+__device__ __forceinline__ int wave_run_rank(int* counter, int bin, bool act)
+{
+    const int lane = threadIdx.x & 63;
+    const int key = act ? bin : -1 - lane;                // never equal to a neighbour
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || key != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int s = 63 - __clzll(heads & upto);             // head of this lane's run
+    int base = 0;
+    if (head && act) {
+        const unsigned long long rest = lane == 63 ? 0ull : (heads >> (lane + 1));
+        const int next = rest ? lane + __ffsll((long long)rest) : 64;
+        base = atomicAdd(&counter[bin], next - lane);
+    }
+    base = __shfl(base, s, 64);
+    return base + (lane - s);
+}
+
 // Pass 1: block b counts the bins of points [b*chunk, (b+1)*chunk) in an LDS
-// histogram (integer LDS atomics); rank[i] is the point's rank inside its
-// (block, bin).  hist is [nblk][nbins].
+// histogram (integer LDS atomics, one per run of equal bins in a wave);
+// rank[i] is the point's rank inside its (block, bin).  hist is [nblk][nbins].
+constexpr int kHistThreads = 1024;   // 16 waves per block: 4 per SIMD to hide the load latency
 template <typename T>
-__global__ __launch_bounds__(256) void k_hist(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+__global__ __launch_bounds__(kHistThreads) void k_hist(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
                                               int L1, int L2, T inv1, T inv2, int nb1, int nbins,
                                               int* __restrict__ hist, int* __restrict__ rank)
 {
     extern __shared__ int h[];
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0;
     __syncthreads();
+    // batches of kHistBatch points per thread, all loads of a batch in flight
+    // together (a loop with one load per iteration pays the full memory
+    // latency every time)
+    constexpr int kHistBatch = 4;
     const int lo = blockIdx.x * chunk, hi = min(lo + chunk, M);
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        int m1, m2;
-        T t1, t2;
-        const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
-        rank[i] = atomicAdd(&h[bin], 1);
+    for (int b = lo; b < hi; b += kHistBatch * kHistThreads) {
+        T xv[kHistBatch], yv[kHistBatch];
+#pragma unroll
+        for (int q = 0; q < kHistBatch; ++q) {
+            const int i = b + q * kHistThreads + (int)threadIdx.x;
+            xv[q] = i < hi ? x[i] : (T)0;
+            yv[q] = i < hi ? y[i] : (T)0;
+        }
+#pragma unroll
+        for (int q = 0; q < kHistBatch; ++q) {
+            const int i = b + q * kHistThreads + (int)threadIdx.x;
+            const bool act = i < hi;
+            int m1, m2;
+            T t1, t2;
+            const int bin = bin_of(xv[q], yv[q], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+            const int r = wave_run_rank(h, bin, act);
+            if (act) rank[i] = r;
+        }
     }
     __syncthreads();
     int* out = hist + (size_t)blockIdx.x * nbins;
@@ -383,25 +425,34 @@ __global__ __launch_bounds__(256) void k_hist(const T* __restrict__ x, const T* 
 }
 
 // Pass 1 for grids with more than kMaxLdsBins bins: one logical block, global
-// integer atomics.
+// integer atomics (one per run of equal bins in a wave).
 template <typename T>
 __global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ x, const T* __restrict__ y, int M, int L1,
                                                      int L2, T inv1, T inv2, int nb1, int* __restrict__ hist,
                                                      int* __restrict__ rank)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    int m1, m2;
-    T t1, t2;
-    const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
-    rank[i] = atomicAdd(&hist[bin], 1);
+    const bool act = i < M;
+    int bin = 0;
+    if (act) {
+        int m1, m2;
+        T t1, t2;
+        bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+    }
+    const int r = wave_run_rank(hist, bin, act);
+    if (act) rank[i] = r;
 }
 
 // Pass 2: per bin, turn hist[blk][bin] into the offset of (blk, bin) inside
 // the bin and write the bin total.  A block covers 16 bins x 16 slices of the
 // block range; each thread's loads are independent and coalesced across bins.
+// The block also writes, for its group of 16 bins, the exclusive prefix of
+// points and subproblems inside the group (gexcl, as int2) and the group sums
+// (gsum[blockIdx.x], as int2).
 constexpr int kColBins = 16, kColSlices = 16;
-__global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int* __restrict__ tot)
+__global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int maxpts,
+                                                 int* __restrict__ tot, int2* __restrict__ gexcl,
+                                                 int2* __restrict__ gsum)
 {
     __shared__ int part[kColSlices][kColBins + 1];
     const int bl = threadIdx.x % kColBins, q = threadIdx.x / kColBins;
@@ -427,44 +478,52 @@ __global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nbl
             run += t;
         }
         if (bin < nbins) tot[bin] = run;
+        // exclusive prefix over the 16 bins of the group (lanes 0..15 of wave 0)
+        const int ns = (run + maxpts - 1) / maxpts;
+        int ia = run, ib = ns;
+#pragma unroll
+        for (int o = 1; o < kColBins; o <<= 1) {
+            const int na = __shfl_up(ia, o, kColBins), nb = __shfl_up(ib, o, kColBins);
+            if (bl >= o) { ia += na; ib += nb; }
+        }
+        if (bin < nbins) gexcl[bin] = make_int2(ia - run, ib - ns);
+        if (bl == kColBins - 1) gsum[blockIdx.x] = make_int2(ia, ib);
     }
     __syncthreads();
-    if (bin >= nbins) return;
-    int run = part[q][bl];
+    if (bin < nbins) {
+        int run = part[q][bl];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int kb = lo + k;
-        if (kb < hi) {
-            hist[(size_t)kb * nbins + bin] = run;
-            run += v[k];
+        for (int k = 0; k < 16; ++k) {
+            const int kb = lo + k;
+            if (kb < hi) {
+                hist[(size_t)kb * nbins + bin] = run;
+                run += v[k];
+            }
         }
-    }
-    for (int kb = lo + 16; kb < hi; ++kb) {
-        const int t = hist[(size_t)kb * nbins + bin];
-        hist[(size_t)kb * nbins + bin] = run;
-        run += t;
+        for (int kb = lo + 16; kb < hi; ++kb) {
+            const int t = hist[(size_t)kb * nbins + bin];
+            hist[(size_t)kb * nbins + bin] = run;
+            run += t;
+        }
     }
 }
 
-// Pass 3 (one block): exclusive scan of the bin totals -> binstart, the first
-// subproblem of every bin (subbeg), and the subproblem list (bin, start,
-// count <= maxpts) in bin order.  Bins are scanned in rounds of 1024 with
-// coalesced loads.
+// Pass 2b (one block): exclusive scan of the group sums in place, in rounds of
+// kScanThreads groups; the totals go to nsub[0] and subbeg[nbins] (number of
+// subproblems).  (A last-block scan inside k_colscan would need a device-scope
+// fence in every block, which writes back L2 and costs more than this launch.)
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan_bins(const int* __restrict__ tot, int nbins, int maxpts,
-                                                            int* __restrict__ binstart, int* __restrict__ subbeg,
-                                                            int4* __restrict__ sub, int* __restrict__ nsub)
+__global__ __launch_bounds__(kScanThreads) void k_scan_groups(int2* __restrict__ gsum, int ngroups, int nbins,
+                                                              int* __restrict__ subbeg, int* __restrict__ nsub)
 {
     __shared__ int w_pts[kScanThreads / 64], w_sub[kScanThreads / 64];
     __shared__ int carry[2];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) { carry[0] = 0; carry[1] = 0; }
-    __syncthreads();
-    for (int base = 0; base < nbins; base += kScanThreads) {
-        const int i = base + t;
-        const int c = i < nbins ? tot[i] : 0;
-        const int ns = (c + maxpts - 1) / maxpts;
-        int ia = c, ib = ns;
+    int cpts = 0, csub = 0;
+    for (int base = 0; base < ngroups; base += kScanThreads) {
+        const int g = base + t;
+        const int2 v = g < ngroups ? gsum[g] : make_int2(0, 0);
+        int ia = v.x, ib = v.y;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
@@ -472,39 +531,49 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_bins(const int* __restric
         }
         if (lane == 63) { w_pts[w] = ia; w_sub[w] = ib; }
         __syncthreads();
-        int ba = carry[0], bb = carry[1];
-        for (int k = 0; k < w; ++k) { ba += w_pts[k]; bb += w_sub[k]; }
-        const int pos = ba + ia - c, sp = bb + ib - ns;
-        if (i < nbins) {
-            binstart[i] = pos;
-            subbeg[i] = sp;
-            for (int k = 0; k < ns; ++k) sub[sp + k] = make_int4(i, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
-        }
+        int pa = cpts + ia - v.x, pb = csub + ib - v.y;
+        for (int k = 0; k < w; ++k) { pa += w_pts[k]; pb += w_sub[k]; }
+        if (g < ngroups) gsum[g] = make_int2(pa, pb);
+        if (t == kScanThreads - 1) { carry[0] = pa + v.x; carry[1] = pb + v.y; }
         __syncthreads();
-        if (t == kScanThreads - 1) { carry[0] = pos + c; carry[1] = sp + ns; }
+        cpts = carry[0];
+        csub = carry[1];
         __syncthreads();
     }
     if (t == 0) {
-        *nsub = carry[1];
-        subbeg[nbins] = carry[1];
+        *nsub = csub;
+        subbeg[nbins] = csub;
     }
 }
 
-// Pass 5: scatter the points to their sorted positions, storing what the
+// Pass 3: scatter the points to their sorted positions, storing what the
 // gather and spread need: the fractional offsets t = c x - rint(c x) (exact),
-// the packed cell (mu2 << 16 | mu1) and the original index.
+// the packed cell (mu2 << 16 | mu1) and the original index.  The first nbins
+// threads also write their bin's first subproblem (subbeg) and subproblems
+// (bin, start, count <= maxpts), in bin order.  A bin starts at
+// gsum[bin / kColBins].x + gexcl[bin].x.
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
-                                                 int L1, int L2, T inv1, T inv2, int nb1, int nbins,
-                                                 const int* __restrict__ off, const int* __restrict__ binstart,
-                                                 const int* __restrict__ rank, PtRec<T>* __restrict__ pts)
+                                                 int L1, int L2, T inv1, T inv2, int nb1, int nbins, int maxpts,
+                                                 const int* __restrict__ off, const int* __restrict__ tot,
+                                                 const int2* __restrict__ gexcl, const int2* __restrict__ gsum,
+                                                 const int* __restrict__ rank, PtRec<T>* __restrict__ pts,
+                                                 int* __restrict__ subbeg, int4* __restrict__ sub)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nbins) {
+        const int c = tot[i];
+        const int2 e = gexcl[i], gb = gsum[i / kColBins];
+        const int pos = gb.x + e.x, sp = gb.y + e.y;
+        subbeg[i] = sp;
+        const int ns = (c + maxpts - 1) / maxpts;
+        for (int k = 0; k < ns; ++k) sub[sp + k] = make_int4(i, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
+    }
     if (i >= M) return;
     int m1, m2;
     T t1, t2;
     const int bin = bin_of(x[i], y[i], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
-    const int p = binstart[bin] + off[(size_t)(i / chunk) * nbins + bin] + rank[i];
+    const int p = gsum[bin / kColBins].x + gexcl[bin].x + off[(size_t)(i / chunk) * nbins + bin] + rank[i];
     PtRec<T> rec;
     rec.t1 = t1;
     rec.t2 = t2;
@@ -574,19 +643,31 @@ __device__ __forceinline__ int wrap1(int g, int L) { return g < 0 ? g + L : (g >
 #endif
 constexpr int kTileLd = NFFT2D_TILE_LD;     // LDS row stride of a gather tile (complex elements)
 
+// Launched with 256 threads; the 784 loads of a thread group are issued
+// together (a thread has at most 4).
 template <typename C>
 __device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
 {
     constexpr int W = kBin + 2 * kK;
-    if (L1 >= W && L2 >= W) {
-        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    constexpr int kPer = (W * W + 255) / 256;
+    C v[kPer];
+    const bool big = L1 >= W && L2 >= W;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        if (e < W * W) {
             const int r = e / W, c = e - r * W;
-            tile[r * kTileLd + c] = Gt[(size_t)wrap1(o2 + r, L2) * L1 + wrap1(o1 + c, L1)];
+            const int g2 = big ? wrap1(o2 + r, L2) : wrap(o2 + r, L2);
+            const int g1 = big ? wrap1(o1 + c, L1) : wrap(o1 + c, L1);
+            v[q] = Gt[(size_t)g2 * L1 + g1];
         }
-    } else {
-        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        if (e < W * W) {
             const int r = e / W, c = e - r * W;
-            tile[r * kTileLd + c] = Gt[(size_t)wrap(o2 + r, L2) * L1 + wrap(o1 + c, L1)];
+            tile[r * kTileLd + c] = v[q];
         }
     }
 }
@@ -908,14 +989,29 @@ __global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__
         end[j] = in ? subbeg[nb + 1] : 0;
         off[j] = r * kHalo + c;
     }
+    // tiles are summed in (neighbour, subproblem) order; the loads go out in
+    // groups of 8 so that a bin with many subproblems does not pay the memory
+    // latency once per tile
     float sr = 0.f, si = 0.f;
 #pragma unroll
-    for (int j = 0; j < 9; ++j)
-        for (int s = beg[j]; s < end[j]; ++s) {
+    for (int j = 0; j < 9; ++j) {
+        int s = beg[j];
+        for (; s + 8 <= end[j]; s += 8) {
+            float2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = scratch[(size_t)(s + q) * kTileCells + off[j]];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sr += v[q].x;
+                si += v[q].y;
+            }
+        }
+        for (; s < end[j]; ++s) {
             const float2 v = scratch[(size_t)s * kTileCells + off[j]];
             sr += v.x;
             si += v.y;
         }
+    }
     Gt[(size_t)n2 * geo.L1 + n1] = make_float2(sr, si);
 }
 
