@@ -175,6 +175,40 @@ int nfft2d_ndft_eval(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const float *x,
 int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x,
                       const float *y, int M, void *stream);
 
+/* ------------------------------------------------------------------ batches */
+
+/* Many complex64 type-2 transforms of different sizes (K = 6, c = 2; N1, N2
+ * powers of two in [8, 1024]; any M), run as ONE kernel launch per stage over
+ * all of them: nfft2d_batch_setpts is 5 launches and nfft2d_batch_type2 is 3,
+ * whatever the number of transforms.  Neither allocates nor records a
+ * hipMemset, so both can be captured into one hipGraph and replayed.  There is
+ * no reference counterpart: the reference runs one NFFT2_2D_GPU call per
+ * transform (e.g. once per particle per generation in
+ * Synthesis/PSO/PSO.cu:348).
+ *   create       N1[t], N2[t], M[t] for t < count (host arrays); allocates
+ *                every transform's plan and point buffers;
+ *   set_points   x[t], y[t]: device coordinate arrays (host array of device
+ *                pointers), kept by reference;
+ *   set_io       modes[t] (N1 x N2, layout of NFFT1_2D_GPU_c64) and result[t]
+ *                (M[t]): device arrays, kept by reference;
+ *   setpts       tile-sort the points of every transform;
+ *   type2        result[t] = type-2 of modes[t] at the sorted points;
+ *   autotune     choose the subproblem size (64, 128 or 256 points) per size
+ *                class (N1, N2) by timing reps setpts + type2 passes of the
+ *                whole batch per candidate; synchronizes the stream;
+ *   set_maxpts / get_maxpts  the subproblem size of transform t ([64, 256]). */
+typedef struct nfft2d_batch_s nfft2d_batch;
+int nfft2d_batch_create(nfft2d_batch **batch, int count, const int *N1, const int *N2, const int *M, unsigned flags);
+int nfft2d_batch_destroy(nfft2d_batch *batch);
+int nfft2d_batch_count(const nfft2d_batch *batch);
+int nfft2d_batch_set_points(nfft2d_batch *batch, const float *const *x, const float *const *y);
+int nfft2d_batch_set_io(nfft2d_batch *batch, const nfft2d_c64 *const *modes, nfft2d_c64 *const *result);
+int nfft2d_batch_setpts(nfft2d_batch *batch, void *stream);
+int nfft2d_batch_type2(nfft2d_batch *batch, void *stream);
+int nfft2d_batch_autotune(nfft2d_batch *batch, void *stream, int reps);
+int nfft2d_batch_set_maxpts(nfft2d_batch *batch, int t, int maxpts);
+int nfft2d_batch_get_maxpts(const nfft2d_batch *batch, int t);
+
 #ifdef __cplusplus
 }
 #endif

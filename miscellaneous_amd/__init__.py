@@ -4,7 +4,9 @@ A from-scratch HIP implementation of the 2D NFFT path of
 OrangeOwlSolutions/Miscellaneous (``NFFT/NFFT2_2D.cu``, ``NFFT/NDFT2_2D.cu``
 and the MATLAB ``BP_FBP_FFBP/NFFT1_2D.m``), behind the reference's call
 signatures (``include/nfft2d.h``), plus a differentiable type-2 transform
-and the type-2 NDFT as a split-precision MFMA contraction (``NdftPlan``).
+the type-2 NDFT as a split-precision MFMA contraction (``NdftPlan``) and
+batches of type-2 transforms of different sizes in one launch per stage
+(``Batch``).
 
 Everything here runs through ``libnfft2d.so`` (built in-tree by
 ``miscellaneous_amd.build``); there is no CPU fallback.
@@ -14,7 +16,8 @@ from .nfft import (Plan, NFFT2_2D_GPU, NFFT1_2D_GPU, NDFT2_2D_GPU, Calculate_cuF
                    Calculate_cuFFT_plan_Z2Z_NFFT2_2D, Destroy_cuFFT_plan_NFFT2_2D)
 from .autograd import NFFT2DType2, nfft2d_type2
 from .ndft import NdftPlan
+from .batch import Batch
 
 __all__ = ["NFFT2DError", "load_library", "LIB_PATH", "Plan", "NFFT2_2D_GPU", "NFFT1_2D_GPU", "NDFT2_2D_GPU",
            "Calculate_cuFFT_plan_C2C_NFFT2_2D", "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D",
-           "NFFT2DType2", "nfft2d_type2", "NdftPlan"]
+           "NFFT2DType2", "nfft2d_type2", "NdftPlan", "Batch"]

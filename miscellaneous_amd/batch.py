@@ -1,0 +1,102 @@
+"""Batched type-2 NFFTs of different sizes, over the C ABI (nfft2d_batch_*).
+
+``Batch(sizes, device)`` with ``sizes = [(N1, N2, M), ...]`` builds one plan
+per transform (complex64, K = 6, c = 2; N1, N2 powers of two in [8, 1024]).
+Every stage then runs as one kernel launch over all transforms:
+``setpts()`` is 5 launches and ``type2()`` 3, so both can be captured into
+one graph.  Arrays are kept by reference (``set_points`` / ``set_io``): keep
+the tensors alive while the batch uses them.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check
+
+__all__ = ["Batch"]
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _check(t: torch.Tensor, dtype, numel: int, name: str, device) -> None:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.device != device:
+        raise ValueError(f"{name} must be a tensor on {device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() != numel:
+        raise ValueError(f"{name} must have {numel} elements, got {t.numel()}")
+
+
+class Batch:
+    """Many complex64 type-2 NFFTs of different sizes, one launch per stage."""
+
+    def __init__(self, sizes, device=None):
+        self.sizes = [(int(a), int(b), int(m)) for a, b, m in sizes]
+        self.count = len(self.sizes)
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self._lib = _lib.load_library()
+        n = self.count
+        arr = ctypes.c_int * max(n, 1)
+        N1 = arr(*[s[0] for s in self.sizes])
+        N2 = arr(*[s[1] for s in self.sizes])
+        M = arr(*[s[2] for s in self.sizes])
+        self._h = ctypes.c_void_p()
+        self._keep = ()
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_create(ctypes.byref(self._h), n, N1, N2, M, 0), "nfft2d_batch_create")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.nfft2d_batch_destroy(h)
+            self._h = None
+
+    def _ptrs(self, tensors):
+        return (ctypes.c_void_p * max(self.count, 1))(*[ctypes.c_void_p(t.data_ptr()) for t in tensors])
+
+    def set_points(self, xs, ys) -> "Batch":
+        if len(xs) != self.count or len(ys) != self.count:
+            raise ValueError("one x and one y tensor per transform")
+        for t, (x, y) in enumerate(zip(xs, ys)):
+            _check(x, torch.float32, self.sizes[t][2], f"x[{t}]", self.device)
+            _check(y, torch.float32, self.sizes[t][2], f"y[{t}]", self.device)
+        self._pts = (list(xs), list(ys))
+        check(self._lib.nfft2d_batch_set_points(self._h, self._ptrs(xs), self._ptrs(ys)), "nfft2d_batch_set_points")
+        return self
+
+    def set_io(self, modes, results) -> "Batch":
+        if len(modes) != self.count or len(results) != self.count:
+            raise ValueError("one mode array and one result array per transform")
+        for t, (f, r) in enumerate(zip(modes, results)):
+            _check(f, torch.complex64, self.sizes[t][0] * self.sizes[t][1], f"modes[{t}]", self.device)
+            _check(r, torch.complex64, self.sizes[t][2], f"results[{t}]", self.device)
+        self._io = (list(modes), list(results))
+        check(self._lib.nfft2d_batch_set_io(self._h, self._ptrs(modes), self._ptrs(results)), "nfft2d_batch_set_io")
+        return self
+
+    def setpts(self) -> None:
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_setpts(self._h, _stream(self.device)), "nfft2d_batch_setpts")
+
+    def type2(self) -> None:
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_type2(self._h, _stream(self.device)), "nfft2d_batch_type2")
+
+    def autotune(self, reps: int = 3) -> list:
+        """Choose the subproblem size per size class by timing; returns the per-transform choice."""
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_autotune(self._h, _stream(self.device), int(reps)), "nfft2d_batch_autotune")
+        return self.maxpts()
+
+    def maxpts(self) -> list:
+        return [self._lib.nfft2d_batch_get_maxpts(self._h, t) for t in range(self.count)]
+
+    def set_maxpts(self, t: int, maxpts: int) -> None:
+        check(self._lib.nfft2d_batch_set_maxpts(self._h, int(t), int(maxpts)), "nfft2d_batch_set_maxpts")

@@ -306,8 +306,19 @@ static int zero_fill(void *ptr, long long n, hipStream_t s)
     return NFFT2D_OK;
 }
 
+// Blocking of setpts pass 1 for M points: LDS histograms for up to
+// kMaxLdsBins bins (nblk blocks of chunk points), else one logical block with
+// global atomics.
+static void setpts_blocking(const nfft2d_plan *p, int M, int &nblk, int &chunk)
+{
+    const bool lds = p->nbins <= kMaxLdsBins;
+    nblk = lds ? (int)std::max<long long>(1, std::min<long long>(256, ((long long)M + 2047) / 2048)) : 1;
+    chunk = std::max(1, (M + nblk - 1) / nblk);
+}
+
+// Point-dependent buffers of the plan for M points (grown, never shrunk).
 template <typename T>
-static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_t s)
+static int ensure_point_buffers(nfft2d_plan *p, int M)
 {
     const int subcap = p->nbins + (M + p->maxpts - 1) / p->maxpts;
     if (M > p->Mcap || subcap > p->subcap) {
@@ -323,11 +334,8 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
         p->Mcap = cap;
         p->subcap = std::max(subcap, 1);
     }
-    // blocking of pass 1: LDS histograms for up to kMaxLdsBins bins, else one
-    // logical block with global atomics
-    const bool lds = p->nbins <= kMaxLdsBins;
-    const int nblk = lds ? (int)std::max<long long>(1, std::min<long long>(256, ((long long)M + 2047) / 2048)) : 1;
-    const int chunk = std::max(1, (M + nblk - 1) / nblk);
+    int nblk, chunk;
+    setpts_blocking(p, M, nblk, chunk);
     const long long hneed = (long long)nblk * p->nbins;
     if (hneed > p->histcap) {
         free_hist(p);
@@ -337,6 +345,17 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
         }
         p->histcap = (int)hneed;
     }
+    return NFFT2D_OK;
+}
+
+template <typename T>
+static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_t s)
+{
+    int rc = ensure_point_buffers<T>(p, M);
+    if (rc) return rc;
+    int nblk, chunk;
+    setpts_blocking(p, M, nblk, chunk);
+    const bool lds = p->nbins <= kMaxLdsBins;
     const T inv1 = (T)(1.0 / p->L1), inv2 = (T)(1.0 / p->L2);
     if (lds) {
         hipLaunchKernelGGL(k_hist<T>, dim3(nblk), dim3(kHistThreads), sizeof(int) * p->nbins, s, x, y, M, chunk, p->L1,
@@ -699,4 +718,271 @@ extern "C" void Destroy_cuFFT_plan_NFFT2_2D(void)
     (void)hipDeviceSynchronize();
     for (auto &kv : g_cache) free_plan(kv.second);
     g_cache.clear();
+}
+
+// ============================================================================
+// Batches: many complex64 type-2 NFFTs of different sizes, one launch per stage
+// (kernels in nfft2d_batch.hpp).
+#include "nfft2d_batch.hpp"
+
+struct nfft2d_batch_s {
+    int count = 0, device = 0;
+    std::vector<nfft2d_plan *> plans;
+    std::vector<BatchDesc> hdesc;
+    BatchDesc *ddesc = nullptr;
+    int2 *own = nullptr;                  // owner tables of the 5 per-transform stages, back to back
+    int off_hist = 0, n_hist = 0, off_col = 0, n_col = 0, off_scat = 0, n_scat = 0;
+    int off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0;
+    int *base = nullptr;                  // count + 1: subproblem offsets, base[count] = total
+    int4 *bsub = nullptr;                 // the batch's subproblem list
+    long long bsubcap = 0;
+    int max_nbins = 0;
+    bool have_points = false, have_io = false;
+};
+
+static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
+static int batch_cols_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
+static constexpr int kBatchMinMaxpts = 64;   // smallest subproblem size the autotuner tries
+static constexpr int kBatchGatherBlocks = 2048;
+
+static void batch_free(nfft2d_batch *b)
+{
+    if (!b) return;
+    for (auto *p : b->plans) nfft2d_plan_destroy(p);
+    hfree(b->ddesc); hfree(b->own); hfree(b->base); hfree(b->bsub);
+    delete b;
+}
+
+// desc[t] from the plan (sizes, buffers); the caller's pointers are kept.
+static void batch_fill_desc(nfft2d_batch *b, int t)
+{
+    nfft2d_plan *p = b->plans[t];
+    BatchDesc &d = b->hdesc[t];
+    d.N1 = p->N1; d.N2 = p->N2; d.L1 = p->L1; d.L2 = p->L2; d.nb1 = p->nb1; d.nbins = p->nbins;
+    d.maxpts = p->maxpts;
+    setpts_blocking(p, d.M, d.nblk, d.chunk);
+    d.ngroups = (int)grid1d(p->nbins, kColBins);
+    d.inv1 = (float)(1.0 / p->L1); d.inv2 = (float)(1.0 / p->L2);
+    d.geo = p->geo;
+    d.Tm = (float2 *)p->tmp; d.Gt = (float2 *)p->grid;
+    d.D1 = (const float *)p->D1; d.D2 = (const float *)p->D2;
+    d.tw1 = (const float2 *)p->tw1; d.tw2 = (const float2 *)p->tw2;
+    d.hist = p->hist; d.rank = p->rank; d.tot = p->tot; d.subbeg = p->subbeg; d.nsub = p->nsub;
+    d.gexcl = p->gexcl; d.gsum = p->gsum; d.sub = p->sub; d.pts = (PtRec<float> *)p->pts;
+}
+
+static int batch_upload(nfft2d_batch *b)
+{
+    NFFT2D_HIP_TRY(hipMemcpy(b->ddesc, b->hdesc.data(), sizeof(BatchDesc) * b->count, hipMemcpyHostToDevice));
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1, const int *N2, const int *M,
+                                   unsigned flags)
+{
+    if (!out) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: null batch pointer");
+    *out = nullptr;
+    if (count <= 0 || !N1 || !N2 || !M) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: count <= 0 or null sizes");
+    if (flags != 0) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: flags must be 0");
+    for (int t = 0; t < count; ++t) {
+        if (!is_pow2(N1[t]) || !is_pow2(N2[t]) || N1[t] < 8 || N2[t] < 8 || N1[t] > kBatchMaxN || N2[t] > kBatchMaxN)
+            return nfft2d_fail(NFFT2D_ERR_SIZE, "batch_create: transform %d: N1=%d N2=%d must be powers of two in [8, %d]",
+                               t, N1[t], N2[t], kBatchMaxN);
+        if (M[t] < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: transform %d: M < 0", t);
+    }
+    nfft2d_batch *b = new (std::nothrow) nfft2d_batch_s();
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_create: host allocation failed");
+    b->count = count;
+    (void)hipGetDevice(&b->device);
+    b->plans.assign(count, nullptr);
+    b->hdesc.assign(count, BatchDesc{});
+    std::vector<int2> own;
+    long long subcap = 0;
+    for (int t = 0; t < count; ++t) {
+        int rc = nfft2d_plan_create(&b->plans[t], N1[t], N2[t], NFFT2D_C64, 6, 2, 0);
+        if (rc) { batch_free(b); return rc; }
+        nfft2d_plan *p = b->plans[t];
+        // buffers for M points at the smallest subproblem size the autotuner may pick
+        p->maxpts = kBatchMinMaxpts;
+        rc = ensure_point_buffers<float>(p, M[t]);
+        p->maxpts = kGatherThreads;
+        if (rc) { batch_free(b); return rc; }
+        b->hdesc[t].M = M[t];
+        batch_fill_desc(b, t);
+        b->max_nbins = std::max(b->max_nbins, p->nbins);
+        subcap += p->nbins + (M[t] + kBatchMinMaxpts - 1) / kBatchMinMaxpts;
+    }
+    auto add_stage = [&](int &off, int &n, auto blocks_of) {
+        off = (int)own.size();
+        for (int t = 0; t < b->count; ++t) {
+            const int nb = blocks_of(t);
+            for (int k = 0; k < nb; ++k) own.push_back(make_int2(t, k));
+        }
+        n = (int)own.size() - off;
+    };
+    add_stage(b->off_hist, b->n_hist, [&](int t) { return b->hdesc[t].nblk; });
+    add_stage(b->off_col, b->n_col, [&](int t) { return b->hdesc[t].ngroups; });
+    add_stage(b->off_scat, b->n_scat, [&](int t) {
+        return (int)grid1d(std::max(b->hdesc[t].M, b->hdesc[t].nbins), 256);
+    });
+    add_stage(b->off_rows, b->n_rows, [&](int t) {
+        return (int)grid1d(b->hdesc[t].N1, batch_rows_per_block(b->hdesc[t].N2));
+    });
+    add_stage(b->off_cols, b->n_cols, [&](int t) {
+        return (int)grid1d(b->hdesc[t].L2, batch_cols_per_block(b->hdesc[t].N1));
+    });
+    b->bsubcap = std::max<long long>(subcap, 1);
+    if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * count) != hipSuccess ||
+        hipMalloc(&b->own, sizeof(int2) * own.size()) != hipSuccess ||
+        hipMalloc(&b->base, sizeof(int) * (count + 1)) != hipSuccess ||
+        hipMalloc(&b->bsub, sizeof(int4) * b->bsubcap) != hipSuccess) {
+        (void)hipGetLastError();
+        batch_free(b);
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_create: device allocation failed");
+    }
+    if (hipMemcpy(b->own, own.data(), sizeof(int2) * own.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        batch_upload(b) != NFFT2D_OK) {
+        batch_free(b);
+        return nfft2d_fail(NFFT2D_ERR_HIP, "batch_create: upload of the batch tables failed");
+    }
+    *out = b;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_destroy(nfft2d_batch *b)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_destroy: null batch");
+    batch_free(b);
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_count(const nfft2d_batch *b) { return b ? b->count : nfft2d_fail(NFFT2D_ERR_ARG, "null batch"); }
+
+extern "C" int nfft2d_batch_set_points(nfft2d_batch *b, const float *const *x, const float *const *y)
+{
+    if (!b || !x || !y) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_points: null argument");
+    for (int t = 0; t < b->count; ++t) {
+        if (b->hdesc[t].M > 0 && (!x[t] || !y[t]))
+            return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_points: transform %d: null coordinates", t);
+        b->hdesc[t].x = x[t];
+        b->hdesc[t].y = y[t];
+    }
+    b->have_points = true;
+    return batch_upload(b);
+}
+
+extern "C" int nfft2d_batch_set_io(nfft2d_batch *b, const nfft2d_c64 *const *modes, nfft2d_c64 *const *result)
+{
+    if (!b || !modes || !result) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_io: null argument");
+    for (int t = 0; t < b->count; ++t) {
+        if (!modes[t] || (b->hdesc[t].M > 0 && !result[t]))
+            return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_io: transform %d: null array", t);
+        b->hdesc[t].f = reinterpret_cast<const float2 *>(modes[t]);
+        b->hdesc[t].out = reinterpret_cast<float2 *>(result[t]);
+    }
+    b->have_io = true;
+    return batch_upload(b);
+}
+
+extern "C" int nfft2d_batch_set_maxpts(nfft2d_batch *b, int t, int maxpts)
+{
+    if (!b || t < 0 || t >= b->count) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_maxpts: bad batch or index");
+    if (maxpts < kBatchMinMaxpts || maxpts > kGatherThreads)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_maxpts: maxpts must be in [%d, %d]", kBatchMinMaxpts,
+                           kGatherThreads);
+    b->plans[t]->maxpts = maxpts;
+    b->hdesc[t].maxpts = maxpts;
+    return batch_upload(b);
+}
+
+extern "C" int nfft2d_batch_get_maxpts(const nfft2d_batch *b, int t)
+{
+    if (!b || t < 0 || t >= b->count) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_get_maxpts: bad batch or index");
+    return b->hdesc[t].maxpts;
+}
+
+extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_setpts: null batch");
+    if (!b->have_points) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts: nfft2d_batch_set_points was not called");
+    hipStream_t s = (hipStream_t)stream;
+    if (b->n_hist > 0)
+        hipLaunchKernelGGL(kb_hist, dim3(b->n_hist), dim3(kHistThreads), sizeof(int) * b->max_nbins, s, b->ddesc,
+                           b->own + b->off_hist);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_colscan, dim3(b->n_col), dim3(256), 0, s, b->ddesc, b->own + b->off_col);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_scan_groups, dim3(b->count), dim3(kScanThreads), 0, s, b->ddesc);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_sub_prefix, dim3(1), dim3(1024), 0, s, b->ddesc, b->count, b->base);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_scatter, dim3(b->n_scat), dim3(256), 0, s, b->ddesc, b->own + b->off_scat, b->base, b->bsub);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type2: null batch");
+    if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: nfft2d_batch_set_io was not called");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(kb_t2_rows, dim3(b->n_rows), dim3(128), 0, s, b->ddesc, b->own + b->off_rows);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_t2_cols, dim3(b->n_cols), dim3(256), 0, s, b->ddesc, b->own + b->off_cols);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_gather, dim3(kBatchGatherBlocks), dim3(kGatherThreads), 0, s, b->ddesc, b->bsub,
+                       b->base + b->count, b->plans[0]->wa);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+// Subproblem size per size class (N1, N2), chosen by timing the whole batch
+// (setpts + type-2) with every candidate for one class at a time, the other
+// classes held fixed.  Synchronizes the stream; leaves the best sizes set.
+extern "C" int nfft2d_batch_autotune(nfft2d_batch *b, void *stream, int reps)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_autotune: null batch");
+    if (!b->have_points || !b->have_io)
+        return nfft2d_fail(NFFT2D_ERR_STATE, "batch_autotune: set the points and the input/output arrays first");
+    if (reps < 1) reps = 3;
+    hipStream_t s = (hipStream_t)stream;
+    static const int cand[] = {64, 128, 256};
+    std::map<std::pair<int, int>, std::vector<int>> classes;
+    for (int t = 0; t < b->count; ++t) classes[{b->hdesc[t].N1, b->hdesc[t].N2}].push_back(t);
+    hipEvent_t e0, e1;
+    NFFT2D_HIP_TRY(hipEventCreate(&e0));
+    NFFT2D_HIP_TRY(hipEventCreate(&e1));
+    int rc = NFFT2D_OK;
+    auto time_once = [&](float &ms) -> int {
+        int r = nfft2d_batch_setpts(b, s);
+        if (!r) r = nfft2d_batch_type2(b, s);      // warm-up
+        if (r) return r;
+        if (hipEventRecord(e0, s) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event record");
+        for (int k = 0; k < reps && !r; ++k) {
+            r = nfft2d_batch_setpts(b, s);
+            if (!r) r = nfft2d_batch_type2(b, s);
+        }
+        if (r) return r;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event timing failed");
+        return NFFT2D_OK;
+    };
+    for (auto &kv : classes) {
+        float best = 1e30f;
+        int best_mp = b->hdesc[kv.second[0]].maxpts;
+        for (int mp : cand) {
+            for (int t : kv.second) { b->plans[t]->maxpts = mp; b->hdesc[t].maxpts = mp; }
+            if ((rc = batch_upload(b))) break;
+            float ms = 0.f;
+            if ((rc = time_once(ms))) break;
+            if (ms < best) { best = ms; best_mp = mp; }
+        }
+        if (rc) break;
+        for (int t : kv.second) { b->plans[t]->maxpts = best_mp; b->hdesc[t].maxpts = best_mp; }
+        if ((rc = batch_upload(b))) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }

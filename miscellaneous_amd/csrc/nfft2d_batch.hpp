@@ -1,0 +1,201 @@
+// nfft2d_batch.hpp -- many type-2 NFFTs of different sizes in one launch per stage.
+//
+// A batch holds one plan per transform (complex64, K = 6, c = 2, N1 and N2
+// powers of two up to 1024).  Every stage of setpts and of the type-2 runs as
+// ONE kernel over the blocks of all transforms: block b finds its transform
+// and its block index inside the transform in an owner table built when the
+// batch is created (the block counts of these stages depend on the sizes
+// only).  The gather is persistent: its blocks walk one list of the
+// subproblems of all transforms, each entry tagged with its transform.
+// Together: 5 launches for setpts and 3 for the type-2, whatever the number
+// of transforms, all stream-ordered and free of allocation, so the whole
+// sequence can be captured into one hipGraph.
+#pragma once
+
+#include "nfft2d_kernels.hpp"
+
+namespace nfft2d {
+
+// Per-transform state, device-resident (one array per batch).
+struct BatchDesc {
+    int N1, N2, L1, L2, nb1, nbins, M, maxpts, nblk, chunk, ngroups;
+    float inv1, inv2;
+    BinGeom geo;
+    // plan buffers
+    float2 *Tm, *Gt;
+    const float *D1, *D2;
+    const float2 *tw1, *tw2;
+    int *hist, *rank, *tot, *subbeg, *nsub;
+    int2 *gexcl, *gsum;
+    int4* sub;
+    PtRec<float>* pts;
+    // the caller's arrays
+    const float *x, *y;
+    const float2* f;
+    float2* out;
+};
+
+// Largest N of the batched FFT passes.
+constexpr int kBatchMaxN = 1024;
+
+template <typename F>
+__device__ __forceinline__ void batch_dispatch_n(int N, F&& f)
+{
+    switch (N) {
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    case 256: f(std::integral_constant<int, 256>{}); break;
+    case 512: f(std::integral_constant<int, 512>{}); break;
+    case 1024: f(std::integral_constant<int, 1024>{}); break;
+    default: break;
+    }
+}
+
+template <int N> constexpr int cmax_rows_lds() { return t2_rows_lds<float, N>(); }
+constexpr int batch_rows_lds()
+{
+    int m = 0;
+    m = m > cmax_rows_lds<8>() ? m : cmax_rows_lds<8>();
+    m = m > cmax_rows_lds<16>() ? m : cmax_rows_lds<16>();
+    m = m > cmax_rows_lds<32>() ? m : cmax_rows_lds<32>();
+    m = m > cmax_rows_lds<64>() ? m : cmax_rows_lds<64>();
+    m = m > cmax_rows_lds<128>() ? m : cmax_rows_lds<128>();
+    m = m > cmax_rows_lds<256>() ? m : cmax_rows_lds<256>();
+    m = m > cmax_rows_lds<512>() ? m : cmax_rows_lds<512>();
+    m = m > cmax_rows_lds<1024>() ? m : cmax_rows_lds<1024>();
+    return m;
+}
+template <int N> constexpr int cmax_cols_lds() { return t2_cols_lds<float, N>(); }
+constexpr int batch_cols_lds()
+{
+    int m = 0;
+    m = m > cmax_cols_lds<8>() ? m : cmax_cols_lds<8>();
+    m = m > cmax_cols_lds<16>() ? m : cmax_cols_lds<16>();
+    m = m > cmax_cols_lds<32>() ? m : cmax_cols_lds<32>();
+    m = m > cmax_cols_lds<64>() ? m : cmax_cols_lds<64>();
+    m = m > cmax_cols_lds<128>() ? m : cmax_cols_lds<128>();
+    m = m > cmax_cols_lds<256>() ? m : cmax_cols_lds<256>();
+    m = m > cmax_cols_lds<512>() ? m : cmax_cols_lds<512>();
+    m = m > cmax_cols_lds<1024>() ? m : cmax_cols_lds<1024>();
+    return m;
+}
+
+// ---------------------------------------------------------------- setpts
+__global__ __launch_bounds__(kHistThreads) void kb_hist(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    extern __shared__ int h[];
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    hist_body<float>(o.y, h, d.x, d.y, d.M, d.chunk, d.L1, d.L2, d.inv1, d.inv2, d.nb1, d.nbins, d.hist, d.rank);
+}
+
+__global__ __launch_bounds__(256) void kb_colscan(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    colscan_body(o.y, d.hist, d.nblk, d.nbins, d.maxpts, d.tot, d.gexcl, d.gsum);
+}
+
+// one block per transform
+__global__ __launch_bounds__(kScanThreads) void kb_scan_groups(const BatchDesc* __restrict__ bd)
+{
+    const BatchDesc& d = bd[blockIdx.x];
+    scan_groups_body(d.gsum, d.ngroups, d.nbins, d.subbeg, d.nsub);
+}
+
+// (one block) base[t] = sum of nsub over the transforms before t; base[count] = total
+__global__ __launch_bounds__(1024) void kb_sub_prefix(const BatchDesc* __restrict__ bd, int count, int* __restrict__ base)
+{
+    __shared__ int w_sum[16], carry;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < count; b0 += 1024) {
+        const int i = b0 + t;
+        const int v = i < count ? *bd[i].nsub : 0;
+        int a = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int n = __shfl_up(a, o, 64);
+            if (lane >= o) a += n;
+        }
+        if (lane == 63) w_sum[w] = a;
+        __syncthreads();
+        int pre = carry + a - v;
+        for (int k = 0; k < w; ++k) pre += w_sum[k];
+        if (i < count) base[i] = pre;
+        __syncthreads();
+        if (t == 1023) carry = pre + v;
+        __syncthreads();
+    }
+    if (t == 0) base[count] = carry;
+}
+
+// Scatter of every transform's points, and each transform's subproblems into
+// the batch list: entry base[t] + s = (bin, start, count, t).
+__global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ bd, const int2* __restrict__ own,
+                                                  const int* __restrict__ base, int4* __restrict__ bsub)
+{
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    const int i = o.y * 256 + threadIdx.x;
+    if (i < d.nbins) {
+        const int c = d.tot[i];
+        const int2 e = d.gexcl[i], gb = d.gsum[i / kColBins];
+        const int pos = gb.x + e.x, sp = base[o.x] + gb.y + e.y;
+        const int ns = (c + d.maxpts - 1) / d.maxpts;
+        for (int k = 0; k < ns; ++k)
+            bsub[sp + k] = make_int4(i, pos + k * d.maxpts, min(d.maxpts, c - k * d.maxpts), o.x);
+    }
+    if (i >= d.M) return;
+    int m1, m2;
+    float t1, t2;
+    const int bin = bin_of(d.x[i], d.y[i], d.L1, d.L2, d.inv1, d.inv2, d.nb1, m1, m2, t1, t2);
+    const int p = d.gsum[bin / kColBins].x + d.gexcl[bin].x + d.hist[(size_t)(i / d.chunk) * d.nbins + bin] + d.rank[i];
+    PtRec<float> rec;
+    rec.t1 = t1;
+    rec.t2 = t2;
+    rec.cell = (m2 << 16) | m1;
+    rec.idx = i;
+    d.pts[p] = rec;
+}
+
+// ---------------------------------------------------------------- type-2
+__global__ __launch_bounds__(128) void kb_t2_rows(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    __shared__ float2 lds[batch_rows_lds()];
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    batch_dispatch_n(d.N2, [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        t2_rows_body<float, N, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    __shared__ float2 lds[batch_cols_lds()];
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    batch_dispatch_n(d.N1, [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        t2_cols_body<float, N, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.L2);
+    });
+}
+
+// Persistent gather over the batch's subproblem list (total = base[count]).
+__global__ __launch_bounds__(kGatherThreads) void kb_gather(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
+                                                            const int* __restrict__ total, WinArgs wa)
+{
+    const int n = *total;
+    for (int s = blockIdx.x; s < n; s += gridDim.x) {
+        const BatchDesc& d = bd[bsub[s].w];
+        gather_body<float, 0>(s, d.Gt, d.geo, d.pts, bsub, wa, d.out, nullptr, nullptr, nullptr, nullptr, nullptr);
+        __syncthreads();
+    }
+}
+
+}  // namespace nfft2d

@@ -76,19 +76,19 @@ __global__ void k_fill_zero(T* __restrict__ p, long long n)
 // n2 in [0, L2).  The window pre-compensation and zero padding of
 // BP_FBP_FFBP/NFFT1_2D.m:48-49 are applied while loading; the padding is
 // never materialised.
+template <typename T, int N> constexpr int t2_rows_lds() { return rows_per_block<N>() * 2 * lds_line<N>(); }
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t2_rows(
-    const cplx<T>* __restrict__ f, cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
-    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
+__device__ __forceinline__ void t2_rows_body(int blk, cplx<T>* lds, const cplx<T>* __restrict__ f,
+                                             cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
+                                             const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
 {
     using C = cplx<T>;
     constexpr int ROWS = rows_per_block<N>();
     constexpr int TPL = N / 8, LL = lds_line<N>(), V = 2;
     constexpr int R0 = FftPlan<N>::radix(0);
     constexpr int RL = FftPlan<N>::radix(FftPlan<N>::stages - 1);
-    __shared__ C lds[ROWS * V * LL];
     const int g = threadIdx.x / TPL, j = threadIdx.x % TPL;
-    const int i1 = blockIdx.x * ROWS + g;
+    const int i1 = blk * ROWS + g;
     const int row = i1 < N1 ? i1 : N1 - 1;
     const T d1 = D1[row];
     C v[V][8];
@@ -115,24 +115,31 @@ __global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t2_rows(
             }
     }
 }
+template <typename T, int N, int SIGN>
+__global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t2_rows(
+    const cplx<T>* __restrict__ f, cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
+    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
+{
+    __shared__ cplx<T> lds[t2_rows_lds<T, N>()];
+    t2_rows_body<T, N, SIGN>(blockIdx.x, lds, f, Tm, D1, D2, tw, N1);
+}
 
 // (b) column pass: Gt[n2][n1] = sum_{k1} T[k1][n2] e^{-2 pi i k1 n1 / L1}, n1 in [0, L1).
 // Loads B columns of T per block (B consecutive elements per row), writes B
 // contiguous rows of the transposed grid.
+template <typename T, int N> constexpr int t2_cols_lds() { return cols_per_block<N>() * 2 * lds_line<N>(); }
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
-    const cplx<T>* __restrict__ Tm, cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int L2)
+__device__ __forceinline__ void t2_cols_body(int blk, cplx<T>* lds, const cplx<T>* __restrict__ Tm,
+                                             cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int L2)
 {
     using C = cplx<T>;
     constexpr int B = cols_per_block<N>();
     constexpr int TPL = N / 8, LL = lds_line<N>(), V = 2;
     constexpr int R0 = FftPlan<N>::radix(0);
     constexpr int RL = FftPlan<N>::radix(FftPlan<N>::stages - 1);
-    __shared__ C lds[B * V * LL];
     const int t = threadIdx.x;
     const int bF = t % B, jF = t / B;      // first stage: B columns side by side
     const int bL = t / TPL, jL = t % TPL;  // last stage: whole lines
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int n2F = min(blk * B + bF, L2 - 1);
     C v[V][8];
 #pragma unroll
@@ -157,6 +164,13 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
                 store_pair(&out[2 * q], v[0][b * RL + r], v[1][b * RL + r]);
             }
     }
+}
+template <typename T, int N, int SIGN>
+__global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
+    const cplx<T>* __restrict__ Tm, cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int L2)
+{
+    __shared__ cplx<T> lds[t2_cols_lds<T, N>()];
+    t2_cols_body<T, N, SIGN>(xcd_block(blockIdx.x, gridDim.x), lds, Tm, Gt, tw, L2);
 }
 
 // ================================================================= type-1 passes
@@ -388,18 +402,17 @@ __device__ __forceinline__ int wave_run_rank(int* counter, int bin, bool act)
 // rank[i] is the point's rank inside its (block, bin).  hist is [nblk][nbins].
 constexpr int kHistThreads = 1024;   // 16 waves per block: 4 per SIMD to hide the load latency
 template <typename T>
-__global__ __launch_bounds__(kHistThreads) void k_hist(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
-                                              int L1, int L2, T inv1, T inv2, int nb1, int nbins,
-                                              int* __restrict__ hist, int* __restrict__ rank)
+__device__ __forceinline__ void hist_body(int blk, int* h, const T* __restrict__ x, const T* __restrict__ y, int M,
+                                          int chunk, int L1, int L2, T inv1, T inv2, int nb1, int nbins,
+                                          int* __restrict__ hist, int* __restrict__ rank)
 {
-    extern __shared__ int h[];
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0;
+    for (int i = threadIdx.x; i < nbins; i += kHistThreads) h[i] = 0;
     __syncthreads();
     // batches of kHistBatch points per thread, all loads of a batch in flight
     // together (a loop with one load per iteration pays the full memory
     // latency every time)
     constexpr int kHistBatch = 4;
-    const int lo = blockIdx.x * chunk, hi = min(lo + chunk, M);
+    const int lo = blk * chunk, hi = min(lo + chunk, M);
     for (int b = lo; b < hi; b += kHistBatch * kHistThreads) {
         T xv[kHistBatch], yv[kHistBatch];
 #pragma unroll
@@ -420,8 +433,16 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const T* __restrict__ x, 
         }
     }
     __syncthreads();
-    int* out = hist + (size_t)blockIdx.x * nbins;
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) out[i] = h[i];
+    int* out = hist + (size_t)blk * nbins;
+    for (int i = threadIdx.x; i < nbins; i += kHistThreads) out[i] = h[i];
+}
+template <typename T>
+__global__ __launch_bounds__(kHistThreads) void k_hist(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+                                                       int L1, int L2, T inv1, T inv2, int nb1, int nbins,
+                                                       int* __restrict__ hist, int* __restrict__ rank)
+{
+    extern __shared__ int h[];
+    hist_body<T>(blockIdx.x, h, x, y, M, chunk, L1, L2, inv1, inv2, nb1, nbins, hist, rank);
 }
 
 // Pass 1 for grids with more than kMaxLdsBins bins: one logical block, global
@@ -450,13 +471,12 @@ __global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ x, co
 // points and subproblems inside the group (gexcl, as int2) and the group sums
 // (gsum[blockIdx.x], as int2).
 constexpr int kColBins = 16, kColSlices = 16;
-__global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int maxpts,
-                                                 int* __restrict__ tot, int2* __restrict__ gexcl,
-                                                 int2* __restrict__ gsum)
+__device__ __forceinline__ void colscan_body(int blk, int* __restrict__ hist, int nblk, int nbins, int maxpts,
+                                             int* __restrict__ tot, int2* __restrict__ gexcl, int2* __restrict__ gsum)
 {
     __shared__ int part[kColSlices][kColBins + 1];
     const int bl = threadIdx.x % kColBins, q = threadIdx.x / kColBins;
-    const int bin = blockIdx.x * kColBins + bl;
+    const int bin = blk * kColBins + bl;
     const int per = (nblk + kColSlices - 1) / kColSlices;
     const int lo = min(q * per, nblk), hi = min(lo + per, nblk);
     int v[16];
@@ -487,7 +507,7 @@ __global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nbl
             if (bl >= o) { ia += na; ib += nb; }
         }
         if (bin < nbins) gexcl[bin] = make_int2(ia - run, ib - ns);
-        if (bl == kColBins - 1) gsum[blockIdx.x] = make_int2(ia, ib);
+        if (bl == kColBins - 1) gsum[blk] = make_int2(ia, ib);
     }
     __syncthreads();
     if (bin < nbins) {
@@ -508,13 +528,20 @@ __global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nbl
     }
 }
 
+__global__ __launch_bounds__(256) void k_colscan(int* __restrict__ hist, int nblk, int nbins, int maxpts,
+                                                 int* __restrict__ tot, int2* __restrict__ gexcl,
+                                                 int2* __restrict__ gsum)
+{
+    colscan_body(blockIdx.x, hist, nblk, nbins, maxpts, tot, gexcl, gsum);
+}
+
 // Pass 2b (one block): exclusive scan of the group sums in place, in rounds of
 // kScanThreads groups; the totals go to nsub[0] and subbeg[nbins] (number of
 // subproblems).  (A last-block scan inside k_colscan would need a device-scope
 // fence in every block, which writes back L2 and costs more than this launch.)
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan_groups(int2* __restrict__ gsum, int ngroups, int nbins,
-                                                              int* __restrict__ subbeg, int* __restrict__ nsub)
+__device__ __forceinline__ void scan_groups_body(int2* __restrict__ gsum, int ngroups, int nbins,
+                                                 int* __restrict__ subbeg, int* __restrict__ nsub)
 {
     __shared__ int w_pts[kScanThreads / 64], w_sub[kScanThreads / 64];
     __shared__ int carry[2];
@@ -545,6 +572,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_groups(int2* __restrict__
         subbeg[nbins] = csub;
     }
 }
+__global__ __launch_bounds__(kScanThreads) void k_scan_groups(int2* __restrict__ gsum, int ngroups, int nbins,
+                                                              int* __restrict__ subbeg, int* __restrict__ nsub)
+{
+    scan_groups_body(gsum, ngroups, nbins, subbeg, nsub);
+}
 
 // Pass 3: scatter the points to their sorted positions, storing what the
 // gather and spread need: the fractional offsets t = c x - rint(c x) (exact),
@@ -553,14 +585,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_groups(int2* __restrict__
 // (bin, start, count <= maxpts), in bin order.  A bin starts at
 // gsum[bin / kColBins].x + gexcl[bin].x.
 template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
-                                                 int L1, int L2, T inv1, T inv2, int nb1, int nbins, int maxpts,
-                                                 const int* __restrict__ off, const int* __restrict__ tot,
-                                                 const int2* __restrict__ gexcl, const int2* __restrict__ gsum,
-                                                 const int* __restrict__ rank, PtRec<T>* __restrict__ pts,
-                                                 int* __restrict__ subbeg, int4* __restrict__ sub)
+__device__ __forceinline__ void scatter_body(int blk, const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+                                             int L1, int L2, T inv1, T inv2, int nb1, int nbins, int maxpts,
+                                             const int* __restrict__ off, const int* __restrict__ tot,
+                                             const int2* __restrict__ gexcl, const int2* __restrict__ gsum,
+                                             const int* __restrict__ rank, PtRec<T>* __restrict__ pts,
+                                             int* __restrict__ subbeg, int4* __restrict__ sub)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * 256 + threadIdx.x;
     if (i < nbins) {
         const int c = tot[i];
         const int2 e = gexcl[i], gb = gsum[i / kColBins];
@@ -580,6 +612,17 @@ __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const 
     rec.cell = (m2 << 16) | m1;
     rec.idx = i;
     pts[p] = rec;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const T* __restrict__ y, int M, int chunk,
+                                                 int L1, int L2, T inv1, T inv2, int nb1, int nbins, int maxpts,
+                                                 const int* __restrict__ off, const int* __restrict__ tot,
+                                                 const int2* __restrict__ gexcl, const int2* __restrict__ gsum,
+                                                 const int* __restrict__ rank, PtRec<T>* __restrict__ pts,
+                                                 int* __restrict__ subbeg, int4* __restrict__ sub)
+{
+    scatter_body<T>(blockIdx.x, x, y, M, chunk, L1, L2, inv1, inv2, nb1, nbins, maxpts, off, tot, gexcl, gsum, rank,
+                    pts, subbeg, sub);
 }
 
 // ================================================================= window taps
@@ -700,11 +743,11 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
 // counting-sorts its points by cell in LDS so that neighbouring lanes read
 // neighbouring tile addresses.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kGatherThreads) void k_gather(
-    const cplx<T>* __restrict__ Gt, BinGeom geo, const PtRec<T>* __restrict__ pts,
-    const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, cplx<T>* __restrict__ out,
-    cplx<T>* __restrict__ out_d1, cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g, T* __restrict__ gx1,
-    T* __restrict__ gx2)
+__device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
+                                            const PtRec<T>* __restrict__ pts, const int4* __restrict__ sub,
+                                            const WinArgs& wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1,
+                                            cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g,
+                                            T* __restrict__ gx1, T* __restrict__ gx2)
 {
     using C = cplx<T>;
     constexpr int W = kTileLd;
@@ -713,8 +756,6 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
     __shared__ int s_warp[kGatherThreads / 64];
     __shared__ T s_f1[kGatherThreads], s_f2[kGatherThreads];
     __shared__ int s_cell[kGatherThreads], s_idx[kGatherThreads];
-    const int s = blockIdx.x;
-    if (s >= *nsub) return;
     const int4 d = sub[s];
     const int b1 = d.x % geo.nb1, b2 = d.x / geo.nb1;
     const int O1 = b1 * kBin, O2 = b2 * kBin;
@@ -747,7 +788,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
         s_idx[dst] = o;
     }
     __syncthreads();
-    if (!have) return;
+    if (!have) return;         // (no barrier follows in this function)
     f1 = s_f1[tid];
     f2 = s_f2[tid];
     cell = s_cell[tid];
@@ -806,6 +847,17 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
             gx2[o] = cc * (gg.x * cx + gg.y * cy);
         }
     }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kGatherThreads) void k_gather(
+    const cplx<T>* __restrict__ Gt, BinGeom geo, const PtRec<T>* __restrict__ pts,
+    const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, cplx<T>* __restrict__ out,
+    cplx<T>* __restrict__ out_d1, cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g, T* __restrict__ gx1,
+    T* __restrict__ gx2)
+{
+    if ((int)blockIdx.x >= *nsub) return;
+    gather_body<T, MODE>(blockIdx.x, Gt, geo, pts, sub, wa, out, out_d1, out_d2, g, gx1, gx2);
 }
 
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates

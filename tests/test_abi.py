@@ -93,6 +93,39 @@ def test_ndft_null_arguments_rejected(lib):
     assert lib.nfft2d_ndft_type2(None, None, None, None, None, 10, None) == -1
 
 
+def _ints(v):
+    return (ctypes.c_int * len(v))(*v)
+
+
+@pytest.mark.parametrize("N1,N2,M,code,needle", [
+    ([64, 48], [64, 64], [10, 10], -2, "powers of two"),
+    ([64, 2048], [64, 64], [10, 10], -2, "1024"),
+    ([64, 4], [64, 64], [10, 10], -2, "[8,"),
+    ([64, 64], [64, 64], [10, -1], -1, "M < 0"),
+])
+def test_batch_create_validation(lib, N1, N2, M, code, needle):
+    h = ctypes.c_void_p()
+    rc = lib.nfft2d_batch_create(ctypes.byref(h), len(N1), _ints(N1), _ints(N2), _ints(M), 0)
+    assert rc == code
+    assert needle in lib.nfft2d_last_error().decode()
+    assert not h.value
+
+
+def test_batch_null_arguments_rejected(lib):
+    h = ctypes.c_void_p()
+    one = _ints([64])
+    assert lib.nfft2d_batch_create(None, 1, one, one, one, 0) == -1
+    assert lib.nfft2d_batch_create(ctypes.byref(h), 0, one, one, one, 0) == -1
+    assert lib.nfft2d_batch_create(ctypes.byref(h), 1, one, one, one, 3) == -1
+    assert lib.nfft2d_batch_destroy(None) == -1
+    assert lib.nfft2d_batch_setpts(None, None) == -1
+    assert lib.nfft2d_batch_type2(None, None) == -1
+    assert lib.nfft2d_batch_autotune(None, None, 1) == -1
+    assert lib.nfft2d_batch_set_maxpts(None, 0, 128) == -1
+    assert lib.nfft2d_batch_get_maxpts(None, 0) == -1
+    assert lib.nfft2d_batch_count(None) == -1
+
+
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         _lib.load_library(str(tmp_path / "libnfft2d.so"))

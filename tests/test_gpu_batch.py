@@ -1,0 +1,131 @@
+"""GPU parity of the batched type-2 (one launch per stage over many sizes).
+
+Every transform of a batch must give exactly what its own plan gives (the
+batch runs the same per-point arithmetic, so the results are bitwise equal),
+and match the FP64 direct sum on seeded outputs to rel-L2 1e-5.  The batch
+mixes sizes, empty and tiny point sets, uniform and clustered points.
+"""
+import numpy as np
+import pytest
+import torch
+
+import miscellaneous_amd as m
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def cn01(rng, shape):
+    return ((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)).astype(np.complex64)
+
+
+def batch_points(rng, N1, N2, M):
+    """Half uniform in [-1/2, 1/2)^2, half Gaussian (sigma 0.05) wrapped, in grid units."""
+    h = M // 2
+    u = rng.random((h, 2)) - 0.5
+    g = rng.normal(0.0, 0.05, size=(M - h, 2))
+    g = (g + 0.5) % 1.0 - 0.5
+    p = np.concatenate([u, g]).astype(np.float32)
+    x1 = (p[:, 0] * np.float32(N1)).astype(np.float32)
+    x2 = (p[:, 1] * np.float32(N2)).astype(np.float32)
+    return x1, x2
+
+
+SIZES = [(64, 64, 5000), (8, 8, 37), (16, 64, 1), (128, 128, 20000), (256, 256, 0), (32, 16, 700),
+         (512, 512, 30000), (64, 128, 4096), (1024, 1024, 9000), (8, 1024, 300), (128, 64, 12345), (256, 256, 60000)]
+
+
+def make_batch(seed=5):
+    rng = np.random.default_rng(seed)
+    fs, xs, ys, outs = [], [], [], []
+    for (N1, N2, M) in SIZES:
+        fs.append(cn01(rng, (N1, N2)))
+        x1, x2 = batch_points(rng, N1, N2, M)
+        xs.append(x1)
+        ys.append(x2)
+    b = m.Batch(SIZES, device=DEV)
+    tf = [T(f) for f in fs]
+    tx = [T(x) for x in xs]
+    ty = [T(y) for y in ys]
+    to = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in SIZES]
+    b.set_points(tx, ty).set_io(tf, to)
+    return b, fs, xs, ys, tf, tx, ty, to
+
+
+def per_plan(f, x1, x2):
+    N1, N2 = f.shape
+    p = m.Plan(N1, N2, torch.complex64, device=DEV)
+    p.setpts(T(x1), T(x2))
+    return p.type2(T(f)).cpu().numpy()
+
+
+def test_batch_matches_plans_and_oracle():
+    b, fs, xs, ys, tf, tx, ty, to = make_batch()
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(9)
+    for t, (N1, N2, M) in enumerate(SIZES):
+        yb = to[t].cpu().numpy()
+        if M == 0:
+            continue
+        yp = per_plan(fs[t], xs[t], ys[t])
+        assert np.array_equal(yb, yp), f"transform {t} {SIZES[t]}: batch differs from its plan"
+        sel = np.sort(rng.choice(M, min(M, 256), replace=False))
+        ref = oracle.ndft_type2(fs[t].astype(np.complex128), xs[t][sel], ys[t][sel])
+        err = oracle.rel_l2(yb[sel], ref)
+        print(f"batch transform {t} {SIZES[t]}: rel-L2 {err:.2e}")
+        assert err <= 1e-5, (t, err)
+
+
+def test_batch_graph_replay():
+    b, fs, xs, ys, tf, tx, ty, to = make_batch(seed=6)
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in to]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        b.setpts()
+        b.type2()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.setpts()
+        b.type2()
+    for rep in range(3):
+        for o in to:
+            o.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for t in range(len(SIZES)):
+            assert torch.equal(to[t], eager[t]), (rep, t)
+
+
+def test_batch_autotune_keeps_results():
+    b, fs, xs, ys, tf, tx, ty, to = make_batch(seed=7)
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    before = [o.clone() for o in to]
+    choice = b.autotune(reps=2)
+    assert all(c in (64, 128, 256) for c in choice)
+    # one choice per size class
+    by_class = {}
+    for (N1, N2, _), c in zip(SIZES, choice):
+        by_class.setdefault((N1, N2), set()).add(c)
+    assert all(len(v) == 1 for v in by_class.values())
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    for t in range(len(SIZES)):
+        assert torch.equal(to[t], before[t]), t
