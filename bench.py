@@ -16,6 +16,11 @@ Secondary: fwd+adjoint gradient steps/s on 256 x 256 modes, M = 262,144
 uniform points (loss ||A(x) f - y||^2, gradient w.r.t. f and x, gradient
 descent update), also graph-replayed.
 
+Also reported (skip with --no-extra): the batch of 256 uneven type-2 NFFTs
+(batched setpts + type-2 in one graph, against the same transforms one plan at
+a time), the type-2 NDFT as an MFMA contraction in FP32 / BF16x3 / FP16 with
+its MFMA utilisation, and the 1024^2 / M=4M uniform type-2.
+
 cpu_baseline: the CPU restatement of the reference algorithm (FP32,
 OpenMP over the host cores) on the same type-2 workload, timed for a bounded
 number of repetitions.
@@ -206,6 +211,148 @@ def bench_grad(args, dev):
                 bytes=408 * N * N + 80 * M)
 
 
+def batch_sizes(seed: int, count: int = 256):
+    """The batch workload: N uniform in {64, 128, 256, 512} (square), M log-uniform in [4096, 262144]."""
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice([64, 128, 256, 512], size=count)
+    Ms = np.round(np.exp(rng.uniform(np.log(4096), np.log(262144), size=count))).astype(int)
+    return [(int(n), int(n), int(mm)) for n, mm in zip(Ns, Ms)], rng
+
+
+def batch_points(rng, N, M):
+    """Half uniform in [-1/2, 1/2)^2, half Gaussian (sigma 0.05) wrapped to [-1/2, 1/2); grid units."""
+    h = M // 2
+    u = rng.random((h, 2)) - 0.5
+    g = (rng.normal(0.0, 0.05, size=(M - h, 2)) + 0.5) % 1.0 - 0.5
+    p = np.concatenate([u, g]).astype(np.float32)
+    return (p[:, 0] * np.float32(N)).astype(np.float32), (p[:, 1] * np.float32(N)).astype(np.float32)
+
+
+def bench_batch(args, dev):
+    """256 type-2 NFFTs of uneven sizes: one graph replay of the batched setpts + type-2 (8 launches),
+    against the same transforms run one plan at a time (also graph-replayed)."""
+    import miscellaneous_amd as m
+
+    sizes, rng = batch_sizes(SEED + 3)
+    gen = torch.Generator().manual_seed(SEED + 3)
+    fs = [cn01(gen, (n1, n2), dev) for (n1, n2, _) in sizes]
+    pts = [batch_points(rng, n1, mm) for (n1, _, mm) in sizes]
+    xs = [torch.from_numpy(a).to(dev) for a, _ in pts]
+    ys = [torch.from_numpy(b).to(dev) for _, b in pts]
+    outs = [torch.empty(mm, dtype=torch.complex64, device=dev) for (_, _, mm) in sizes]
+    b = m.Batch(sizes, device=dev).set_points(xs, ys).set_io(fs, outs)
+    t0 = time.perf_counter()
+    choice = b.autotune(reps=3)
+    tune_s = time.perf_counter() - t0
+
+    def step():
+        b.setpts()
+        b.type2()
+
+    g = capture(step)
+    s_batch = time_graph(g, max(args.steps // 4, 10), 3)
+
+    # accuracy: 256 seeded outputs of 16 transforms against an FP64 direct sum
+    step()
+    torch.cuda.synchronize()
+    worst = 0.0
+    sel_rng = np.random.default_rng(SEED + 4)
+    for t in sel_rng.choice(len(sizes), 16, replace=False):
+        mm = sizes[t][2]
+        sel = torch.from_numpy(sel_rng.choice(mm, min(mm, 256), replace=False)).to(dev)
+        ref = direct_type2_subset(fs[t], xs[t][sel], ys[t][sel])
+        worst = max(worst, ((outs[t][sel].to(torch.complex128) - ref).norm() / ref.norm()).item())
+
+    # the same transforms through one plan each (the single-transform FP32 path)
+    plans = [m.Plan(n1, n2, torch.complex64, device=dev) for (n1, n2, _) in sizes]
+    outs2 = [torch.empty_like(o) for o in outs]
+
+    def step_plain():
+        for p, f, x, y, o in zip(plans, fs, xs, ys, outs2):
+            p.setpts(x, y)
+            p.type2(f, out=o)
+
+    g2 = capture(step_plain)
+    s_plain = time_graph(g2, max(args.steps // 20, 3), 1)
+    same = all(torch.equal(a, c) for a, c in zip(outs, outs2))
+    total_m = sum(mm for (_, _, mm) in sizes)
+    nbytes = sum(200 * n1 * n2 + 16 * mm for (n1, n2, mm) in sizes)
+    tuned = {}
+    for (n1, _, _), c in zip(sizes, choice):
+        tuned[str(n1)] = c
+    del plans, g2
+    return {"config": "256 type-2 NFFTs, N uniform in {64,128,256,512} (square), M log-uniform in [4096, 262144], "
+                      "half uniform / half Gaussian (sigma 0.05, wrapped) points, complex64, K=6, c=2; "
+                      "step = batched setpts + type-2 (8 launches) replayed from one graph",
+            "transforms": len(sizes), "total_points": total_m, "value": total_m / s_batch / 1e6, "unit": "Mpts/s",
+            "ms_per_step": s_batch * 1e3, "plain_ms_per_step": s_plain * 1e3, "speedup_vs_plain": s_plain / s_batch,
+            "identical_to_plain": same, "rel_l2_max_16x256": worst,
+            "hbm_frac": nbytes / s_batch / (HBM_PEAK_GBS * 1e9), "algorithmic_bytes": nbytes,
+            "autotuned_maxpts_by_N": tuned, "autotune_s": tune_s}
+
+
+def bench_ndft(args, dev):
+    """Type-2 NDFT as an MFMA contraction, 256 x 256 modes, M = 65536 uniform points, three precisions."""
+    import miscellaneous_amd as m
+
+    N, M = 256, 65536
+    gen = torch.Generator().manual_seed(SEED + 5)
+    f = cn01(gen, (N, N), dev)
+    x1 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    x2 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    sel = torch.from_numpy(np.random.default_rng(SEED + 5).choice(M, 4096, replace=False)).to(dev)
+    ref = direct_type2_subset(f, x1[sel], x2[sel])
+    peak = {"fp32": 157.3e12, "bf16x3": 2.5e15, "fp16": 2.5e15}
+    prods = {"fp32": 1, "bf16x3": 3, "fp16": 1}
+    res = {}
+    for prec in ("fp32", "bf16x3", "fp16"):
+        p = m.NdftPlan(N, N, prec, device=dev).set_modes(f)
+        out = torch.empty(M, dtype=torch.complex64, device=dev)
+        p.eval(x1, x2, out)
+        torch.cuda.synchronize()
+        rel = ((out[sel].to(torch.complex128) - ref).norm() / ref.norm()).item()
+        reps = 5 if prec == "fp32" else 20
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            p.eval(x1, x2, out)
+        ev[1].record()
+        torch.cuda.synchronize()
+        t = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+        flops = 8.0 * M * N * N * prods[prec]
+        res[prec] = {"ms": t * 1e3, "rel_l2": rel, "executed_tflops": flops / t / 1e12,
+                     "mfma_util": flops / t / peak[prec], "useful_tflops": 8.0 * M * N * N / t / 1e12}
+    return {"config": "type-2 NDFT, 256x256 CN(0,1) modes, M=65536 uniform points, T = B F^T on MFMA "
+                      "(k_ndft_t2), rel-L2 on 4096 points vs FP64 direct sum; MFMA peaks: FP32 157.3, "
+                      "BF16/FP16 2500 TFLOP/s (spec)", **res}
+
+
+def bench_large(args, dev):
+    """Type-2 NFFT at 1024 x 1024 modes, M = 4194304 uniform points (setpts + type-2, graph-replayed)."""
+    import miscellaneous_amd as m
+
+    N, M = 1024, 4194304
+    gen = torch.Generator().manual_seed(SEED + 6)
+    f = cn01(gen, (N, N), dev)
+    x1 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    x2 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+    y = torch.empty(M, dtype=torch.complex64, device=dev)
+    plan = m.Plan(N, N, torch.complex64, device=dev)
+
+    def step():
+        plan.setpts(x1, x2)
+        plan.type2(f, out=y)
+
+    g = capture(step)
+    s = time_graph(g, max(args.steps // 10, 5), 2)
+    sel = torch.from_numpy(np.random.default_rng(SEED + 6).choice(M, 4096, replace=False)).to(dev)
+    ref = direct_type2_subset(f, x1[sel], x2[sel])
+    rel = ((y[sel].to(torch.complex128) - ref).norm() / ref.norm()).item()
+    return {"config": "type-2 NFFT 1024x1024 modes, M=4194304 uniform, K=6, c=2; step = setpts + type-2",
+            "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3, "rel_l2": rel,
+            "hbm_frac": (200 * N * N + 16 * M) / s / (HBM_PEAK_GBS * 1e9)}
+
+
 def cpu_baseline(t2, seconds: float):
     import oracle
 
@@ -232,6 +379,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batch, NDFT and 1024^2 workloads (headline and gradient only)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "gather_polar512_traffic.json"),
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -253,6 +402,11 @@ def main():
         del src, dst
     t2 = bench_type2(args, dev)
     gr = None if args.no_grad else bench_grad(args, dev)
+    extra = {}
+    if not args.no_extra:
+        extra["batch"] = bench_batch(args, dev)
+        extra["ndft"] = bench_ndft(args, dev)
+        extra["large_1024"] = bench_large(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]
@@ -291,6 +445,7 @@ def main():
                        "hbm_frac": gr["bytes"] / gr["s"] / (HBM_PEAK_GBS * 1e9),
                        "loss_first": gr["loss0"], "loss_after_steps": gr["loss_final"], "gd_steps": gr["steps"],
                        "gd_lr": gr["lr"]}
+    out.update(extra)
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
