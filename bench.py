@@ -100,10 +100,6 @@ def time_graph(g, steps: int, warmup: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def event_ms(pairs) -> float:
-    return float(np.mean([a.elapsed_time(b) for a, b in pairs]))
-
-
 def bench_type2(args, dev):
     import miscellaneous_amd as m
 
@@ -125,22 +121,24 @@ def bench_type2(args, dev):
     y_ref = direct_type2_subset(f, x1[sel], x2[sel])
     rel = ((y[sel].to(torch.complex128) - y_ref).norm() / y_ref.norm()).item()
 
-    # per-kernel times with HIP events on the stream the kernels run on
-    ev = []
-    for _ in range(max(args.steps, 20)):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        e[0].record()
-        plan.setpts(x1, x2)
-        e[1].record()
-        plan.type2_grid(f)
-        e[2].record()
-        plan.type2_interp(y)
-        e[3].record()
-        ev.append(e)
-    torch.cuda.synchronize()
-    setpts_ms = event_ms([(e[0], e[1]) for e in ev])
-    grid_ms = event_ms([(e[1], e[2]) for e in ev])
-    gather_ms = event_ms([(e[2], e[3]) for e in ev])
+    # per-stage launch durations: HIP events on the stream the kernels run on,
+    # around n back-to-back launches of the stage (the queue stays full, so
+    # the average is the stage's device time per launch)
+    def stage_ms(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    n_ev = max(args.steps, 20)
+    setpts_ms = stage_ms(lambda: plan.setpts(x1, x2), n_ev)
+    grid_ms = stage_ms(lambda: plan.type2_grid(f), n_ev)
+    gather_ms = stage_ms(lambda: plan.type2_interp(y), n_ev)
 
     def step_full():
         plan.setpts(x1, x2)
