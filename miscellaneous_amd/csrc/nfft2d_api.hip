@@ -501,6 +501,14 @@ static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, 
 {
     using C = cplx<T>;
     if (p->M == 0) return NFFT2D_OK;
+#ifdef NFFT2D_WAVE_GATHER
+    if constexpr (std::is_same<T, float>::value && MODE == 0) {
+        hipLaunchKernelGGL(k_gather_w, dim3((sub_blocks(p) + 3) / 4), dim3(256), 0, s, (const float2 *)p->grid, p->geo,
+                           (const PtRec<float> *)p->pts, p->sub, p->nsub, p->wa, (float2 *)out);
+        NFFT2D_LAUNCH_CHECK();
+        return NFFT2D_OK;
+    }
+#endif
     hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(kGatherThreads), 0, s, (const C *)p->grid, p->geo,
                        (const PtRec<T> *)p->pts, p->sub, p->nsub, p->wa, (C *)out, (C *)d1, (C *)d2, (const C *)g,
                        (T *)gx1, (T *)gx2);
@@ -744,6 +752,7 @@ static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
 static int batch_cols_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
 static constexpr int kBatchMinMaxpts = 64;   // smallest subproblem size the autotuner tries
 static constexpr int kBatchGatherBlocks = 2048;
+static constexpr int kBatchGatherWaveBlocks = 256 * 5;   // 5 blocks of 4 waves per CU (LDS: 4 tiles each)
 
 static void batch_free(nfft2d_batch *b)
 {
@@ -930,8 +939,13 @@ extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_t2_cols, dim3(b->n_cols), dim3(256), 0, s, b->ddesc, b->own + b->off_cols);
     NFFT2D_LAUNCH_CHECK();
+#ifdef NFFT2D_BATCH_BLOCK_GATHER
     hipLaunchKernelGGL(kb_gather, dim3(kBatchGatherBlocks), dim3(kGatherThreads), 0, s, b->ddesc, b->bsub,
                        b->base + b->count, b->plans[0]->wa);
+#else
+    hipLaunchKernelGGL(kb_gather_w, dim3(kBatchGatherWaveBlocks), dim3(256), 0, s, b->ddesc, b->bsub,
+                       b->base + b->count, b->plans[0]->wa);
+#endif
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }

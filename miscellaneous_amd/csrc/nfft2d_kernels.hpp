@@ -860,6 +860,103 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(
     gather_body<T, MODE>(blockIdx.x, Gt, geo, pts, sub, wa, out, out_d1, out_d2, g, gx1, gx2);
 }
 
+// LDS writes of a wave become visible to the whole wave.
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// (c) type-2 gather, one WAVE per subproblem (complex64, MODE 0): the wave
+// stages the subproblem's 28 x 28 tile in its own LDS tile (13 loads per lane,
+// all in flight) and then walks the points in chunks of 64, one per lane.
+// Four subproblems share a 256-thread block, so many more tiles are in flight
+// per CU than with one subproblem per block; this pays off when subproblems
+// are small (sparse point sets).  Same arithmetic per point as k_gather<float, 0>.
+constexpr int kGwTileElems = (kBin + 2 * kK) * kTileLd;
+
+__device__ __forceinline__ void gather_wave(const int4& d, float2* tile, const float2* __restrict__ Gt,
+                                            const BinGeom& geo, const PtRec<float>* __restrict__ pts,
+                                            const WinArgs& wa, float2* __restrict__ out)
+{
+    constexpr int W = kBin + 2 * kK;
+    constexpr int kPer = (W * W + 63) / 64;                  // 13
+    const int lane = threadIdx.x & 63;
+    const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
+    // point records of the first chunk and the tile, all loads in flight
+    PtRec<float> rec;
+    rec.t1 = rec.t2 = 0.f;
+    rec.cell = rec.idx = 0;
+    if (lane < d.z) rec = pts[d.y + lane];
+    float2 v[kPer];
+    const bool big = geo.L1 >= W && geo.L2 >= W;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = lane + 64 * q;
+        v[q] = make_float2(0.f, 0.f);
+        if (e < W * W) {
+            const int r = e / W, c = e - r * W;
+            const int o2 = O2 - kK + r, o1 = O1 - kK + c;
+            const int g2 = big ? wrap1(o2, geo.L2) : wrap(o2, geo.L2);
+            const int g1 = big ? wrap1(o1, geo.L1) : wrap(o1, geo.L1);
+            v[q] = Gt[(size_t)g2 * geo.L1 + g1];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = lane + 64 * q;
+        if (e < W * W) {
+            const int r = e / W, c = e - r * W;
+            tile[r * kTileLd + c] = v[q];
+        }
+    }
+    wave_sync_lds();
+    for (int base = 0; base < d.z; base += 64) {
+        const int i = base + lane;
+        PtRec<float> nxt;
+        nxt.t1 = nxt.t2 = 0.f;
+        nxt.cell = nxt.idx = 0;
+        if (i + 64 < d.z) nxt = pts[d.y + i + 64];           // next chunk, in flight during this one
+        if (i < d.z) {
+            const int r0 = min(max((rec.cell >> 16) - O2, 0), kBin - 1);
+            const int c0 = min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
+            float w1[kW], w2[kW];
+            taps(rec.t1, wa, w1);
+            taps(rec.t2, wa, w2);
+            const float2* bp = tile + r0 * kTileLd + c0;
+            float ax = 0.f, ay = 0.f;
+#pragma unroll
+            for (int b = 0; b < kW; ++b) {
+                float sx = 0.f, sy = 0.f;
+#pragma unroll
+                for (int a = 0; a < kW; ++a) {
+                    const float2 u = bp[b * kTileLd + a];
+                    sx = fmaf(w1[a], u.x, sx);
+                    sy = fmaf(w1[a], u.y, sy);
+                }
+                ax = fmaf(w2[b], sx, ax);
+                ay = fmaf(w2[b], sy, ay);
+            }
+            out[rec.idx] = make_float2(ax, ay);
+        }
+        rec = nxt;
+    }
+    wave_sync_lds();                                         // the tile is reused by the wave's next subproblem
+}
+
+// One wave per subproblem of a plan: wave g = 4 blockIdx.x + w takes subproblem g.
+__global__ __launch_bounds__(256) void k_gather_w(const float2* __restrict__ Gt, BinGeom geo,
+                                                  const PtRec<float>* __restrict__ pts, const int4* __restrict__ sub,
+                                                  const int* __restrict__ nsub, WinArgs wa, float2* __restrict__ out)
+{
+    __shared__ float2 tiles[4][kGwTileElems];
+    const int w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 4 + w;
+    if (s >= *nsub) return;
+    gather_wave(sub[s], tiles[w], Gt, geo, pts, wa, out);
+}
+
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates
 // the bin + K-halo in LDS, then adds it into the grid with global atomics.
 template <typename T>
@@ -930,12 +1027,6 @@ constexpr int kHalo = kBin + 2 * kK;          // 28 used rows / columns of a til
 constexpr int kTileCells = kHalo * kHalo;     // 784
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Per-wave staging of 64 points, one column per point (odd stride 65):
 // rows 0-12 w2 taps, 13-25 w1 taps, 26 zeros.
