@@ -110,6 +110,15 @@ int nfft2d_setpts(nfft2d_plan *plan, const void *x, const void *y, int M, void *
 
 /* Type-2: result[l] = sum_k data[k] e^{-2 pi i k.x_l / N}  (result: M, data: N1*N2). */
 int nfft2d_type2(nfft2d_plan *plan, void *result, const void *data, void *stream);
+/* nfft2d_setpts followed by nfft2d_type2, with the tile sort on a second
+ * stream owned by the plan, concurrent with the grid passes (pre-compensation,
+ * zero padding, FFT) on `stream`; the gather waits for both.  Everything stays
+ * ordered on `stream` for the caller and can be captured into a hipGraph (the
+ * first call creates the second stream: make it outside a capture).  The
+ * one-call form of the reference's type-2 (BP_FBP_FFBP/NFFT1_2D.m), which
+ * NFFT1_2D_GPU_c64 / _c128 use. */
+int nfft2d_setpts_type2(nfft2d_plan *plan, void *result, const void *data, const void *x, const void *y, int M,
+                        void *stream);
 /* Type-1: result[k] = sum_l data[l] e^{-2 pi i k.x_l / N}  (result: N1*N2, data: M). */
 int nfft2d_type1(nfft2d_plan *plan, void *result, const void *data, void *stream);
 /* Adjoint of type-2: result[k] = sum_l data[l] e^{+2 pi i k.x_l / N}. */
@@ -193,8 +202,11 @@ int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c
  *                (M[t]): device arrays, kept by reference;
  *   setpts       tile-sort the points of every transform;
  *   type2        result[t] = type-2 of modes[t] at the sorted points;
+ *   setpts_type2 setpts + type2, the tile sort on a second stream owned by the
+ *                batch, concurrent with the grid passes (as
+ *                nfft2d_setpts_type2; the first call creates the stream);
  *   autotune     choose the subproblem size (64, 128 or 256 points) per size
- *                class (N1, N2) by timing reps setpts + type2 passes of the
+ *                class (N1, N2) by timing reps setpts_type2 passes of the
  *                whole batch per candidate; synchronizes the stream;
  *   set_maxpts / get_maxpts  the subproblem size of transform t ([64, 256]). */
 typedef struct nfft2d_batch_s nfft2d_batch;
@@ -205,6 +217,7 @@ int nfft2d_batch_set_points(nfft2d_batch *batch, const float *const *x, const fl
 int nfft2d_batch_set_io(nfft2d_batch *batch, const nfft2d_c64 *const *modes, nfft2d_c64 *const *result);
 int nfft2d_batch_setpts(nfft2d_batch *batch, void *stream);
 int nfft2d_batch_type2(nfft2d_batch *batch, void *stream);
+int nfft2d_batch_setpts_type2(nfft2d_batch *batch, void *stream);
 int nfft2d_batch_autotune(nfft2d_batch *batch, void *stream, int reps);
 int nfft2d_batch_set_maxpts(nfft2d_batch *batch, int t, int maxpts);
 int nfft2d_batch_get_maxpts(const nfft2d_batch *batch, int t);

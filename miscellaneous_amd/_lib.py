@@ -51,6 +51,7 @@ _SIGS = {
     "nfft2d_plan_destroy": (_i, [_vp]),
     "nfft2d_setpts": (_i, [_vp, _vp, _vp, _i, _vp]),
     "nfft2d_type2": (_i, [_vp, _vp, _vp, _vp]),
+    "nfft2d_setpts_type2": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "nfft2d_type1": (_i, [_vp, _vp, _vp, _vp]),
     "nfft2d_adjoint": (_i, [_vp, _vp, _vp, _vp]),
     "nfft2d_type2_dx": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -74,6 +75,7 @@ _SIGS = {
     "nfft2d_batch_set_io": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "nfft2d_batch_setpts": (_i, [_vp, _vp]),
     "nfft2d_batch_type2": (_i, [_vp, _vp]),
+    "nfft2d_batch_setpts_type2": (_i, [_vp, _vp]),
     "nfft2d_batch_autotune": (_i, [_vp, _vp, _i]),
     "nfft2d_batch_set_maxpts": (_i, [_vp, _i, _i]),
     "nfft2d_batch_get_maxpts": (_i, [_vp, _i]),

@@ -89,6 +89,12 @@ class Batch:
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_batch_type2(self._h, _stream(self.device)), "nfft2d_batch_type2")
 
+    def setpts_type2(self) -> None:
+        """setpts() and type2() with the tile sort on the batch's second stream,
+        concurrent with the grid passes."""
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_setpts_type2(self._h, _stream(self.device)), "nfft2d_batch_setpts_type2")
+
     def autotune(self, reps: int = 3) -> list:
         """Choose the subproblem size per size class by timing; returns the per-transform choice."""
         with torch.cuda.device(self.device):

@@ -133,6 +133,53 @@ static void make_window_args(WinArgs &wa)
 }
 
 // ============================================================== plan
+// A second stream and two events, for running a plan's (or batch's) bin sort
+// beside its grid passes: fork = record on the caller's stream s, aux waits;
+// join = record on aux, s waits.  Both are stream-ordered, so the pattern is
+// captured into a hipGraph like any other work on s (as two branches).  The
+// stream and the events are created at the first use (not inside a capture:
+// the first fused call also sizes the point buffers, as nfft2d_setpts does).
+struct Fork {
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int init(int device)
+    {
+        if (aux) return NFFT2D_OK;
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(device) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_HIP, "fork: cannot select the device");
+        const bool ok = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) == hipSuccess &&
+                        hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) == hipSuccess &&
+                        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) == hipSuccess;
+        (void)hipSetDevice(cur);
+        if (!ok) {
+            destroy();
+            return nfft2d_fail(NFFT2D_ERR_HIP, "fork: cannot create the second stream");
+        }
+        return NFFT2D_OK;
+    }
+    int fork(hipStream_t s)
+    {
+        if (hipEventRecord(ev_fork, s) != hipSuccess || hipStreamWaitEvent(aux, ev_fork, 0) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_HIP, "fork: event record / wait failed");
+        return NFFT2D_OK;
+    }
+    int join(hipStream_t s)
+    {
+        if (hipEventRecord(ev_join, aux) != hipSuccess || hipStreamWaitEvent(s, ev_join, 0) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_HIP, "join: event record / wait failed");
+        return NFFT2D_OK;
+    }
+    void destroy()
+    {
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux) (void)hipStreamDestroy(aux);
+        aux = nullptr;
+        ev_fork = ev_join = nullptr;
+    }
+};
+
 struct nfft2d_plan_s {
     int N1 = 0, N2 = 0, L1 = 0, L2 = 0, dtype = 0, device = 0;
     int nb1 = 0, nb2 = 0, nbins = 0, maxpts = kGatherThreads;
@@ -152,6 +199,7 @@ struct nfft2d_plan_s {
     int scratchcap = 0;
     BinGeom geo;
     WinArgs wa;
+    Fork fk;                              // setpts beside the grid passes (nfft2d_setpts_type2)
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -197,6 +245,7 @@ static void free_plan(nfft2d_plan *p)
     free_points(p);
     free_hist(p);
     free_scratch(p);
+    p->fk.destroy();
     hfree(p->subbeg);
     hfree(p->gexcl); hfree(p->gsum);
     (void)hipSetDevice(cur);
@@ -372,12 +421,21 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
     hipLaunchKernelGGL(k_colscan, dim3(ngroups), dim3(256), 0, s, p->hist, nblk, p->nbins, p->maxpts, p->tot, p->gexcl,
                        p->gsum);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_groups, dim3(1), dim3(kScanThreads), 0, s, p->gsum, ngroups, p->nbins, p->subbeg, p->nsub);
-    NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(std::max(M, p->nbins), 256)), dim3(256), 0, s, x, y, M, chunk, p->L1,
-                       p->L2, inv1, inv2, p->nb1, p->nbins, p->maxpts, p->hist, p->tot, p->gexcl, p->gsum, p->rank,
-                       (PtRec<T> *)p->pts, p->subbeg, p->sub);
-    NFFT2D_LAUNCH_CHECK();
+    if (ngroups <= kScatThreads) {
+        const long long nb = std::max(grid1d(M, kScatThreads * kScatPer), grid1d(p->nbins, kScatThreads));
+        hipLaunchKernelGGL(k_scatter_scan<T>, dim3(nb), dim3(kScatThreads), 0, s, x, y, M, chunk, p->L1, p->L2, inv1,
+                           inv2, p->nb1, p->nbins, p->maxpts, p->hist, p->tot, p->gexcl, p->gsum, ngroups, p->rank,
+                           (PtRec<T> *)p->pts, p->subbeg, p->sub, p->nsub);
+        NFFT2D_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(k_scan_groups, dim3(1), dim3(kScanThreads), 0, s, p->gsum, ngroups, p->nbins, p->subbeg,
+                           p->nsub);
+        NFFT2D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_scatter<T>, dim3(grid1d(std::max(M, p->nbins), 256)), dim3(256), 0, s, x, y, M, chunk,
+                           p->L1, p->L2, inv1, inv2, p->nb1, p->nbins, p->maxpts, p->hist, p->tot, p->gexcl, p->gsum,
+                           p->rank, (PtRec<T> *)p->pts, p->subbeg, p->sub);
+        NFFT2D_LAUNCH_CHECK();
+    }
     p->nblk = nblk;
     p->chunk = chunk;
     p->M = M;
@@ -607,6 +665,26 @@ extern "C" int nfft2d_type2(nfft2d_plan *p, void *result, const void *data, void
     return nfft2d_type2_interp(p, result, stream);
 }
 
+// setpts + type-2 with the bin sort on the plan's second stream, concurrent
+// with the grid passes on `stream`; the gather waits for both.
+extern "C" int nfft2d_setpts_type2(nfft2d_plan *p, void *result, const void *data, const void *x, const void *y, int M,
+                                   void *stream)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: null plan");
+    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: M < 0");
+    if (M > 0 && (!x || !y || !data || !result))
+        return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: null data, result or coordinate pointer");
+    if (M == 0) return nfft2d_setpts(p, x, y, 0, stream);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = p->fk.init(p->device);
+    if (rc || (rc = p->fk.fork(s))) return rc;
+    rc = nfft2d_setpts(p, x, y, M, p->fk.aux);
+    const int rg = rc ? NFFT2D_OK : nfft2d_type2_grid(p, data, stream);
+    const int rj = p->fk.join(s);               // always rejoin, so that a capture stays well formed
+    if (rc || (rc = rg) || (rc = rj)) return rc;
+    return nfft2d_type2_interp(p, result, stream);
+}
+
 extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, const void *data, void *stream)
 {
     int rc = check_exec(p, y, data);
@@ -681,8 +759,8 @@ int dropin(int dtype, bool type2, void *result, const void *data, const void *x,
     nfft2d_plan *p = nullptr;
     int rc = cached_plan(N1, N2, dtype, &p);
     if (rc) return rc;
+    if (type2) return nfft2d_setpts_type2(p, result, data, x, y, M, nullptr);
     if ((rc = nfft2d_setpts(p, x, y, M, nullptr))) return rc;
-    if (type2) return M == 0 ? NFFT2D_OK : nfft2d_type2(p, result, data, nullptr);
     return nfft2d_type1(p, result, data, nullptr);
 }
 }  // namespace
@@ -746,6 +824,7 @@ struct nfft2d_batch_s {
     long long bsubcap = 0;
     int max_nbins = 0;
     bool have_points = false, have_io = false;
+    Fork fk;                              // setpts beside the grid passes (nfft2d_batch_setpts_type2)
 };
 
 static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
@@ -759,6 +838,7 @@ static void batch_free(nfft2d_batch *b)
     if (!b) return;
     for (auto *p : b->plans) nfft2d_plan_destroy(p);
     hfree(b->ddesc); hfree(b->own); hfree(b->base); hfree(b->bsub);
+    b->fk.destroy();
     delete b;
 }
 
@@ -930,15 +1010,17 @@ extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
     return NFFT2D_OK;
 }
 
-extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
+static int batch_grid(nfft2d_batch *b, hipStream_t s)
 {
-    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type2: null batch");
-    if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: nfft2d_batch_set_io was not called");
-    hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(kb_t2_rows, dim3(b->n_rows), dim3(128), 0, s, b->ddesc, b->own + b->off_rows);
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_t2_cols, dim3(b->n_cols), dim3(256), 0, s, b->ddesc, b->own + b->off_cols);
     NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+static int batch_gather(nfft2d_batch *b, hipStream_t s)
+{
 #ifdef NFFT2D_BATCH_BLOCK_GATHER
     hipLaunchKernelGGL(kb_gather, dim3(kBatchGatherBlocks), dim3(kGatherThreads), 0, s, b->ddesc, b->bsub,
                        b->base + b->count, b->plans[0]->wa);
@@ -948,6 +1030,32 @@ extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
 #endif
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type2: null batch");
+    if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: nfft2d_batch_set_io was not called");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = batch_grid(b, s);
+    return rc ? rc : batch_gather(b, s);
+}
+
+// setpts + type-2 of the batch with the bin sort on the batch's second stream,
+// concurrent with the grid passes on `stream`; the gather waits for both.
+extern "C" int nfft2d_batch_setpts_type2(nfft2d_batch *b, void *stream)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_setpts_type2: null batch");
+    if (!b->have_points) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts_type2: nfft2d_batch_set_points was not called");
+    if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts_type2: nfft2d_batch_set_io was not called");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = b->fk.init(b->device);
+    if (rc || (rc = b->fk.fork(s))) return rc;
+    rc = nfft2d_batch_setpts(b, b->fk.aux);
+    const int rg = rc ? NFFT2D_OK : batch_grid(b, s);
+    const int rj = b->fk.join(s);
+    if (rc || (rc = rg) || (rc = rj)) return rc;
+    return batch_gather(b, s);
 }
 
 // Subproblem size per size class (N1, N2), chosen by timing the whole batch
@@ -968,14 +1076,10 @@ extern "C" int nfft2d_batch_autotune(nfft2d_batch *b, void *stream, int reps)
     NFFT2D_HIP_TRY(hipEventCreate(&e1));
     int rc = NFFT2D_OK;
     auto time_once = [&](float &ms) -> int {
-        int r = nfft2d_batch_setpts(b, s);
-        if (!r) r = nfft2d_batch_type2(b, s);      // warm-up
+        int r = nfft2d_batch_setpts_type2(b, s);    // warm-up
         if (r) return r;
         if (hipEventRecord(e0, s) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event record");
-        for (int k = 0; k < reps && !r; ++k) {
-            r = nfft2d_batch_setpts(b, s);
-            if (!r) r = nfft2d_batch_type2(b, s);
-        }
+        for (int k = 0; k < reps && !r; ++k) r = nfft2d_batch_setpts_type2(b, s);
         if (r) return r;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)

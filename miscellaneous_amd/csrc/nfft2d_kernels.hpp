@@ -625,6 +625,82 @@ __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const 
                     pts, subbeg, sub);
 }
 
+// Pass 3 with the group scan folded in (grids of at most kMaxLdsBins bins,
+// i.e. at most kScatThreads groups): every block scans the group sums gsum
+// itself into LDS (at most 8 KiB of loads per block), so no launch separates
+// k_colscan from the scatter.  A block scatters kScatThreads * kScatPer points
+// and, if its bin range [blk * kScatThreads, ...) holds bins, writes their
+// subproblems.  Same positions as k_scan_groups + k_scatter.
+constexpr int kScatThreads = 1024, kScatPer = 4;
+template <typename T>
+__global__ __launch_bounds__(kScatThreads) void k_scatter_scan(
+    const T* __restrict__ x, const T* __restrict__ y, int M, int chunk, int L1, int L2, T inv1, T inv2, int nb1,
+    int nbins, int maxpts, const int* __restrict__ off, const int* __restrict__ tot, const int2* __restrict__ gexcl,
+    const int2* __restrict__ gsum, int ngroups, const int* __restrict__ rank, PtRec<T>* __restrict__ pts,
+    int* __restrict__ subbeg, int4* __restrict__ sub, int* __restrict__ nsub)
+{
+    __shared__ int2 pre[kScatThreads + 1];
+    __shared__ int w_a[kScatThreads / 64], w_b[kScatThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // this block's points: loads in flight during the group scan
+    const int p0 = blockIdx.x * (kScatThreads * kScatPer);
+    T xv[kScatPer], yv[kScatPer];
+    int rv[kScatPer];
+#pragma unroll
+    for (int q = 0; q < kScatPer; ++q) {
+        const int i = p0 + q * kScatThreads + t;
+        const bool in = i < M;
+        xv[q] = in ? x[i] : (T)0;
+        yv[q] = in ? y[i] : (T)0;
+        rv[q] = in ? rank[i] : 0;
+    }
+    // exclusive scan of the (points, subproblems) group sums, one per thread
+    const int2 v = t < ngroups ? gsum[t] : make_int2(0, 0);
+    int ia = v.x, ib = v.y;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
+        if (lane >= o) { ia += na; ib += nb; }
+    }
+    if (lane == 63) { w_a[w] = ia; w_b[w] = ib; }
+    __syncthreads();
+    int pa = ia - v.x, pb = ib - v.y;
+    for (int k = 0; k < w; ++k) { pa += w_a[k]; pb += w_b[k]; }
+    pre[t] = make_int2(pa, pb);
+    if (t == kScatThreads - 1) pre[kScatThreads] = make_int2(pa + v.x, pb + v.y);
+    __syncthreads();
+    const int bi = blockIdx.x * kScatThreads + t;
+    if (bi < nbins) {
+        const int c = tot[bi];
+        const int2 e = gexcl[bi], gb = pre[bi / kColBins];
+        const int pos = gb.x + e.x, sp = gb.y + e.y;
+        subbeg[bi] = sp;
+        const int ns = (c + maxpts - 1) / maxpts;
+        for (int k = 0; k < ns; ++k) sub[sp + k] = make_int4(bi, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
+    }
+    if (blockIdx.x == 0 && t == 0) {
+        const int total = pre[kScatThreads].y;
+        *nsub = total;
+        subbeg[nbins] = total;
+    }
+#pragma unroll
+    for (int q = 0; q < kScatPer; ++q) {
+        const int i = p0 + q * kScatThreads + t;
+        if (i < M) {
+            int m1, m2;
+            T t1, t2;
+            const int bin = bin_of(xv[q], yv[q], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+            const int p = pre[bin / kColBins].x + gexcl[bin].x + off[(size_t)(i / chunk) * nbins + bin] + rv[q];
+            PtRec<T> rec;
+            rec.t1 = t1;
+            rec.t2 = t2;
+            rec.cell = (m2 << 16) | m1;
+            rec.idx = i;
+            pts[p] = rec;
+        }
+    }
+}
+
 // ================================================================= window taps
 struct WinArgs {
     WinPoly val;    // phi_hat / S

@@ -135,6 +135,25 @@ class Plan:
             check(self._lib.nfft2d_type2(self._h, _ptr(out), _ptr(f), _stream(self.device)), "nfft2d_type2")
         return out
 
+    def setpts_type2(self, x: torch.Tensor, y: torch.Tensor, f: torch.Tensor,
+                     out: torch.Tensor | None = None) -> torch.Tensor:
+        """setpts(x, y) then type2(f): the tile sort runs on the plan's second
+        stream, concurrent with the grid passes; the result is ordered on
+        torch's current stream."""
+        M = x.numel()
+        _require(x, self.rdtype, M, "x", self.device)
+        _require(y, self.rdtype, M, "y", self.device)
+        _require(f, self.dtype, self.N1 * self.N2, "f", self.device)
+        if out is None:
+            out = torch.empty(M, dtype=self.dtype, device=self.device)
+        _require(out, self.dtype, M, "out", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_setpts_type2(self._h, _ptr(out), _ptr(f), _ptr(x), _ptr(y), M,
+                                                _stream(self.device)), "nfft2d_setpts_type2")
+        self.M = M
+        self.pts_version += 1
+        return out
+
     def type1(self, d: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """F[k] = sum_l d[l] e^{-2 pi i k.x_l/N}  (the reference NFFT2_2D_GPU)."""
         self._need_points()

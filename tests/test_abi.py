@@ -63,6 +63,7 @@ def test_null_arguments_rejected(lib):
     assert lib.nfft2d_plan_destroy(None) == -1
     assert lib.nfft2d_setpts(None, None, None, 0, None) == -1
     assert lib.nfft2d_type2(None, None, None, None) == -1
+    assert lib.nfft2d_setpts_type2(None, None, None, None, None, 10, None) == -1
     assert lib.nfft2d_type1(None, None, None, None) == -1
     assert lib.nfft2d_plan_num_points(None) == -1
     assert lib.NFFT2_2D_GPU_c64(None, None, None, None, 64, 64, 10) == -1
@@ -120,6 +121,7 @@ def test_batch_null_arguments_rejected(lib):
     assert lib.nfft2d_batch_destroy(None) == -1
     assert lib.nfft2d_batch_setpts(None, None) == -1
     assert lib.nfft2d_batch_type2(None, None) == -1
+    assert lib.nfft2d_batch_setpts_type2(None, None) == -1
     assert lib.nfft2d_batch_autotune(None, None, 1) == -1
     assert lib.nfft2d_batch_set_maxpts(None, 0, 128) == -1
     assert lib.nfft2d_batch_get_maxpts(None, 0) == -1

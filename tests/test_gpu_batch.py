@@ -111,6 +111,49 @@ def test_batch_graph_replay():
             assert torch.equal(to[t], eager[t]), (rep, t)
 
 
+def test_batch_fused_matches_and_replays():
+    """setpts + type-2 of the batch in one call (the tile sort on the batch's
+    second stream) equals the two calls, eagerly and from a graph replayed with
+    new coordinates written into the captured inputs between the replays."""
+    b, fs, xs, ys, tf, tx, ty, to = make_batch(seed=8)
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    sep = [o.clone() for o in to]
+    for o in to:
+        o.zero_()
+    b.setpts_type2()
+    torch.cuda.synchronize()
+    for t in range(len(SIZES)):
+        assert torch.equal(to[t], sep[t]), t
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        b.setpts_type2()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.setpts_type2()
+    rng = np.random.default_rng(10)
+    for rep in range(3):
+        for t, (N1, N2, M) in enumerate(SIZES):
+            x1, x2 = batch_points(rng, N1, N2, M)
+            tx[t].copy_(T(x1))
+            ty[t].copy_(T(x2))
+        for o in to:
+            o.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = [o.clone() for o in to]
+        b.setpts()
+        b.type2()
+        torch.cuda.synchronize()
+        for t in range(len(SIZES)):
+            assert torch.equal(got[t], to[t]), (rep, t)
+
+
 def test_batch_autotune_keeps_results():
     b, fs, xs, ys, tf, tx, ty, to = make_batch(seed=7)
     b.setpts()

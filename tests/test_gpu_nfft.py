@@ -244,6 +244,53 @@ def test_type2_bitwise_deterministic():
     assert torch.equal(a, b)
 
 
+def test_setpts_type2_fused_matches_and_replays():
+    """setpts + type-2 in one call (the tile sort on the plan's second stream,
+    beside the grid passes) gives exactly what the two calls give, eagerly and
+    replayed from a graph, with new coordinates written into the captured
+    inputs between the replays."""
+    rng = np.random.default_rng(12)
+    N, M = 256, 200000
+
+    def pts():
+        return (((rng.random(M) - 0.5) * N).astype(np.float32), ((rng.random(M) - 0.5) * N).astype(np.float32))
+
+    fn = cn01(rng, (N, N))
+    f = T(fn)
+    a1, a2 = pts()
+    x1, x2 = T(a1), T(a2)
+    plan = m.Plan(N, N, device=DEV)
+    ref = m.Plan(N, N, device=DEV)
+    y = plan.setpts_type2(x1, x2, f)
+    torch.cuda.synchronize()
+    assert torch.equal(y, ref.setpts(x1, x2).type2(f))
+    sel = np.sort(rng.choice(M, 512, replace=False))
+    err = oracle.rel_l2(y.cpu().numpy()[sel], oracle.ndft_type2(fn.astype(np.complex128), a1[sel], a2[sel]))
+    print(f"fused setpts+type2 256^2 M={M}: rel-L2 {err:.2e}")
+    assert err < TOL32
+    out = torch.empty(M, dtype=torch.complex64, device=DEV)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.setpts_type2(x1, x2, f, out)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.setpts_type2(x1, x2, f, out)
+    for rep in range(3):
+        b1, b2 = pts()
+        x1.copy_(T(b1))
+        x2.copy_(T(b2))
+        out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        expect = ref.setpts(x1, x2).type2(f)
+        torch.cuda.synchronize()
+        assert torch.equal(out, expect), rep
+
+
 def test_dropins_match_plan_api():
     rng = np.random.default_rng(8)
     N1, N2, M = 64, 128, 5000
