@@ -559,14 +559,12 @@ static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, 
 {
     using C = cplx<T>;
     if (p->M == 0) return NFFT2D_OK;
-#ifdef NFFT2D_WAVE_GATHER
     if constexpr (std::is_same<T, float>::value && MODE == 0) {
-        hipLaunchKernelGGL(k_gather_w, dim3((sub_blocks(p) + 3) / 4), dim3(256), 0, s, (const float2 *)p->grid, p->geo,
-                           (const PtRec<float> *)p->pts, p->sub, p->nsub, p->wa, (float2 *)out);
+        hipLaunchKernelGGL(k_gather_pairs, dim3(sub_blocks(p)), dim3(kGatherThreads), 0, s, (const float2 *)p->grid,
+                           p->geo, (const PtRec<float> *)p->pts, p->sub, p->nsub, p->wa, (float2 *)out);
         NFFT2D_LAUNCH_CHECK();
         return NFFT2D_OK;
     }
-#endif
     hipLaunchKernelGGL((k_gather<T, MODE>), dim3(sub_blocks(p)), dim3(kGatherThreads), 0, s, (const C *)p->grid, p->geo,
                        (const PtRec<T> *)p->pts, p->sub, p->nsub, p->wa, (C *)out, (C *)d1, (C *)d2, (const C *)g,
                        (T *)gx1, (T *)gx2);

@@ -763,16 +763,18 @@ __device__ __forceinline__ int wrap1(int g, int L) { return g < 0 ? g + L : (g >
 constexpr int kTileLd = NFFT2D_TILE_LD;     // LDS row stride of a gather tile (complex elements)
 
 // Launched with 256 threads; the 784 loads of a thread group are issued
-// together (a thread has at most 4).
+// together (a thread has at most 4): tile_fetch into registers, tile_store
+// from them into LDS (split so that a persistent kernel can fetch the next
+// tile while it works on the current one).
+constexpr int kTileW = kBin + 2 * kK;                       // 28 used rows / columns
+constexpr int kTilePer = (kTileW * kTileW + 255) / 256;     // tile values per thread
 template <typename C>
-__device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
+__device__ __forceinline__ void tile_fetch(C (&v)[kTilePer], const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
 {
-    constexpr int W = kBin + 2 * kK;
-    constexpr int kPer = (W * W + 255) / 256;
-    C v[kPer];
+    constexpr int W = kTileW;
     const bool big = L1 >= W && L2 >= W;
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
+    for (int q = 0; q < kTilePer; ++q) {
         const int e = (int)threadIdx.x + 256 * q;
         if (e < W * W) {
             const int r = e / W, c = e - r * W;
@@ -781,14 +783,26 @@ __device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int
             v[q] = Gt[(size_t)g2 * L1 + g1];
         }
     }
+}
+template <typename C>
+__device__ __forceinline__ void tile_store(C* tile, const C (&v)[kTilePer])
+{
+    constexpr int W = kTileW;
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
+    for (int q = 0; q < kTilePer; ++q) {
         const int e = (int)threadIdx.x + 256 * q;
         if (e < W * W) {
             const int r = e / W, c = e - r * W;
             tile[r * kTileLd + c] = v[q];
         }
     }
+}
+template <typename C>
+__device__ __forceinline__ void load_tile(C* tile, const C* __restrict__ Gt, int L1, int L2, int o1, int o2)
+{
+    C v[kTilePer];
+    tile_fetch(v, Gt, L1, L2, o1, o2);
+    tile_store(tile, v);
 }
 
 constexpr int kGatherThreads = 256;     // = points per gather subproblem
@@ -819,7 +833,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
 // counting-sorts its points by cell in LDS so that neighbouring lanes read
 // neighbouring tile addresses.
 template <typename T, int MODE>
-__device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
+__device__ __forceinline__ void gather_body_single(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
                                             const PtRec<T>* __restrict__ pts, const int4* __restrict__ sub,
                                             const WinArgs& wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1,
                                             cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g,
@@ -922,6 +936,156 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
             gx1[o] = cc * (gg.x * bx + gg.y * by);
             gx2[o] = cc * (gg.x * cx + gg.y * cy);
         }
+    }
+}
+
+// (c) type-2 gather, complex64, MODE 0, with up to TWO points per lane.
+// The tile reads are what binds the gather (a lane reads 13 x 13 tile values
+// per point from LDS; the FMAs have room to spare), so a lane takes two points
+// of one cell pair (cells 2j and 2j+1 of a cell row of the bin) and reads
+// their union window once: 13 rows of 14 values serve both points, 182 reads
+// instead of 338, as 16-byte reads (the window starts at an even column).
+// The block counting-sorts its points by cell pair; a pair with n points takes
+// ceil(n / 2) lanes.  Each point's 13 column weights are placed in a row of 14
+// with a zero at the end its window does not reach, so a point's sum runs in
+// the same order as in gather_body_single (plus one FMA with a zero weight):
+// the results are bitwise the same.
+constexpr int kPairs = kBin * kBin / 2;                     // cell pairs of a bin
+struct __attribute__((aligned(16))) PairLds {
+    float2 tile[kTileW * kTileLd];            // 16-byte aligned: the window reads are 16-byte loads
+    int cnt[kPairs];
+    int s_warp[kGatherThreads / 64];
+    float t1[2][kGatherThreads], t2[2][kGatherThreads];
+    int cell[2][kGatherThreads], idx[2][kGatherThreads];
+    int nlanes;
+};
+
+// Sort the subproblem's points (thread t holds point t in rec when t < d.z)
+// by cell pair into the lane slots of L; returns the number of lanes.  The
+// tile may be stored into L before the call (the first barrier publishes it).
+__device__ __forceinline__ int pairs_stage(PairLds& L, const int4& d, int O1, int O2, const PtRec<float>& rec)
+{
+    const int tid = threadIdx.x;
+    const bool have = tid < d.z;
+    const int cell = have ? min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin +
+                                min(max((rec.cell & 0xffff) - O1, 0), kBin - 1)
+                          : 0;
+    const int pr = cell >> 1;                                // cell pair: row * 8 + column / 2
+    if (tid < kPairs) L.cnt[tid] = 0;
+    L.idx[1][tid] = -1;
+    __syncthreads();
+    const int rk = have ? atomicAdd(&L.cnt[pr], 1) : 0;
+    __syncthreads();
+    const int lanes = tid < kPairs ? (L.cnt[tid] + 1) >> 1 : 0;
+    const int start = block_exclusive_scan(lanes, L.s_warp);
+    __syncthreads();
+    if (tid < kPairs) L.cnt[tid] = start;
+    if (tid == kGatherThreads - 1) L.nlanes = start + lanes;
+    __syncthreads();
+    if (have) {
+        const int l = L.cnt[pr] + (rk >> 1), k = rk & 1;
+        L.t1[k][l] = rec.t1;
+        L.t2[k][l] = rec.t2;
+        L.cell[k][l] = cell;
+        L.idx[k][l] = rec.idx;
+    }
+    __syncthreads();
+    return L.nlanes;
+}
+
+// The 13 x 13 sums of lane l's one or two points.
+__device__ __forceinline__ void pairs_mac(const PairLds& L, int l, const WinArgs& wa, float2* __restrict__ out)
+{
+    constexpr int W = kTileLd;
+    const int ca = L.cell[0][l], oa = L.idx[0][l];
+    const int ob = L.idx[1][l];
+    const bool two = ob >= 0;
+    const int cb = two ? L.cell[1][l] : ca;
+    const float fa1 = L.t1[0][l], fa2 = L.t2[0][l];
+    const float fb1 = two ? L.t1[1][l] : fa1, fb2 = two ? L.t2[1][l] : fa2;
+    const int r0 = ca / kBin, c0 = (ca % kBin) & ~1;          // window: rows r0.., columns c0..c0+13
+    const bool sa = (ca & 1) != 0, sb = (cb & 1) != 0;       // point in the odd cell: window shifted by one
+    float wa1[kW], wa2[kW], wb1[kW], wb2[kW];
+    taps(fa1, wa, wa1);
+    taps(fa2, wa, wa2);
+    taps(fb1, wa, wb1);
+    taps(fb2, wa, wb2);
+    float ea[kW + 1], eb[kW + 1];
+#pragma unroll
+    for (int j = 0; j <= kW; ++j) {
+        const float a0 = j < kW ? wa1[j] : 0.f, a1 = j > 0 ? wa1[j - 1] : 0.f;
+        const float b0 = j < kW ? wb1[j] : 0.f, b1 = j > 0 ? wb1[j - 1] : 0.f;
+        ea[j] = sa ? a1 : a0;
+        eb[j] = sb ? b1 : b0;
+    }
+    const float2* base = L.tile + r0 * W + c0;
+    float xa = 0.f, ya = 0.f, xb = 0.f, yb = 0.f;
+#pragma unroll
+    for (int b = 0; b < kW; ++b) {
+        float sxa = 0.f, sya = 0.f, sxb = 0.f, syb = 0.f;
+#pragma unroll
+        for (int j = 0; j <= kW; ++j) {
+            const float2 u = base[b * W + j];
+            sxa = fmaf(ea[j], u.x, sxa);
+            sya = fmaf(ea[j], u.y, sya);
+            sxb = fmaf(eb[j], u.x, sxb);
+            syb = fmaf(eb[j], u.y, syb);
+        }
+        xa = fmaf(wa2[b], sxa, xa);
+        ya = fmaf(wa2[b], sya, ya);
+        xb = fmaf(wb2[b], sxb, xb);
+        yb = fmaf(wb2[b], syb, yb);
+    }
+    // A lane with one point carries it in both slots (same inputs, so the same
+    // bits): the second store is unconditional, which keeps the compiler from
+    // sinking the second point's sums into a branch (it would then hold every
+    // tile row until that branch).
+    out[oa] = make_float2(xa, ya);
+    out[two ? ob : oa] = make_float2(xb, yb);
+}
+
+__device__ __forceinline__ void gather_pairs_body(int s, const float2* __restrict__ Gt, const BinGeom& geo,
+                                                  const PtRec<float>* __restrict__ pts, const int4* __restrict__ sub,
+                                                  const WinArgs& wa, float2* __restrict__ out)
+{
+    __shared__ PairLds L;
+    const int4 d = sub[s];
+    const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
+    PtRec<float> rec;
+    rec.t1 = rec.t2 = 0.f;
+    rec.cell = rec.idx = 0;
+    if ((int)threadIdx.x < d.z) rec = pts[d.y + threadIdx.x];
+    load_tile(L.tile, Gt, geo.L1, geo.L2, O1 - kK, O2 - kK);
+    const int nl = pairs_stage(L, d, O1, O2, rec);
+    if ((int)threadIdx.x < nl) pairs_mac(L, threadIdx.x, wa, out);
+}
+
+#ifndef NFFT2D_GATHER_WAVES
+#define NFFT2D_GATHER_WAVES 6
+#endif
+// The complex64 type-2 gather of a plan: one block per subproblem.
+__global__ __launch_bounds__(kGatherThreads) __attribute__((amdgpu_waves_per_eu(NFFT2D_GATHER_WAVES, 8))) void
+k_gather_pairs(const float2* __restrict__ Gt, BinGeom geo, const PtRec<float>* __restrict__ pts,
+               const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, float2* __restrict__ out)
+{
+    if ((int)blockIdx.x >= *nsub) return;
+    gather_pairs_body(blockIdx.x, Gt, geo, pts, sub, wa, out);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
+                                            const PtRec<T>* __restrict__ pts, const int4* __restrict__ sub,
+                                            const WinArgs& wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1,
+                                            cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g,
+                                            T* __restrict__ gx1, T* __restrict__ gx2)
+{
+#ifndef NFFT2D_SINGLE_POINT_GATHER
+    if constexpr (MODE == 0 && std::is_same<T, float>::value) {
+        gather_pairs_body(s, Gt, geo, pts, sub, wa, out);
+    } else
+#endif
+    {
+        gather_body_single<T, MODE>(s, Gt, geo, pts, sub, wa, out, out_d1, out_d2, g, gx1, gx2);
     }
 }
 
