@@ -172,19 +172,23 @@ extern "C" int nfft2d_ndft_eval(nfft2d_ndft_plan *p, nfft2d_c64 *result, const f
     hipStream_t s = (hipStream_t)stream;
     const dim3 grid((unsigned)((M + ndft::kPtsPerGroup - 1) / ndft::kPtsPerGroup));
     float2 *out = reinterpret_cast<float2 *>(result);
+#ifndef NDFT_SPLIT
+#define NDFT_SPLIT 2
+#endif
+    const dim3 block(64 * ndft::kWaves * NDFT_SPLIT);
     switch (p->prec) {
     case NFFT2D_NDFT_FP32:
-        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp32, 4>), grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp32, 4, NDFT_SPLIT>), grid, block, 0, s,
                            (const ndft::Slice<ndft::kNdftFp32> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
                            p->nch, p->nks);
         break;
     case NFFT2D_NDFT_BF16X3:
-        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftBf16x3, 16>), grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftBf16x3, 16, NDFT_SPLIT>), grid, block, 0, s,
                            (const ndft::Slice<ndft::kNdftBf16x3> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
                            p->nch, p->nks);
         break;
     default:
-        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp16, 16>), grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((ndft::k_ndft_t2<ndft::kNdftFp16, 16, NDFT_SPLIT>), grid, block, 0, s,
                            (const ndft::Slice<ndft::kNdftFp16> *)p->G, p->inv_scale, x, y, out, p->N1, p->N2, M,
                            p->nch, p->nks);
         break;

@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace ndft {
 
@@ -97,15 +98,19 @@ template <int P, int K> __device__ __forceinline__ void mma_part(const Left<P>& 
         acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.h, b.h, acc, 0, 0, 0);
     }
 }
-// One slice into two row tiles, the products interleaved so that no MFMA
-// waits on the one before it.
-template <int P, int K = 0>
-__device__ __forceinline__ void mma_pair(const Left<P>& a0, const Left<P>& a1, const Slice<P>& b, f16v& acc0,
-                                         f16v& acc1)
+// All products of one K-step: C slices into two row tiles, product part K
+// outermost, so that two MFMAs into the same accumulator are 2 C - 1 MFMAs
+// apart (an MFMA's result is ready about two MFMA issue slots after it starts).
+template <int P, int C, int K = 0>
+__device__ __forceinline__ void mma_step(const Left<P>& a0, const Left<P>& a1, const Slice<P> (&b)[C],
+                                         f16v (&acc0)[C], f16v (&acc1)[C])
 {
-    mma_part<P, K>(a0, b, acc0);
-    mma_part<P, K>(a1, b, acc1);
-    if constexpr (K + 1 < nprod<P>()) mma_pair<P, K + 1>(a0, a1, b, acc0, acc1);
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+        mma_part<P, K>(a0, b[t], acc0[t]);
+        mma_part<P, K>(a1, b[t], acc1[t]);
+    }
+    if constexpr (K + 1 < nprod<P>()) mma_step<P, C, K + 1>(a0, a1, b, acc0, acc1);
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
@@ -235,7 +240,15 @@ __global__ __launch_bounds__(256) void k_ndft_rowscale(const float2* __restrict_
 }
 
 // The plan pads the K-step count to a multiple of this with zero slices.
-constexpr int kKsPad = 2;
+constexpr int kKsPad = 4;
+// The point operand is evaluated exactly (phase reduction + sincospi) every
+// kKsExact K-steps and advanced by one complex product in between.
+constexpr int kKsExact = 4;
+static_assert(kKsPad % kKsExact == 0, "chunks start on an exact K-step");
+// K-steps of mode slices in flight ahead of the one being multiplied: the
+// L2-to-LDS copy of a K-step takes longer than the K-step's MFMAs, so the
+// copies run kAhead steps ahead (kAhead + 1 LDS buffers, rotating).
+constexpr int kAhead = 3;
 // Column tiles per chunk: 8 tiles = 4 groups of 32 k1 = 128 k1 per pass over K.
 constexpr int kChunkTiles = 8;
 constexpr int kChunkK1 = 16 * kChunkTiles;
@@ -254,6 +267,13 @@ __device__ __forceinline__ void copy16_to_lds(const uint4* gsrc, uint4* lds_base
 }
 
 __device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Wait until at most N vector-memory operations of this wave are in flight.
+template <int N> __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory"); }
+// Workgroup barrier without a memory fence: __syncthreads' release fence
+// waits for every copy in flight (vmcnt(0)), which would undo the copies that
+// run ahead.  Each wave waits for its own copies with wait_vm first; the
+// "memory" clobber keeps the compiler from moving LDS accesses across.
+__device__ __forceinline__ void barrier_no_fence() { asm volatile("s_barrier" ::: "memory"); }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -275,39 +295,59 @@ __device__ __forceinline__ void wave_sync()
 // of 32 k1) is transposed through LDS so that lane (h, row) folds columns
 // 16 h .. 16 h + 15 of its row with A[m][k1] = e^{-2 pi i k1 x1_m / N1}, A
 // advancing by e^{-2 pi i x1_m / N1} per column from an exact value every
-// EANCHOR columns.  The point operand of every K-step is evaluated exactly
-// (one phase reduction, then three products with the k2 step phasors).
-template <int P, int EANCHOR>
-__global__ __launch_bounds__(256, 1) void k_ndft_t2(const Slice<P>* __restrict__ G, const float* __restrict__ inv_scale,
-                                                    const float* __restrict__ x1, const float* __restrict__ x2,
-                                                    float2* __restrict__ out, int N1, int N2, int M, int nch, int nks)
+// EANCHOR columns.  The point operand is evaluated exactly (one phase
+// reduction, then three products with the k2 step phasors) on every
+// kKsExact-th K-step and advanced by e^{-2 pi i 8 x2 / N2} on the others.
+//
+// SPLIT = 2 runs two waves per SIMD: waves w and w + 4 hold the same points
+// and split each chunk's column tiles between them (128 accumulator registers
+// each instead of 256), so that one wave's VALU work and LDS waits run under
+// the other wave's MFMAs; both generate the same point operands, and the two
+// partial sums of a point meet in LDS at the end.
+template <int P, int EANCHOR, int SPLIT>
+__global__ __launch_bounds__(256 * SPLIT, 1) void k_ndft_t2(const Slice<P>* __restrict__ G,
+                                                            const float* __restrict__ inv_scale,
+                                                            const float* __restrict__ x1, const float* __restrict__ x2,
+                                                            float2* __restrict__ out, int N1, int N2, int M, int nch,
+                                                            int nks)
 {
-    constexpr int C = kChunkTiles, R = kRowTiles;
+    constexpr int C = kChunkTiles, R = kRowTiles, CW = C / SPLIT, NT = 256 * SPLIT;
     constexpr int kWords = C * 64 * (int)sizeof(Slice<P>) / 16;      // 16-byte words per K-step
-    constexpr int kWordsPerThread = kWords / 256;
-    static_assert(kWords % 256 == 0, "a K-step must split evenly over the workgroup");
-    // two distinct buffers (not one indexed array), so that the compiler can
-    // tell reads of one from the copy in flight into the other
+    constexpr int kWordsPerThread = kWords / NT;
+    static_assert(kWords % NT == 0, "a K-step must split evenly over the workgroup");
+    // distinct buffers (not one indexed array), so that the compiler can tell
+    // reads of one from the copies in flight into the others
+    static_assert(kAhead == 3, "four rotating buffers");
     __shared__ uint4 s_stage0[kWords];
     __shared__ uint4 s_stage1[kWords];
-    __shared__ float2 s_t[kWaves][32][33];
+    __shared__ uint4 s_stage2[kWords];
+    __shared__ uint4 s_stage3[kWords];
+    __shared__ float2 s_t[kWaves * SPLIT][32][33];
+    __shared__ float2 s_y[SPLIT > 1 ? kWaves * R * 32 : 1];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
+    const int pg = w % kWaves, hc = w / kWaves;                    // point group, column share
     const int h = lane >> 5, cc = lane & 31;
-    const int m0 = blockIdx.x * kPtsPerGroup + kPtsPerWave * w;   // first point of this wave
+    const int m0 = blockIdx.x * kPtsPerGroup + kPtsPerWave * pg;  // first point of this wave
     const double invN1 = 1.0 / N1, invN2 = 1.0 / N2;
     const int half1 = N1 / 2, half2 = N2 / 2;
     const int total = nch * nks;
     const uint4* Gw = reinterpret_cast<const uint4*>(G);
 
-    // K-step 0 in flight while the phasors are set up
+    // copy K-step gs (clamped to the last one) into buffer dst
+    auto issue = [&](int gs, uint4* dst) {
+        const uint4* src = Gw + (size_t)min(gs, total - 1) * kWords + 64 * w + lane;
 #pragma unroll
-    for (int i = 0; i < kWordsPerThread; ++i)
-        copy16_to_lds(Gw + 256 * i + 64 * w + lane, &s_stage0[256 * i + 64 * w]);
+        for (int i = 0; i < kWordsPerThread; ++i) copy16_to_lds(src + NT * i, dst + NT * i + 64 * w);
+    };
+    // K-steps 0 .. kAhead - 1 in flight while the phasors are set up
+    issue(0, s_stage0);
+    issue(1, s_stage1);
+    issue(2, s_stage2);
 
     // point of row cc of row tile r (left operand, and row cc in the epilogue)
     float xr1[R], xr2[R];
-    float2 w1[R], w2[R], w3[R], e1[R], y[R];
+    float2 w1[R], w2[R], w3[R], w8[R], z0[R], e1[R], y[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int m = m0 + 32 * r + cc;
@@ -316,18 +356,22 @@ __global__ __launch_bounds__(256, 1) void k_ndft_t2(const Slice<P>* __restrict__
         w1[r] = expi_neg(1, xr2[r], invN2);
         w2[r] = expi_neg(2, xr2[r], invN2);
         w3[r] = expi_neg(3, xr2[r], invN2);
+        w8[r] = expi_neg(8, xr2[r], invN2);
         e1[r] = expi_neg(1, xr1[r], invN1);
         y[r] = make_float2(0.f, 0.f);
     }
-    wait_vm_all();
-    __syncthreads();
+    wait_vm<(kAhead - 1) * kWordsPerThread>();    // K-step 0 has landed
+    barrier_no_fence();
 
-    f16v acc[R][C];
+    f16v acc[R][CW];
     // point operand of row tile r at K-step ks: e^{-2 pi i k2 x2 / N2} for
     // k2 = 8 ks + 4 h + 0..3 - N2/2, (Re, Im) interleaved
-    auto gen = [&](int r, int ks) {
+    // exact: z0 = e^{-2 pi i k2 x2 / N2} at the K-step's first k2 of this lane;
+    // otherwise z0 advances by 8 k2 from the previous K-step
+    auto gen = [&](int r, int ks, bool exact) {
+        z0[r] = exact ? expi_neg(8 * ks + 4 * h - half2, xr2[r], invN2) : cmul(z0[r], w8[r]);
         float2 z[4];
-        z[0] = expi_neg(8 * ks + 4 * h - half2, xr2[r], invN2);
+        z[0] = z0[r];
         z[1] = cmul(z[0], w1[r]);
         z[2] = cmul(z[0], w2[r]);
         z[3] = cmul(z[0], w3[r]);
@@ -343,57 +387,70 @@ __global__ __launch_bounds__(256, 1) void k_ndft_t2(const Slice<P>* __restrict__
     };
     Left<P> a[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[r] = gen(r, 0);
-    // one K-step: multiply the slices in `rd` with a[] while the next K-step is
-    // copied into `nx` and its point operands are generated
-    auto step = [&](int gs, int ks, const uint4* rd, uint4* nx) {
-        {
-            // the last K-step copies its own slices again into the idle buffer
-            // (no branch in the loop body)
-            const uint4* src = Gw + (size_t)min(gs + 1, total - 1) * kWords + 64 * w + lane;
+    for (int r = 0; r < R; ++r) a[r] = gen(r, 0, true);
+    // one K-step: multiply the slices in `rd` with a[] while K-step gs + kAhead
+    // is copied into `nx` (the buffer read one step earlier; past the end the
+    // last K-step is copied again, so the loop body has no branch) and the
+    // next point operands are generated
+    auto step = [&](int gs, int ks, const uint4* rd, uint4* nx, auto exact_next) {
+        constexpr bool kExact = decltype(exact_next)::value;
+        issue(gs + kAhead, nx);
+        Slice<P> b[CW];
 #pragma unroll
-            for (int i = 0; i < kWordsPerThread; ++i) copy16_to_lds(src + 256 * i, nx + 256 * i + 64 * w);
-        }
-        Slice<P> b[C];
-#pragma unroll
-        for (int t = 0; t < C; ++t) b[t] = load_slice<P>(rd, t, lane);
-#pragma unroll
-        for (int t = 0; t < C; ++t) mma_pair<P>(a[0], a[1], b[t], acc[0][t], acc[1][t]);
+        for (int t = 0; t < CW; ++t) b[t] = load_slice<P>(rd, CW * hc + t, lane);
+        mma_step<P, CW>(a[0], a[1], b, acc[0], acc[1]);
         const int kn = ks + 1 == nks ? 0 : ks + 1;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = gen(r, kn);
-        // schedule: the reads of 2 tiles, then per tile the reads of the tile
-        // two ahead, the tile's MFMAs (both row tiles) and a share of the next
-        // K-step's point-operand arithmetic, so that each MFMA group waits only
-        // for reads issued a group earlier and the arithmetic runs under the
-        // MFMAs (sched_group_barrier masks: 0x100 LDS read, 0x008 MFMA, 0x002 VALU)
+        for (int r = 0; r < R; ++r) a[r] = gen(r, kn, kExact);
+        // schedule: the reads of 2 tiles; then, for product part 0, per tile
+        // the reads of the tile two ahead and the tile's two MFMAs; then the
+        // MFMAs of the other parts; a share of the next K-step's point-operand
+        // arithmetic after each MFMA pair, so that it runs under the MFMAs
+        // (sched_group_barrier masks: 0x100 LDS read, 0x008 MFMA, 0x002 VALU)
         constexpr int kR = (int)sizeof(Slice<P>) / 16, kQ = nprod<P>();
+        constexpr int kV = (kExact ? 224 : 64) / (CW * kQ) + 1;
         __builtin_amdgcn_sched_group_barrier(0x100, 2 * kR, 0);
 #pragma unroll
-        for (int t = 0; t < C; ++t) {
-            if (t + 2 < C) __builtin_amdgcn_sched_group_barrier(0x100, kR, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 2 * kQ, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 28, 0);
+        for (int q = 0; q < kQ; ++q) {
+#pragma unroll
+            for (int t = 0; t < CW; ++t) {
+                if (q == 0 && t + 2 < CW) __builtin_amdgcn_sched_group_barrier(0x100, kR, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, kV, 0);
+            }
         }
-        wait_vm_all();
-        __syncthreads();
+        // every MFMA of the step issues before the wait for the copy in flight
+        // (left alone, the scheduler hoists the wait and the barrier above most
+        // of the MFMAs, which then wait for the copy issued at the step's start)
+        __builtin_amdgcn_sched_barrier(0);
+        wait_vm<(kAhead - 1) * kWordsPerThread>();   // K-step gs + 1 has landed
+        barrier_no_fence();
     };
-    // nks is even, so K-step ks of every chunk reads buffer ks % 2
+    // nks is a multiple of 4, so K-step ks of every chunk reads buffer ks % 4
+    // and the operands of K-steps 0, 4, 8, ... are the exact ones
+    using Exact = std::true_type;
+    using Advance = std::false_type;
+    static_assert(kKsExact == 4, "the loop body below covers 4 K-steps");
     for (int ch = 0; ch < nch; ++ch) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int t = 0; t < C; ++t) acc[r][t] = f16v{};
-        for (int ks = 0; ks < nks; ks += 2) {
-            step(ch * nks + ks, ks, s_stage0, s_stage1);
-            step(ch * nks + ks + 1, ks + 1, s_stage1, s_stage0);
+            for (int t = 0; t < CW; ++t) acc[r][t] = f16v{};
+        for (int ks = 0; ks < nks; ks += 4) {
+            const int g0 = ch * nks + ks;
+            step(g0, ks, s_stage0, s_stage3, Advance{});
+            step(g0 + 1, ks + 1, s_stage1, s_stage0, Advance{});
+            step(g0 + 2, ks + 2, s_stage2, s_stage1, Advance{});
+            step(g0 + 3, ks + 3, s_stage3, s_stage2, Exact{});
         }
-        // epilogue of chunk ch: y += sum_k1 A[m][k1] T[m][k1] over its 128 k1
+        // epilogue of chunk ch: y += sum_k1 A[m][k1] T[m][k1] over the wave's
+        // groups of 32 k1 of the chunk (group gg = its Re and Im tiles)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
 #pragma unroll
-            for (int g = 0; g < C / 2; ++g) {
-                const int j1c = ch * kChunkK1 + 32 * g + cc;        // k1 index of column cc
+            for (int g = 0; g < CW / 2; ++g) {
+                const int gg = (CW / 2) * hc + g;
+                const int j1c = ch * kChunkK1 + 32 * gg + cc;       // k1 index of column cc
                 const float isc = (P == kNdftFp16 && j1c < N1) ? inv_scale[j1c] : 1.0f;
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
@@ -401,7 +458,7 @@ __global__ __launch_bounds__(256, 1) void k_ndft_t2(const Slice<P>* __restrict__
                     s_t[w][ti][cc] = make_float2(acc[r][2 * g][q] * isc, acc[r][2 * g + 1][q] * isc);
                 }
                 wave_sync();
-                const int k1s = ch * kChunkK1 + 32 * g + 16 * h - half1;   // k1 of this lane's first column
+                const int k1s = ch * kChunkK1 + 32 * gg + 16 * h - half1;  // k1 of this lane's first column
                 float2 A = make_float2(1.f, 0.f);
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
@@ -415,14 +472,26 @@ __global__ __launch_bounds__(256, 1) void k_ndft_t2(const Slice<P>* __restrict__
             }
         }
     }
-    // the two halves of each row
+    // the two halves of each row, then (SPLIT = 2) the two column shares
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        float2 v = y[r];
-        v.x += __shfl_xor(v.x, 32, 64);
-        v.y += __shfl_xor(v.y, 32, 64);
-        const int m = m0 + 32 * r + cc;
-        if (h == 0 && m < M) out[m] = v;
+        y[r].x += __shfl_xor(y[r].x, 32, 64);
+        y[r].y += __shfl_xor(y[r].y, 32, 64);
+        if (SPLIT > 1 && hc == 1 && h == 0) s_y[(pg * R + r) * 32 + cc] = y[r];
+    }
+    if constexpr (SPLIT > 1) __syncthreads();
+    if (hc == 0 && h == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 v = y[r];
+            if constexpr (SPLIT > 1) {
+                const float2 o = s_y[(pg * R + r) * 32 + cc];
+                v.x += o.x;
+                v.y += o.y;
+            }
+            const int m = m0 + 32 * r + cc;
+            if (m < M) out[m] = v;
+        }
     }
 }
 
