@@ -453,7 +453,7 @@ def main():
                                "with the grid passes)"},
         "stage_ms": {"setpts": t2["setpts_ms"], "grid_a_b": t2["grid_ms"], "gather_c": t2["gather_ms"]},
         "type2_hbm_frac": t2["type2_bytes"] / t2["s_exec"] / (HBM_PEAK_GBS * 1e9),
-        "roofline": {"kernel": "k_gather<float,0> (stage c)", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_gather_pairs (stage c)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": t2["gather_bytes"]},
     }
