@@ -829,7 +829,7 @@ static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
 static int batch_cols_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
 static constexpr int kBatchMinMaxpts = 64;   // smallest subproblem size the autotuner tries
 static constexpr int kBatchGatherBlocks = 2048;
-static constexpr int kBatchGatherWaveBlocks = 256 * 5;   // 5 blocks of 4 waves per CU (LDS: 4 tiles each)
+static constexpr int kBatchGatherWaveBlocks = 256 * 4;   // 4 blocks of 4 waves per CU (LDS: 38 KB each)
 
 static void batch_free(nfft2d_batch *b)
 {

@@ -186,19 +186,22 @@ __global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ 
     });
 }
 
+#ifndef NFFT2D_BATCH_GATHER_WAVES
+#define NFFT2D_BATCH_GATHER_WAVES 4
+#endif
 // Persistent gather over the batch's subproblem list (total = base[count]),
 // one wave per subproblem: wave g = 4 blockIdx.x + w takes subproblems g,
 // g + 4 gridDim.x, ...
-__global__ __launch_bounds__(256) void kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATCH_GATHER_WAVES, 8))) void kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
                                                    const int* __restrict__ total, WinArgs wa)
 {
-    __shared__ float2 tiles[4][kGwTileElems];
+    __shared__ WavePairLds lds[4];
     const int w = threadIdx.x >> 6;
     const int n = *total;
     for (int s = blockIdx.x * 4 + w; s < n; s += gridDim.x * 4) {
         const int4 d = bsub[s];
         const BatchDesc& t = bd[d.w];
-        gather_wave(d, tiles[w], t.Gt, t.geo, t.pts, wa, t.out);
+        gather_wave_pairs(d, lds[w], t.Gt, t.geo, t.pts, wa, t.out);
     }
 }
 

@@ -993,8 +993,10 @@ __device__ __forceinline__ int pairs_stage(PairLds& L, const int4& d, int O1, in
     return L.nlanes;
 }
 
-// The 13 x 13 sums of lane l's one or two points.
-__device__ __forceinline__ void pairs_mac(const PairLds& L, int l, const WinArgs& wa, float2* __restrict__ out)
+// The 13 x 13 sums of lane l's one or two points (L: any layout with the
+// tile and the slot arrays of PairLds).
+template <class Lds>
+__device__ __forceinline__ void pairs_mac(const Lds& L, int l, const WinArgs& wa, float2* __restrict__ out)
 {
     constexpr int W = kTileLd;
     const int ca = L.cell[0][l], oa = L.idx[0][l];
@@ -1108,19 +1110,26 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// (c) type-2 gather, one WAVE per subproblem (complex64, MODE 0): the wave
-// stages the subproblem's 28 x 28 tile in its own LDS tile (13 loads per lane,
-// all in flight) and then walks the points in chunks of 64, one per lane.
-// Four subproblems share a 256-thread block, so many more tiles are in flight
-// per CU than with one subproblem per block; this pays off when subproblems
-// are small (sparse point sets).  Same arithmetic per point as k_gather<float, 0>.
-constexpr int kGwTileElems = (kBin + 2 * kK) * kTileLd;
+// (c) type-2 gather, one WAVE per subproblem (complex64, MODE 0), for the
+// persistent batch gather.  The wave stages the subproblem's 28 x 28 tile in
+// its own LDS tile (13 loads per lane, all in flight), then takes its points
+// 64 at a time: it counting-sorts them by cell pair in LDS (wave-level: LDS
+// atomics, a wave scan, no block barrier) and runs pairs_mac, as the block
+// form does.  Many subproblems are in flight per CU (four per block), which
+// pays off when they are small (sparse point sets).  Same arithmetic per point
+// as k_gather_pairs, so the same bits.
+struct __attribute__((aligned(16))) WavePairLds {
+    float2 tile[kTileW * kTileLd];            // 16-byte aligned: the window reads are 16-byte loads
+    int cnt[kPairs];
+    float t1[2][64], t2[2][64];
+    int cell[2][64], idx[2][64];
+};
 
-__device__ __forceinline__ void gather_wave(const int4& d, float2* tile, const float2* __restrict__ Gt,
-                                            const BinGeom& geo, const PtRec<float>* __restrict__ pts,
-                                            const WinArgs& wa, float2* __restrict__ out)
+__device__ __forceinline__ void gather_wave_pairs(const int4& d, WavePairLds& L, const float2* __restrict__ Gt,
+                                                  const BinGeom& geo, const PtRec<float>* __restrict__ pts,
+                                                  const WinArgs& wa, float2* __restrict__ out)
 {
-    constexpr int W = kBin + 2 * kK;
+    constexpr int W = kTileW;
     constexpr int kPer = (W * W + 63) / 64;                  // 13
     const int lane = threadIdx.x & 63;
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
@@ -1148,53 +1157,51 @@ __device__ __forceinline__ void gather_wave(const int4& d, float2* tile, const f
         const int e = lane + 64 * q;
         if (e < W * W) {
             const int r = e / W, c = e - r * W;
-            tile[r * kTileLd + c] = v[q];
+            L.tile[r * kTileLd + c] = v[q];
         }
     }
-    wave_sync_lds();
     for (int base = 0; base < d.z; base += 64) {
-        const int i = base + lane;
+        const bool have = base + lane < d.z;
         PtRec<float> nxt;
         nxt.t1 = nxt.t2 = 0.f;
         nxt.cell = nxt.idx = 0;
-        if (i + 64 < d.z) nxt = pts[d.y + i + 64];           // next chunk, in flight during this one
-        if (i < d.z) {
-            const int r0 = min(max((rec.cell >> 16) - O2, 0), kBin - 1);
-            const int c0 = min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
-            float w1[kW], w2[kW];
-            taps(rec.t1, wa, w1);
-            taps(rec.t2, wa, w2);
-            const float2* bp = tile + r0 * kTileLd + c0;
-            float ax = 0.f, ay = 0.f;
+        if (base + 64 + lane < d.z) nxt = pts[d.y + base + 64 + lane];   // next chunk, in flight during this one
+        const int cell = have ? min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin +
+                                    min(max((rec.cell & 0xffff) - O1, 0), kBin - 1)
+                              : 0;
+        const int pr = cell >> 1;
+        L.cnt[lane] = 0;
+        L.cnt[lane + 64] = 0;
+        L.idx[1][lane] = -1;
+        wave_sync_lds();
+        const int rk = have ? atomicAdd(&L.cnt[pr], 1) : 0;
+        wave_sync_lds();
+        // lanes per cell pair (ceil(n / 2)), exclusive prefix over the 128 pairs (2 per lane)
+        const int l0 = (L.cnt[2 * lane] + 1) >> 1, l1 = (L.cnt[2 * lane + 1] + 1) >> 1;
+        int inc = l0 + l1;
 #pragma unroll
-            for (int b = 0; b < kW; ++b) {
-                float sx = 0.f, sy = 0.f;
-#pragma unroll
-                for (int a = 0; a < kW; ++a) {
-                    const float2 u = bp[b * kTileLd + a];
-                    sx = fmaf(w1[a], u.x, sx);
-                    sy = fmaf(w1[a], u.y, sy);
-                }
-                ax = fmaf(w2[b], sx, ax);
-                ay = fmaf(w2[b], sy, ay);
-            }
-            out[rec.idx] = make_float2(ax, ay);
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
         }
+        const int nl = __shfl(inc, 63, 64);
+        const int ex = inc - l0 - l1;
+        wave_sync_lds();
+        L.cnt[2 * lane] = ex;
+        L.cnt[2 * lane + 1] = ex + l0;
+        wave_sync_lds();
+        if (have) {
+            const int l = L.cnt[pr] + (rk >> 1), k = rk & 1;
+            L.t1[k][l] = rec.t1;
+            L.t2[k][l] = rec.t2;
+            L.cell[k][l] = cell;
+            L.idx[k][l] = rec.idx;
+        }
+        wave_sync_lds();
+        if (lane < nl) pairs_mac(L, lane, wa, out);
+        wave_sync_lds();                                     // slots reused by the next chunk
         rec = nxt;
     }
-    wave_sync_lds();                                         // the tile is reused by the wave's next subproblem
-}
-
-// One wave per subproblem of a plan: wave g = 4 blockIdx.x + w takes subproblem g.
-__global__ __launch_bounds__(256) void k_gather_w(const float2* __restrict__ Gt, BinGeom geo,
-                                                  const PtRec<float>* __restrict__ pts, const int4* __restrict__ sub,
-                                                  const int* __restrict__ nsub, WinArgs wa, float2* __restrict__ out)
-{
-    __shared__ float2 tiles[4][kGwTileElems];
-    const int w = threadIdx.x >> 6;
-    const int s = blockIdx.x * 4 + w;
-    if (s >= *nsub) return;
-    gather_wave(sub[s], tiles[w], Gt, geo, pts, wa, out);
 }
 
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates
