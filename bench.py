@@ -368,6 +368,86 @@ def bench_large(args, dev):
             "hbm_frac": (200 * N * N + 16 * M) / s / (HBM_PEAK_GBS * 1e9)}
 
 
+def direct_type1_subset(d: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor, k1: torch.Tensor, k2: torch.Tensor,
+                        N1: int, N2: int) -> torch.Tensor:
+    """FP64 direct sum F_s = sum_l d_l e^{-2 pi i (k1_s x1_l/N1 + k2_s x2_l/N2)} for selected modes s, in point chunks."""
+    out = torch.zeros(k1.numel(), dtype=torch.complex128, device=d.device)
+    for a in range(0, d.numel(), 65536):
+        ph = (torch.outer(x1[a:a + 65536].double(), k1.double()) / N1 +
+              torch.outer(x2[a:a + 65536].double(), k2.double()) / N2)
+        E = torch.polar(torch.ones_like(ph), -2 * np.pi * (ph - torch.floor(ph)))
+        out += (d[a:a + 65536].to(torch.complex128)[:, None] * E).sum(dim=0)
+    return out
+
+
+def bench_type1(args, dev):
+    """Type-1 (the reference's NFFT2_2D_GPU direction) on the polar set: 512 x 512 modes, M = 1048576,
+    step = setpts + type-1 (spread, FFT, crop + deconvolution), graph-replayed; and the type-1 alone."""
+    import miscellaneous_amd as m
+
+    N = 512
+    x1n, x2n = polar_points(N)
+    M = len(x1n)
+    gen = torch.Generator().manual_seed(SEED + 7)
+    d = cn01(gen, (M,), dev)
+    x1 = torch.from_numpy(x1n).to(dev)
+    x2 = torch.from_numpy(x2n).to(dev)
+    F = torch.empty(N, N, dtype=torch.complex64, device=dev)
+    plan = m.Plan(N, N, torch.complex64, device=dev)
+
+    def step():
+        plan.setpts(x1, x2)
+        plan.type1(d, out=F)
+
+    def step_exec():
+        plan.type1(d, out=F)
+
+    g = capture(step)
+    s = time_graph(g, max(args.steps // 4, 10), 3)
+    g2 = capture(step_exec)
+    s2 = time_graph(g2, max(args.steps // 4, 10), 3)
+    rng = np.random.default_rng(SEED + 7)
+    sel = torch.from_numpy(rng.choice(N * N, 256, replace=False)).to(dev)
+    k1 = sel // N - N // 2
+    k2 = sel % N - N // 2
+    ref = direct_type1_subset(d, x1, x2, k1, k2, N, N)
+    got = F.reshape(-1)[sel].to(torch.complex128)
+    rel = ((got - ref).norm() / ref.norm()).item()
+    nbytes = 176 * N * N + 16 * M
+    return {"config": "type-1 NFFT (NFFT2_2D_GPU direction), 512x512 modes, M=1048576 polar points, K=6, c=2; "
+                      "step = setpts + type-1",
+            "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3,
+            "exec_only_ms": s2 * 1e3, "rel_l2_256_modes": rel,
+            "hbm_frac_exec": nbytes / s2 / (HBM_PEAK_GBS * 1e9)}
+
+
+def bench_fp64(args, dev):
+    """The complex128 type-2 path on the polar set (512 x 512, M = 1048576): setpts + type-2, graph-replayed."""
+    import miscellaneous_amd as m
+
+    N = 512
+    x1n, x2n = polar_points(N)
+    M = len(x1n)
+    gen = torch.Generator().manual_seed(SEED + 8)
+    f = cn01(gen, (N, N), dev).to(torch.complex128)
+    x1 = torch.from_numpy(x1n.astype(np.float64)).to(dev)
+    x2 = torch.from_numpy(x2n.astype(np.float64)).to(dev)
+    y = torch.empty(M, dtype=torch.complex128, device=dev)
+    plan = m.Plan(N, N, torch.complex128, device=dev)
+
+    def step():
+        plan.setpts(x1, x2)
+        plan.type2(f, out=y)
+
+    g = capture(step)
+    s = time_graph(g, max(args.steps // 4, 10), 3)
+    sel = torch.from_numpy(np.random.default_rng(SEED + 8).choice(M, 4096, replace=False)).to(dev)
+    ref = direct_type2_subset(f, x1[sel], x2[sel])
+    rel = ((y[sel] - ref).norm() / ref.norm()).item()
+    return {"config": "complex128 type-2 NFFT, 512x512 modes, M=1048576 polar points, K=6, c=2; step = setpts + type-2",
+            "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3, "rel_l2": rel}
+
+
 def cpu_baseline(t2, seconds: float):
     import oracle
 
@@ -422,6 +502,8 @@ def main():
         extra["batch"] = bench_batch(args, dev)
         extra["ndft"] = bench_ndft(args, dev)
         extra["large_1024"] = bench_large(args, dev)
+        extra["type1_polar512"] = bench_type1(args, dev)
+        extra["fp64_polar512"] = bench_fp64(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]
