@@ -100,6 +100,26 @@ def time_graph(g, steps: int, warmup: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
+def stage_ms(fn, n):
+    """Per-stage launch duration: HIP events on the stream the kernels run on (torch's current
+    stream, which the library launches on), around n back-to-back launches of the stage (the queue
+    stays full, so the average is the stage's device time per launch)."""
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def stage_roofline(nbytes: int, ms: float) -> dict:
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"algorithmic_bytes": nbytes, "us": ms * 1e3, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
 def bench_type2(args, dev):
     import miscellaneous_amd as m
 
@@ -120,20 +140,6 @@ def bench_type2(args, dev):
     sel = torch.from_numpy(np.random.default_rng(SEED).choice(M, 4096, replace=False)).to(dev)
     y_ref = direct_type2_subset(f, x1[sel], x2[sel])
     rel = ((y[sel].to(torch.complex128) - y_ref).norm() / y_ref.norm()).item()
-
-    # per-stage launch durations: HIP events on the stream the kernels run on,
-    # around n back-to-back launches of the stage (the queue stays full, so
-    # the average is the stage's device time per launch)
-    def stage_ms(fn, n):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(n):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / n
 
     n_ev = max(args.steps, 20)
     setpts_ms = stage_ms(lambda: plan.setpts(x1, x2), n_ev)
@@ -406,6 +412,9 @@ def bench_type1(args, dev):
     s = time_graph(g, max(args.steps // 4, 10), 3)
     g2 = capture(step_exec)
     s2 = time_graph(g2, max(args.steps // 4, 10), 3)
+    n_ev = max(args.steps, 20)
+    spread_ms = stage_ms(lambda: plan.type1_spread(d), n_ev)
+    finish_ms = stage_ms(lambda: plan.type1_finish(F), n_ev)
     rng = np.random.default_rng(SEED + 7)
     sel = torch.from_numpy(rng.choice(N * N, 256, replace=False)).to(dev)
     k1 = sel // N - N // 2
@@ -418,6 +427,8 @@ def bench_type1(args, dev):
                       "step = setpts + type-1",
             "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3,
             "exec_only_ms": s2 * 1e3, "rel_l2_256_modes": rel,
+            "stage_roofline": {"d_spread": stage_roofline(32 * N * N + 16 * M, spread_ms),
+                               "fft_crop_deconvolve": stage_roofline(144 * N * N, finish_ms)},
             "hbm_frac_exec": nbytes / s2 / (HBM_PEAK_GBS * 1e9)}
 
 
@@ -534,6 +545,8 @@ def main():
                        "note": "setpts and type-2 serialised on one stream (the step overlaps the tile sort "
                                "with the grid passes)"},
         "stage_ms": {"setpts": t2["setpts_ms"], "grid_a_b": t2["grid_ms"], "gather_c": t2["gather_ms"]},
+        "stage_roofline": {"a_b_precomp_pad_fft": stage_roofline(168 * t2["N"] * t2["N"], t2["grid_ms"]),
+                           "c_gather": stage_roofline(t2["gather_bytes"], t2["gather_ms"])},
         "type2_hbm_frac": t2["type2_bytes"] / t2["s_exec"] / (HBM_PEAK_GBS * 1e9),
         "roofline": {"kernel": "k_gather_pairs (stage c)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
