@@ -203,13 +203,14 @@ def bench_grad(args, dev):
     def loss_now() -> float:                       # eager, outside the graph
         return float(((plan.type2(f) - y_obs).abs() ** 2).sum().item())
 
+    neg2_y_obs = -2 * y_obs                        # constant of the loss, outside the step
+
     def step():
         plan.type2(f, out=y)
-        r = y - y_obs
-        g = 2 * r                                  # dL/d(conj y), times 2
-        plan.adjoint(g, out=gf)                    # dL/df
-        plan.type2_grad_x(g, f)                    # dL/dx (derivative-window gather)
-        f.sub_(lr * gf)
+        g = torch.add(neg2_y_obs, y, alpha=2)      # 2 (y - y_obs) = dL/d(conj y), times 2
+        plan.type2_grad_x_grid(g)                  # dL/dx (derivative-window gather on the forward's grid)
+        plan.adjoint(g, out=gf)                    # dL/df (overwrites the grid)
+        f.add_(gf, alpha=-lr)
 
     g_step = capture(step)
     f.zero_()

@@ -133,6 +133,11 @@ int nfft2d_type2_dx(nfft2d_plan *plan, void *y, void *dy1, void *dy2, const void
  * forward and the backward). */
 int nfft2d_type2_grad_x(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, const void *data,
                         void *stream);
+/* The same gradient on the oversampled grid the plan already holds: valid
+ * right after nfft2d_type2 / nfft2d_type2_grid of the same data, before any
+ * type-1 / adjoint / spread on the plan (those overwrite the grid).  Saves
+ * the pre-compensation and FFT passes of the recomputation. */
+int nfft2d_type2_grad_x_grid(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, void *stream);
 
 /* The stages of the transforms, for callers that time them or reuse the
  * oversampled grid (which lives in the plan between the calls):

@@ -5,8 +5,9 @@ backward:  grad_f = A^H g                       (nfft2d_adjoint, the adjoint NFF
            grad_x = Re(conj(g) dy/dx)            (nfft2d_type2_grad_x, the
                                                   derivative-window gather)
 with g the incoming gradient in PyTorch's convention for complex outputs.
-The backward recomputes the oversampled grid from f instead of storing it
-between forward and backward.
+The backward uses the oversampled grid the forward left in the plan when
+nothing has rewritten it since (the usual forward-then-backward order), and
+recomputes it from f otherwise; it never keeps a copy of its own.
 
 The reference never differentiates its NFFT (its only optimisers are
 derivative-free: fminsearch and PSO, e.g. Synthesis/Aper_Synth_1D/Aper_Synth_1D.m:161,
@@ -28,6 +29,7 @@ class NFFT2DType2(torch.autograd.Function):
         y = plan.type2(f)
         ctx.plan = plan
         ctx.version = plan.pts_version
+        ctx.grid_version = plan.grid_version
         ctx.save_for_backward(f, x1c, x2c)
         return y
 
@@ -40,12 +42,16 @@ class NFFT2DType2(torch.autograd.Function):
             ctx.version = plan.pts_version
         g = g.contiguous()
         grad_f = grad_x1 = grad_x2 = None
-        if ctx.needs_input_grad[0]:
-            grad_f = plan.adjoint(g).reshape(f.shape)
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
-            gx1, gx2 = plan.type2_grad_x(g, f)
+            # before the adjoint, which overwrites the grid
+            if plan.grid_version == ctx.grid_version:
+                gx1, gx2 = plan.type2_grad_x_grid(g)
+            else:
+                gx1, gx2 = plan.type2_grad_x(g, f)
             grad_x1 = gx1 if ctx.needs_input_grad[1] else None
             grad_x2 = gx2 if ctx.needs_input_grad[2] else None
+        if ctx.needs_input_grad[0]:
+            grad_f = plan.adjoint(g).reshape(f.shape)
         return grad_f, grad_x1, grad_x2, None
 
 

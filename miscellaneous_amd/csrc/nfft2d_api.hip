@@ -697,6 +697,20 @@ extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, co
     return gather<double, 1>(p, y, dy1, dy2, nullptr, nullptr, nullptr, s);
 }
 
+// The derivative gather of nfft2d_type2_grad_x on the oversampled grid the
+// plan holds (left by nfft2d_type2 / nfft2d_type2_grid of the same data, with
+// no spread since): a backward that runs right after its forward skips
+// recomputing the grid.
+extern "C" int nfft2d_type2_grad_x_grid(nfft2d_plan *p, void *gx1, void *gx2, const void *g, void *stream)
+{
+    int rc = check_exec(p, gx1, g);
+    if (rc || p->M == 0) return rc;
+    if (!gx2) return nfft2d_fail(NFFT2D_ERR_ARG, "type2_grad_x_grid: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    return p->dtype == NFFT2D_C64 ? gather<float, 2>(p, nullptr, nullptr, nullptr, g, gx1, gx2, s)
+                                  : gather<double, 2>(p, nullptr, nullptr, nullptr, g, gx1, gx2, s);
+}
+
 extern "C" int nfft2d_type2_grad_x(nfft2d_plan *p, void *gx1, void *gx2, const void *g, const void *data, void *stream)
 {
     int rc = check_exec(p, gx1, data);

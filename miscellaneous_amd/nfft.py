@@ -83,6 +83,7 @@ class Plan:
                   "nfft2d_plan_create")
         self.M = None
         self.pts_version = 0
+        self.grid_version = 0        # bumped by every call that rewrites the oversampled grid
 
     # -------------------------------------------------------------- lifetime
     def close(self) -> None:
@@ -133,6 +134,7 @@ class Plan:
         _require(out, self.dtype, self.M, "out", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type2(self._h, _ptr(out), _ptr(f), _stream(self.device)), "nfft2d_type2")
+            self.grid_version += 1
         return out
 
     def setpts_type2(self, x: torch.Tensor, y: torch.Tensor, f: torch.Tensor,
@@ -150,6 +152,7 @@ class Plan:
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_setpts_type2(self._h, _ptr(out), _ptr(f), _ptr(x), _ptr(y), M,
                                                 _stream(self.device)), "nfft2d_setpts_type2")
+            self.grid_version += 1
         self.M = M
         self.pts_version += 1
         return out
@@ -163,6 +166,7 @@ class Plan:
         _require(out, self.dtype, self.N1 * self.N2, "out", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type1(self._h, _ptr(out), _ptr(d), _stream(self.device)), "nfft2d_type1")
+            self.grid_version += 1
         return out
 
     def adjoint(self, g: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
@@ -174,6 +178,7 @@ class Plan:
         _require(out, self.dtype, self.N1 * self.N2, "out", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_adjoint(self._h, _ptr(out), _ptr(g), _stream(self.device)), "nfft2d_adjoint")
+            self.grid_version += 1
         return out
 
     # -------------------------------------------------------------- stages
@@ -182,6 +187,7 @@ class Plan:
         _require(f, self.dtype, self.N1 * self.N2, "f", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type2_grid(self._h, _ptr(f), _stream(self.device)), "nfft2d_type2_grid")
+            self.grid_version += 1
 
     def type2_interp(self, out: torch.Tensor) -> torch.Tensor:
         """Stage (c): gather the plan's grid at the plan's points into out (M)."""
@@ -197,6 +203,7 @@ class Plan:
         _require(d, self.dtype, self.M, "d", self.device)
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type1_spread(self._h, _ptr(d), _stream(self.device)), "nfft2d_type1_spread")
+            self.grid_version += 1
 
     def type1_finish(self, out: torch.Tensor, sign: int = -1) -> torch.Tensor:
         """FFT + crop + deconvolution of the plan's grid into out (N1 x N2); sign +1 = adjoint."""
@@ -216,6 +223,7 @@ class Plan:
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type2_dx(self._h, _ptr(y), _ptr(d1), _ptr(d2), _ptr(f), _stream(self.device)),
                   "nfft2d_type2_dx")
+            self.grid_version += 1
         return y, d1, d2
 
     def type2_grad_x(self, g: torch.Tensor, f: torch.Tensor):
@@ -228,6 +236,19 @@ class Plan:
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_type2_grad_x(self._h, _ptr(gx1), _ptr(gx2), _ptr(g), _ptr(f),
                                                 _stream(self.device)), "nfft2d_type2_grad_x")
+            self.grid_version += 1
+        return gx1, gx2
+
+    def type2_grad_x_grid(self, g: torch.Tensor):
+        """type2_grad_x on the oversampled grid the plan holds: valid while grid_version is the one
+        right after the type2 / type2_grid of the same f (no type-1, adjoint or spread since)."""
+        self._need_points()
+        _require(g, self.dtype, self.M, "g", self.device)
+        gx1 = torch.empty(self.M, dtype=self.rdtype, device=self.device)
+        gx2 = torch.empty_like(gx1)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type2_grad_x_grid(self._h, _ptr(gx1), _ptr(gx2), _ptr(g), _stream(self.device)),
+                  "nfft2d_type2_grad_x_grid")
         return gx1, gx2
 
 
