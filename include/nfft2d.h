@@ -138,6 +138,14 @@ int nfft2d_type2_grad_x(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, 
  * type-1 / adjoint / spread on the plan (those overwrite the grid).  Saves
  * the pre-compensation and FFT passes of the recomputation. */
 int nfft2d_type2_grad_x_grid(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, void *stream);
+/* Least squares L = ||A(x) data - y_obs||^2 in one pass over the points:
+ *   r[l] = 2 (y_l - y_obs[l])  (M complex; dL/d(conj y) times 2, the input of
+ *          nfft2d_adjoint for dL/d data),
+ *   gx1[l], gx2[l] = Re(conj(r[l]) dy_l/dx_l), Re(conj(r[l]) dy_l/dy_l)  (M reals),
+ * with the value and derivative sums sharing their tile reads.  Leaves the
+ * oversampled grid of `data` in the plan (as nfft2d_type2 does). */
+int nfft2d_type2_lsq(nfft2d_plan *plan, void *r, void *gx1, void *gx2, const void *y_obs, const void *data,
+                     void *stream);
 
 /* The stages of the transforms, for callers that time them or reuse the
  * oversampled grid (which lives in the plan between the calls):

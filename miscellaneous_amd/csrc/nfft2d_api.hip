@@ -697,6 +697,24 @@ extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, co
     return gather<double, 1>(p, y, dy1, dy2, nullptr, nullptr, nullptr, s);
 }
 
+// Least squares in one gather: y = A(x) data, r = 2 (y - y_obs) and the
+// coordinate gradient of |A(x) data - y_obs|^2, Re(conj(r) dy/dx) (the value
+// and derivative sums share their tile reads).
+extern "C" int nfft2d_type2_lsq(nfft2d_plan *p, void *r, void *gx1, void *gx2, const void *y_obs, const void *data,
+                                void *stream)
+{
+    int rc = check_exec(p, r, data);
+    if (rc || p->M == 0) return rc;
+    if (!gx1 || !gx2 || !y_obs) return nfft2d_fail(NFFT2D_ERR_ARG, "type2_lsq: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = t2_grid<float>(p, data, s))) return rc;
+        return gather<float, 3>(p, r, nullptr, nullptr, y_obs, gx1, gx2, s);
+    }
+    if ((rc = t2_grid<double>(p, data, s))) return rc;
+    return gather<double, 3>(p, r, nullptr, nullptr, y_obs, gx1, gx2, s);
+}
+
 // The derivative gather of nfft2d_type2_grad_x on the oversampled grid the
 // plan holds (left by nfft2d_type2 / nfft2d_type2_grid of the same data, with
 // no spread since): a backward that runs right after its forward skips

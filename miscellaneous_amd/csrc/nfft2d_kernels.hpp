@@ -826,7 +826,8 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
 }
 
 // (c) type-2 gather over one subproblem (<= 256 points of one bin), MODE 0:
-// out[l] = y_l;  MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;  MODE 2:
+// out[l] = y_l;  MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;  MODE 3:
+// least squares, g = y_obs: out = r = 2 (y - y_obs) and gx = Re(conj(r) dy/dx);  MODE 2:
 // gradient Re(conj(g_l) dy_l/dx) into gx1/gx2.
 // y_l = sum_{m2} w2[m2] sum_{m1} w1[m1] G[(mu2+m2-K) mod L2][(mu1+m1-K) mod L1]
 // (BP_FBP_FFBP/NFFT1_2D.m:61-67 on the unshifted grid).  The block first
@@ -931,10 +932,16 @@ __device__ __forceinline__ void gather_body_single(int s, const cplx<T>* __restr
             out[o] = mk(ax, ay);
             out_d1[o] = mk(cc * bx, cc * by);
             out_d2[o] = mk(cc * cx, cc * cy);
-        } else {
+        } else if constexpr (MODE == 2) {
             const C gg = g[o];
             gx1[o] = cc * (gg.x * bx + gg.y * by);
             gx2[o] = cc * (gg.x * cx + gg.y * cy);
+        } else {                       // MODE 3: g holds y_obs; r = 2 (y - y_obs) and its MODE 2 gradient
+            const C yo = g[o];
+            const C r = mk((T)2 * (ax - yo.x), (T)2 * (ay - yo.y));
+            out[o] = r;
+            gx1[o] = cc * (r.x * bx + r.y * by);
+            gx2[o] = cc * (r.x * cx + r.y * cy);
         }
     }
 }

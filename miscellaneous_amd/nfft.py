@@ -239,6 +239,23 @@ class Plan:
             self.grid_version += 1
         return gx1, gx2
 
+    def type2_lsq(self, f: torch.Tensor, y_obs: torch.Tensor, r: torch.Tensor | None = None):
+        """Least squares ||A(x) f - y_obs||^2 in one gather: returns r = 2 (A(x) f - y_obs) (the
+        adjoint of r is the gradient w.r.t. f) and the coordinate gradient (gx1, gx2)."""
+        self._need_points()
+        _require(f, self.dtype, self.N1 * self.N2, "f", self.device)
+        _require(y_obs, self.dtype, self.M, "y_obs", self.device)
+        if r is None:
+            r = torch.empty(self.M, dtype=self.dtype, device=self.device)
+        _require(r, self.dtype, self.M, "r", self.device)
+        gx1 = torch.empty(self.M, dtype=self.rdtype, device=self.device)
+        gx2 = torch.empty_like(gx1)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_type2_lsq(self._h, _ptr(r), _ptr(gx1), _ptr(gx2), _ptr(y_obs), _ptr(f),
+                                             _stream(self.device)), "nfft2d_type2_lsq")
+            self.grid_version += 1
+        return r, gx1, gx2
+
     def type2_grad_x_grid(self, g: torch.Tensor):
         """type2_grad_x on the oversampled grid the plan holds: valid while grid_version is the one
         right after the type2 / type2_grid of the same f (no type-1, adjoint or spread since)."""

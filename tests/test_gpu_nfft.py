@@ -364,6 +364,26 @@ def test_grad_x_on_held_grid_matches_recompute():
     assert ((held[0] - recomputed[0]).norm() / recomputed[0].norm()).item() < 1e-6
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+def test_least_squares_gather_matches_the_separate_calls(dtype):
+    """type2_lsq (value, residual and coordinate gradient in one gather) gives
+    the bits of type2 + r = 2 (y - y_obs) + type2_grad_x_grid(r)."""
+    rng = np.random.default_rng(15)
+    N1, N2, M = 128, 64, 20000
+    rt = torch.float32 if dtype == torch.complex64 else torch.float64
+    x1 = T(((rng.random(M) - 0.5) * N1).astype(np.float32)).to(rt)
+    x2 = T(((rng.random(M) - 0.5) * N2).astype(np.float32)).to(rt)
+    f = T(cn01(rng, (N1, N2))).to(dtype)
+    y_obs = T(cn01(rng, M)).to(dtype)
+    plan = m.Plan(N1, N2, dtype, device=DEV).setpts(x1, x2)
+    r, gx1, gx2 = plan.type2_lsq(f, y_obs)
+    y = plan.type2(f)
+    r_ref = torch.add(-2 * y_obs, y, alpha=2)
+    g1, g2 = plan.type2_grad_x_grid(r_ref)
+    assert torch.equal(r, r_ref)
+    assert torch.equal(gx1, g1) and torch.equal(gx2, g2)
+
+
 def test_autograd_gradients_vs_fp64_and_finite_differences():
     """loss = ||A(x) f - y||^2 at 256^2, M = 262,144 (subset checks for dL/dx)."""
     rng = np.random.default_rng(10)
