@@ -860,7 +860,6 @@ struct nfft2d_batch_s {
 static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
 static int batch_cols_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
 static constexpr int kBatchMinMaxpts = 64;   // smallest subproblem size the autotuner tries
-static constexpr int kBatchGatherBlocks = 2048;
 static constexpr int kBatchGatherWaveBlocks = 256 * 4;   // 4 blocks of 4 waves per CU (LDS: 38 KB each)
 
 static void batch_free(nfft2d_batch *b)
@@ -1051,13 +1050,8 @@ static int batch_grid(nfft2d_batch *b, hipStream_t s)
 
 static int batch_gather(nfft2d_batch *b, hipStream_t s)
 {
-#ifdef NFFT2D_BATCH_BLOCK_GATHER
-    hipLaunchKernelGGL(kb_gather, dim3(kBatchGatherBlocks), dim3(kGatherThreads), 0, s, b->ddesc, b->bsub,
-                       b->base + b->count, b->plans[0]->wa);
-#else
     hipLaunchKernelGGL(kb_gather_w, dim3(kBatchGatherWaveBlocks), dim3(256), 0, s, b->ddesc, b->bsub,
                        b->base + b->count, b->plans[0]->wa);
-#endif
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }

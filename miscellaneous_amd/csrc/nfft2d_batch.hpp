@@ -192,8 +192,8 @@ __global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ 
 // Persistent gather over the batch's subproblem list (total = base[count]),
 // one wave per subproblem: wave g = 4 blockIdx.x + w takes subproblems g,
 // g + 4 gridDim.x, ...
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATCH_GATHER_WAVES, 8))) void kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
-                                                   const int* __restrict__ total, WinArgs wa)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATCH_GATHER_WAVES, 8))) void
+kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub, const int* __restrict__ total, WinArgs wa)
 {
     __shared__ WavePairLds lds[4];
     const int w = threadIdx.x >> 6;
@@ -202,18 +202,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATC
         const int4 d = bsub[s];
         const BatchDesc& t = bd[d.w];
         gather_wave_pairs(d, lds[w], t.Gt, t.geo, t.pts, wa, t.out);
-    }
-}
-
-// Persistent gather over the batch's subproblem list (total = base[count]).
-__global__ __launch_bounds__(kGatherThreads) void kb_gather(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
-                                                            const int* __restrict__ total, WinArgs wa)
-{
-    const int n = *total;
-    for (int s = blockIdx.x; s < n; s += gridDim.x) {
-        const BatchDesc& d = bd[bsub[s].w];
-        gather_body<float, 0>(s, d.Gt, d.geo, d.pts, bsub, wa, d.out, nullptr, nullptr, nullptr, nullptr, nullptr);
-        __syncthreads();
     }
 }
 

@@ -825,16 +825,19 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
     return base + inc - v;
 }
 
-// (c) type-2 gather over one subproblem (<= 256 points of one bin), MODE 0:
-// out[l] = y_l;  MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;  MODE 3:
-// least squares, g = y_obs: out = r = 2 (y - y_obs) and gx = Re(conj(r) dy/dx);  MODE 2:
-// gradient Re(conj(g_l) dy_l/dx) into gx1/gx2.
+// (c) type-2 gather over one subproblem (<= 256 points of one bin), one point
+// per thread: the complex128 gathers and the derivative modes of both
+// precisions (the complex64 value gather is k_gather_pairs, below).
+//   MODE 0: out[l] = y_l;
+//   MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;
+//   MODE 2: gradient Re(conj(g_l) dy_l/dx) into gx1 / gx2;
+//   MODE 3: least squares, g = y_obs: out = r = 2 (y - y_obs), gx = Re(conj(r) dy/dx).
 // y_l = sum_{m2} w2[m2] sum_{m1} w1[m1] G[(mu2+m2-K) mod L2][(mu1+m1-K) mod L1]
 // (BP_FBP_FFBP/NFFT1_2D.m:61-67 on the unshifted grid).  The block first
 // counting-sorts its points by cell in LDS so that neighbouring lanes read
 // neighbouring tile addresses.
 template <typename T, int MODE>
-__device__ __forceinline__ void gather_body_single(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
+__device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
                                             const PtRec<T>* __restrict__ pts, const int4* __restrict__ sub,
                                             const WinArgs& wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1,
                                             cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g,
@@ -955,7 +958,7 @@ __device__ __forceinline__ void gather_body_single(int s, const cplx<T>* __restr
 // The block counting-sorts its points by cell pair; a pair with n points takes
 // ceil(n / 2) lanes.  Each point's 13 column weights are placed in a row of 14
 // with a zero at the end its window does not reach, so a point's sum runs in
-// the same order as in gather_body_single (plus one FMA with a zero weight):
+// the same order as in gather_body (plus one FMA with a zero weight):
 // the results are bitwise the same.
 constexpr int kPairs = kBin * kBin / 2;                     // cell pairs of a bin
 struct __attribute__((aligned(16))) PairLds {
@@ -1079,23 +1082,6 @@ k_gather_pairs(const float2* __restrict__ Gt, BinGeom geo, const PtRec<float>* _
 {
     if ((int)blockIdx.x >= *nsub) return;
     gather_pairs_body(blockIdx.x, Gt, geo, pts, sub, wa, out);
-}
-
-template <typename T, int MODE>
-__device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ Gt, const BinGeom& geo,
-                                            const PtRec<T>* __restrict__ pts, const int4* __restrict__ sub,
-                                            const WinArgs& wa, cplx<T>* __restrict__ out, cplx<T>* __restrict__ out_d1,
-                                            cplx<T>* __restrict__ out_d2, const cplx<T>* __restrict__ g,
-                                            T* __restrict__ gx1, T* __restrict__ gx2)
-{
-#ifndef NFFT2D_SINGLE_POINT_GATHER
-    if constexpr (MODE == 0 && std::is_same<T, float>::value) {
-        gather_pairs_body(s, Gt, geo, pts, sub, wa, out);
-    } else
-#endif
-    {
-        gather_body_single<T, MODE>(s, Gt, geo, pts, sub, wa, out, out_d1, out_d2, g, gx1, gx2);
-    }
 }
 
 template <typename T, int MODE>
