@@ -30,6 +30,10 @@
  * Thread safety: the plan cache is internally locked.  Two calls that share
  * one plan on different streams race on the plan's scratch; calls on one
  * stream are ordered.
+ * Devices: a plan (and a batch, an NDFT plan) belongs to the device current
+ * at its creation; calls on it with another device current fail with
+ * NFFT2D_ERR_ARG.  The drop-ins keep one cached plan per device.
+ * Sizes: at most 2^30 points per transform (M), NFFT2D_ERR_ARG above.
  */
 #ifndef NFFT2D_H
 #define NFFT2D_H
@@ -165,7 +169,9 @@ int nfft2d_type1_finish(nfft2d_plan *plan, void *result, int sign, void *stream)
 
 /* Points per subproblem of the spread / gather: 32 to 256 (default 256). */
 int nfft2d_plan_set_maxpts(nfft2d_plan *plan, int maxpts);
-/* Number of points set on the plan, or < 0 on error. */
+/* Number of points set on the plan; NFFT2D_ERR_STATE before the first
+ * nfft2d_setpts (or after nfft2d_plan_set_maxpts), NFFT2D_ERR_ARG for a null
+ * plan. */
 int nfft2d_plan_num_points(const nfft2d_plan *plan);
 
 /* ------------------------------------------------ NDFT as an MFMA contraction */

@@ -137,6 +137,8 @@ extern "C" int nfft2d_ndft_plan_destroy(nfft2d_ndft_plan *p)
 extern "C" int nfft2d_ndft_set_modes(nfft2d_ndft_plan *p, const nfft2d_c64 *data, void *stream)
 {
     if (!p || !data) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_set_modes: null argument");
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const float2 *F = reinterpret_cast<const float2 *>(data);
     hipLaunchKernelGGL(ndft::k_ndft_rowscale, dim3(p->N1), dim3(256), 0, s, F, p->scale, p->inv_scale, p->N2,
@@ -169,6 +171,8 @@ extern "C" int nfft2d_ndft_eval(nfft2d_ndft_plan *p, nfft2d_c64 *result, const f
     if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_eval: negative M");
     if (M == 0) return NFFT2D_OK;
     if (!result || !x || !y) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_eval: null pointer");
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const dim3 grid((unsigned)((M + ndft::kPtsPerGroup - 1) / ndft::kPtsPerGroup));
     float2 *out = reinterpret_cast<float2 *>(result);

@@ -341,7 +341,24 @@ extern "C" int nfft2d_plan_set_maxpts(nfft2d_plan *p, int maxpts)
     return NFFT2D_OK;
 }
 
-extern "C" int nfft2d_plan_num_points(const nfft2d_plan *p) { return p ? p->M : nfft2d_fail(NFFT2D_ERR_ARG, "null plan"); }
+extern "C" int nfft2d_plan_num_points(const nfft2d_plan *p)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
+    if (p->M < 0) return nfft2d_fail(NFFT2D_ERR_STATE, "plan has no points");
+    return p->M;
+}
+
+// A plan's buffers live on the device that was current at its creation; a
+// call made with another device current would launch there on foreign
+// pointers, so it is refused.
+int nfft2d_check_device(int device)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "hipGetDevice failed");
+    if (cur != device)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "plan belongs to device %d but device %d is current", device, cur);
+    return NFFT2D_OK;
+}
 
 static unsigned grid1d(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -445,8 +462,10 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
 extern "C" int nfft2d_setpts(nfft2d_plan *p, const void *x, const void *y, int M, void *stream)
 {
     if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: null plan");
-    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: M < 0");
+    if (M < 0 || M > kMaxPoints) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: M = %d outside [0, %d]", M, kMaxPoints);
     if (M > 0 && (!x || !y)) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts: null coordinates");
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     return p->dtype == NFFT2D_C64 ? setpts_impl<float>(p, (const float *)x, (const float *)y, M, s)
                                   : setpts_impl<double>(p, (const double *)x, (const double *)y, M, s);
@@ -616,13 +635,15 @@ static int check_exec(const nfft2d_plan *p, const void *a, const void *b)
     if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
     if (p->M < 0) return nfft2d_fail(NFFT2D_ERR_STATE, "plan has no points: call nfft2d_setpts first");
     if (p->M > 0 && (!a || !b)) return nfft2d_fail(NFFT2D_ERR_ARG, "null data or result pointer");
-    return NFFT2D_OK;
+    return nfft2d_check_device(p->device);
 }
 
 extern "C" int nfft2d_type2_grid(nfft2d_plan *p, const void *data, void *stream)
 {
     if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
     if (!data) return nfft2d_fail(NFFT2D_ERR_ARG, "null data pointer");
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     return p->dtype == NFFT2D_C64 ? t2_grid<float>(p, data, s) : t2_grid<double>(p, data, s);
 }
@@ -649,6 +670,8 @@ extern "C" int nfft2d_type1_finish(nfft2d_plan *p, void *result, int sign, void 
     if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "null plan");
     if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "null result pointer");
     if (sign != -1 && sign != 1) return nfft2d_fail(NFFT2D_ERR_ARG, "sign must be -1 or +1");
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     if (p->dtype == NFFT2D_C64)
         return sign < 0 ? t1_passes<float, -1>(p, result, s) : t1_passes<float, +1>(p, result, s);
@@ -669,12 +692,15 @@ extern "C" int nfft2d_setpts_type2(nfft2d_plan *p, void *result, const void *dat
                                    void *stream)
 {
     if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: null plan");
-    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: M < 0");
+    if (M < 0 || M > kMaxPoints)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: M = %d outside [0, %d]", M, kMaxPoints);
     if (M > 0 && (!x || !y || !data || !result))
         return nfft2d_fail(NFFT2D_ERR_ARG, "setpts_type2: null data, result or coordinate pointer");
     if (M == 0) return nfft2d_setpts(p, x, y, 0, stream);
+    int rc = nfft2d_check_device(p->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    int rc = p->fk.init(p->device);
+    rc = p->fk.init(p->device);
     if (rc || (rc = p->fk.fork(s))) return rc;
     rc = nfft2d_setpts(p, x, y, M, p->fk.aux);
     const int rg = rc ? NFFT2D_OK : nfft2d_type2_grid(p, data, stream);
@@ -906,7 +932,8 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
         if (!is_pow2(N1[t]) || !is_pow2(N2[t]) || N1[t] < 8 || N2[t] < 8 || N1[t] > kBatchMaxN || N2[t] > kBatchMaxN)
             return nfft2d_fail(NFFT2D_ERR_SIZE, "batch_create: transform %d: N1=%d N2=%d must be powers of two in [8, %d]",
                                t, N1[t], N2[t], kBatchMaxN);
-        if (M[t] < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: transform %d: M < 0", t);
+        if (M[t] < 0 || M[t] > kMaxPoints)
+            return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: transform %d: M = %d outside [0, %d]", t, M[t], kMaxPoints);
     }
     nfft2d_batch *b = new (std::nothrow) nfft2d_batch_s();
     if (!b) return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_create: host allocation failed");
@@ -1023,6 +1050,8 @@ extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
 {
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_setpts: null batch");
     if (!b->have_points) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts: nfft2d_batch_set_points was not called");
+    int rc = nfft2d_check_device(b->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     if (b->n_hist > 0)
         hipLaunchKernelGGL(kb_hist, dim3(b->n_hist), dim3(kHistThreads), sizeof(int) * b->max_nbins, s, b->ddesc,
@@ -1060,8 +1089,10 @@ extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
 {
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type2: null batch");
     if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: nfft2d_batch_set_io was not called");
+    int rc = nfft2d_check_device(b->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    int rc = batch_grid(b, s);
+    rc = batch_grid(b, s);
     return rc ? rc : batch_gather(b, s);
 }
 
@@ -1072,8 +1103,10 @@ extern "C" int nfft2d_batch_setpts_type2(nfft2d_batch *b, void *stream)
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_setpts_type2: null batch");
     if (!b->have_points) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts_type2: nfft2d_batch_set_points was not called");
     if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_setpts_type2: nfft2d_batch_set_io was not called");
+    int rc = nfft2d_check_device(b->device);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    int rc = b->fk.init(b->device);
+    rc = b->fk.init(b->device);
     if (rc || (rc = b->fk.fork(s))) return rc;
     rc = nfft2d_batch_setpts(b, b->fk.aux);
     const int rg = rc ? NFFT2D_OK : batch_grid(b, s);

@@ -3,8 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+// Largest point count of one transform: keeps every point index and every
+// per-block offset of the setpts kernels inside a signed 32-bit int.
+constexpr int kMaxPoints = 1 << 30;
+
 // Records the calling thread's last error (read by nfft2d_last_error) and returns `code`.
 int nfft2d_fail(int code, const char *fmt, ...);
+
+// NFFT2D_OK when `device` is the calling thread's current device, else an
+// NFFT2D_ERR_ARG failure naming both.
+int nfft2d_check_device(int device);
 
 #define NFFT2D_HIP_TRY(expr)                                                                              \
     do {                                                                                                  \

@@ -104,7 +104,7 @@ def _ints(v):
     ([64, 48], [64, 64], [10, 10], -2, "powers of two"),
     ([64, 2048], [64, 64], [10, 10], -2, "1024"),
     ([64, 4], [64, 64], [10, 10], -2, "[8,"),
-    ([64, 64], [64, 64], [10, -1], -1, "M < 0"),
+    ([64, 64], [64, 64], [10, -1], -1, "outside [0, "),
 ])
 def test_batch_create_validation(lib, N1, N2, M, code, needle):
     h = ctypes.c_void_p()

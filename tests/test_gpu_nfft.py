@@ -5,6 +5,8 @@ the reference path; measured ~3e-7), FP64 paths 1e-9.  Coordinates in every
 fixture are fp32 values, so the FP64 expected values see exactly the inputs
 the FP32 kernels see.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -180,6 +182,27 @@ def test_empty_and_single_point():
     k = np.arange(-N // 2, N // 2)
     ref = np.exp(-2j * np.pi * (k[:, None] * np.float32(x) / N + k[None, :] * np.float32(yy) / N))
     assert oracle.rel_l2(F, ref) < TOL32
+
+
+def test_plan_state_and_size_errors():
+    """Host-side refusals of a live plan: no points yet, a point count above
+    2^30, set_maxpts resetting the points.  No kernel is launched by any of the
+    refused calls."""
+    plan = m.Plan(32, 32, device=DEV)
+    lib, h = plan._lib, plan._h
+    assert lib.nfft2d_plan_num_points(h) == -5
+    assert "no points" in lib.nfft2d_last_error().decode()
+    out = torch.empty(4, dtype=torch.complex64, device=DEV)
+    f = torch.zeros(32, 32, dtype=torch.complex64, device=DEV)
+    assert lib.nfft2d_type2(h, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(f.data_ptr()), None) == -5
+    x = torch.zeros(4, device=DEV)
+    assert lib.nfft2d_setpts(h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(x.data_ptr()), (1 << 30) + 1,
+                             None) == -1
+    assert "outside" in lib.nfft2d_last_error().decode()
+    plan.setpts(x, x)
+    assert lib.nfft2d_plan_num_points(h) == 4
+    assert lib.nfft2d_plan_set_maxpts(h, 64) == 0
+    assert lib.nfft2d_plan_num_points(h) == -5
 
 
 def test_clustered_points_and_dense_tiles():
