@@ -1257,91 +1257,157 @@ __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeo
 // 16x16 bin) accumulates its points' footprints into a 32x32 tile whose origin
 // is the bin origin minus the K = 6 halo (rows / columns 0..27 are used):
 //     T[i][j] = sum_p (d_p w2_p[i - r_p]) * w1_p[j - c_p],   r_p, c_p = cell - bin origin.
-// Every point is an outer product of two 32-vectors; v_mfma_f32_32x32x2f32
-// adds two per instruction into a 32x32 accumulator held in registers.  The
-// tiles go to scratch, and k_spread_merge sums, for every grid cell, the tiles
-// of the (up to 9) neighbouring bins that cover it, in a fixed order.  No
-// atomics; needs at least 3 bins and L >= 44 per dimension (the plan falls back
-// to the LDS-atomic k_spread otherwise).
+// Every point is an outer product of two 32-vectors, and
+// v_mfma_f32_32x32x16_f16 adds 16 of them per instruction into a 32x32 FP32
+// accumulator held in registers.  Each operand value x is split x = hi + lo
+// into two FP16 values (round to nearest) and T accumulates
+// lo*hi + hi*lo + hi*hi, which keeps about 22 bits of every product.  A wave
+// scales its data values by a power of two that puts their largest component
+// in [2^14, 2^15) (the window taps are below 1, so every operand stays inside
+// the FP16 range) and scales its tile back before the 4 waves' tiles are
+// summed in a fixed order.  The tiles go to scratch, and k_spread_merge sums,
+// for every grid cell, the tiles of the (up to 9) neighbouring bins that
+// cover it, in a fixed order.  No atomics; needs at least 3 bins and L >= 44
+// per dimension (the plan falls back to the LDS-atomic k_spread otherwise).
+// Operand staging, 16 points at a time: a point's 13 taps sit at rows 15..27
+// of a zeroed column of 44, so tile row i of point p reads column row
+// i - r_p + 15, always inside the column, with no per-element test (rows
+// 28..31 of the 32 x 32 product read row 27's entries; they are not stored).
 constexpr int kHalo = kBin + 2 * kK;          // 28 used rows / columns of a tile
 constexpr int kTileCells = kHalo * kHalo;     // 784
 typedef float f16v __attribute__((ext_vector_type(16)));
 
+constexpr int kSpCol = 44, kSpTap0 = 15, kSpGrp = 16;
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+struct SpreadWaveLds {
+    uint2 a[kSpGrp][kSpCol];       // tap of d s w2: {re_hi, im_hi}, {re_lo, im_lo}
+    unsigned b[kSpGrp][kSpCol];    // tap of w1: {hi, lo}
+    unsigned rc[64];               // (15 - r_p) | (15 - c_p) << 16 of the wave's 64 points
+};
 
-// Per-wave staging of 64 points, one column per point (odd stride 65):
-// rows 0-12 w2 taps, 13-25 w1 taps, 26 zeros.
-constexpr int kStgLd = 65, kStgRows = 27;
-
-__global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restrict__ pts, const float2* __restrict__ data,
-                                                     const int4* __restrict__ sub, const int* __restrict__ nsub,
-                                                     BinGeom geo, WinArgs wa, float2* __restrict__ scratch)
+__device__ __forceinline__ unsigned pack_h2(_Float16 lo16, _Float16 hi16)
 {
-    constexpr int kStageFloats = 4 * kStgRows * kStgLd;                 // 7020
-    constexpr int kRedFloats = 4 * 32 * 32 * 2;                         // 8192
-    __shared__ __attribute__((aligned(16))) float smem[kRedFloats > kStageFloats ? kRedFloats : kStageFloats];
-    __shared__ float4 pinfo[4][64];                                     // (r_p, c_p, d.x, d.y)
+    return (unsigned)__builtin_bit_cast(unsigned short, lo16) | ((unsigned)__builtin_bit_cast(unsigned short, hi16) << 16);
+}
+
+__device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo)
+{
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+__global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restrict__ pts,
+                                                     const float2* __restrict__ data, const int4* __restrict__ sub,
+                                                     const int* __restrict__ nsub, BinGeom geo, WinArgs wa,
+                                                     float2* __restrict__ scratch)
+{
+    __shared__ __attribute__((aligned(16))) union {
+        SpreadWaveLds w[4];
+        float red[4 * 32 * 32 * 2];   // [w][cell(32x32)][re, im], after the products
+    } sm;
     const int s = blockIdx.x;
     if (s >= *nsub) return;
     const int4 d = sub[s];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
-    float* st = smem + w * kStgRows * kStgLd;
-    // stage this wave's points (64 per wave, zeros beyond the subproblem)
+    SpreadWaveLds& L = sm.w[w];
+    {
+        constexpr int n16 = (int)((sizeof(L.a) + sizeof(L.b)) / 16);
+        u4v* z = reinterpret_cast<u4v*>(&L);
+        for (int e = lane; e < n16; e += 64) z[e] = u4v{0u, 0u, 0u, 0u};
+    }
+    // this lane's point (zeros beyond the subproblem)
     const int q = 64 * w + lane;
     float w1[kW], w2[kW];
-    float4 info = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 v = make_float2(0.f, 0.f);
+    int rl = 0, cl = 0;
     if (q < d.z) {
-        const int p = d.y + q;
-        const PtRec<float> rec = pts[p];
-        const int pc = rec.cell;
-        const float2 v = data[rec.idx];
+        const PtRec<float> rec = pts[d.y + q];
+        v = data[rec.idx];
         taps(rec.t1, wa, w1);
         taps(rec.t2, wa, w2);
-        const int rl = min(max((pc >> 16) - O2, 0), kBin - 1), cl = min(max((pc & 0xffff) - O1, 0), kBin - 1);
-        info = make_float4((float)rl, (float)cl, v.x, v.y);
+        rl = min(max((rec.cell >> 16) - O2, 0), kBin - 1);
+        cl = min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
     } else {
 #pragma unroll
         for (int m = 0; m < kW; ++m) w1[m] = w2[m] = 0.f;
     }
+    // the wave's scale 2^(14 - e), e = exponent of its largest data component
+    float mx = fmaxf(fabsf(v.x), fabsf(v.y));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const int ex = min(max((int)((__float_as_uint(mx) >> 23) & 0xff) - 127, -100), 100);
+    const float sc = __uint_as_float((unsigned)(127 + 14 - ex) << 23);
+    const float inv_sc = __uint_as_float((unsigned)(127 - 14 + ex) << 23);
+    const float vx = v.x * sc, vy = v.y * sc;
+    uint2 aw[kW];
+    unsigned bw[kW];
 #pragma unroll
     for (int m = 0; m < kW; ++m) {
-        st[m * kStgLd + lane] = w2[m];
-        st[(kW + m) * kStgLd + lane] = w1[m];
+        _Float16 rh, rlo, ih, ilo, bh, blo;
+        split_h(vx * w2[m], rh, rlo);
+        split_h(vy * w2[m], ih, ilo);
+        split_h(w1[m], bh, blo);
+        aw[m] = make_uint2(pack_h2(rh, ih), pack_h2(rlo, ilo));
+        bw[m] = pack_h2(bh, blo);
     }
-    st[2 * kW * kStgLd + lane] = 0.f;
-    pinfo[w][lane] = info;
+    L.rc[lane] = (unsigned)(kSpTap0 - rl) | ((unsigned)(kSpTap0 - cl) << 16);
     wave_sync_lds();
     f16v acc_re = {}, acc_im = {};
+    const int i = lane & 31, kg = lane >> 5, ic = min(i, kHalo - 1);
     const int nw = min(64, max(0, d.z - 64 * w));     // points of this wave
-    const int i = lane & 31, kk = lane >> 5;
-    for (int s2 = 0; s2 < nw; s2 += 2) {
-        const int pt = s2 + kk;
-        const float4 pi = pinfo[w][pt];
-        const int ra = i - (int)pi.x, ca = i - (int)pi.y;
-        const float a = st[((unsigned)ra < (unsigned)kW ? ra : 2 * kW) * kStgLd + pt];
-        const float b = st[((unsigned)ca < (unsigned)kW ? kW + ca : 2 * kW) * kStgLd + pt];
-        acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a * pi.z, b, acc_re, 0, 0, 0);
-        acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(a * pi.w, b, acc_im, 0, 0, 0);
+    for (int g = 0; g < nw; g += kSpGrp) {
+        if ((lane >> 4) == (g >> 4)) {
+            const int pl = lane & (kSpGrp - 1);
+#pragma unroll
+            for (int m = 0; m < kW; ++m) {
+                L.a[pl][kSpTap0 + m] = aw[m];
+                L.b[pl][kSpTap0 + m] = bw[m];
+            }
+        }
+        wave_sync_lds();
+        u4v ahr, ahi, alr, ali, bh, bl;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int p0 = 8 * kg + 2 * t, p1 = p0 + 1;
+            const unsigned r0 = L.rc[g + p0], r1 = L.rc[g + p1];
+            const uint2 a0 = L.a[p0][ic + (int)(r0 & 0xffffu)], a1 = L.a[p1][ic + (int)(r1 & 0xffffu)];
+            const unsigned b0 = L.b[p0][ic + (int)(r0 >> 16)], b1 = L.b[p1][ic + (int)(r1 >> 16)];
+            ahr[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x05040100u);
+            ahi[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x07060302u);
+            alr[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x05040100u);
+            ali[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x07060302u);
+            bh[t] = __builtin_amdgcn_perm(b1, b0, 0x05040100u);
+            bl[t] = __builtin_amdgcn_perm(b1, b0, 0x07060302u);
+        }
+        const h8v Bh = __builtin_bit_cast(h8v, bh), Bl = __builtin_bit_cast(h8v, bl);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, alr), Bh, acc_re, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bl, acc_re, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bh, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ali), Bh, acc_im, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bl, acc_im, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bh, acc_im, 0, 0, 0);
+        wave_sync_lds();
     }
-    // reduce the 4 waves' tiles in a fixed order and store the 28x28 used part
+    // unscale, reduce the 4 waves' tiles in a fixed order, store the 28x28 used part
     __syncthreads();
-    float* red = smem;                 // [w][cell(32x32)][re, im]
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int ti = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), tj = lane & 31;
-        red[(w * 1024 + ti * 32 + tj) * 2] = acc_re[r];
-        red[(w * 1024 + ti * 32 + tj) * 2 + 1] = acc_im[r];
+        sm.red[(w * 1024 + ti * 32 + tj) * 2] = acc_re[r] * inv_sc;
+        sm.red[(w * 1024 + ti * 32 + tj) * 2 + 1] = acc_im[r] * inv_sc;
     }
     __syncthreads();
     float2* out = scratch + (size_t)s * kTileCells;
     for (int e = threadIdx.x; e < kTileCells; e += 256) {
         const int ti = e / kHalo, tj = e - ti * kHalo;
         const int c = ti * 32 + tj;
-        float sr = red[c * 2], si = red[c * 2 + 1];
+        float sr = sm.red[c * 2], si = sm.red[c * 2 + 1];
 #pragma unroll
         for (int ww = 1; ww < 4; ++ww) {
-            sr += red[(ww * 1024 + c) * 2];
-            si += red[(ww * 1024 + c) * 2 + 1];
+            sr += sm.red[(ww * 1024 + c) * 2];
+            si += sm.red[(ww * 1024 + c) * 2 + 1];
         }
         out[e] = make_float2(sr, si);
     }
