@@ -465,12 +465,13 @@ __global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ x, co
 }
 
 // Pass 2: per bin, turn hist[blk][bin] into the offset of (blk, bin) inside
-// the bin and write the bin total.  A block covers 16 bins x 16 slices of the
+// the bin and write the bin total.  A block covers 32 bins x 8 slices of the
 // block range; each thread's loads are independent and coalesced across bins.
-// The block also writes, for its group of 16 bins, the exclusive prefix of
+// The block also writes, for its group of 32 bins, the exclusive prefix of
 // points and subproblems inside the group (gexcl, as int2) and the group sums
 // (gsum[blockIdx.x], as int2).
-constexpr int kColBins = 16, kColSlices = 16;
+constexpr int kColBins = 32, kColSlices = 8;   // 32 bins: 128-byte rows of hist
+constexpr int kColPer = 256 / kColSlices;         // block-range slice held in registers (nblk <= 256)
 __device__ __forceinline__ void colscan_body(int blk, int* __restrict__ hist, int nblk, int nbins, int maxpts,
                                              int* __restrict__ tot, int2* __restrict__ gexcl, int2* __restrict__ gsum)
 {
@@ -479,15 +480,15 @@ __device__ __forceinline__ void colscan_body(int blk, int* __restrict__ hist, in
     const int bin = blk * kColBins + bl;
     const int per = (nblk + kColSlices - 1) / kColSlices;
     const int lo = min(q * per, nblk), hi = min(lo + per, nblk);
-    int v[16];
+    int v[kColPer];
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < kColPer; ++k) {
         const int kb = lo + k;
         v[k] = (bin < nbins && kb < hi) ? hist[(size_t)kb * nbins + bin] : 0;
         sum += v[k];
     }
-    for (int kb = lo + 16; kb < hi; ++kb) sum += bin < nbins ? hist[(size_t)kb * nbins + bin] : 0;
+    for (int kb = lo + kColPer; kb < hi; ++kb) sum += bin < nbins ? hist[(size_t)kb * nbins + bin] : 0;
     part[q][bl] = sum;
     __syncthreads();
     if (q == 0) {
@@ -498,7 +499,7 @@ __device__ __forceinline__ void colscan_body(int blk, int* __restrict__ hist, in
             run += t;
         }
         if (bin < nbins) tot[bin] = run;
-        // exclusive prefix over the 16 bins of the group (lanes 0..15 of wave 0)
+        // exclusive prefix over the 32 bins of the group (lanes 0..31 of wave 0)
         const int ns = (run + maxpts - 1) / maxpts;
         int ia = run, ib = ns;
 #pragma unroll
@@ -513,14 +514,14 @@ __device__ __forceinline__ void colscan_body(int blk, int* __restrict__ hist, in
     if (bin < nbins) {
         int run = part[q][bl];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
+        for (int k = 0; k < kColPer; ++k) {
             const int kb = lo + k;
             if (kb < hi) {
                 hist[(size_t)kb * nbins + bin] = run;
                 run += v[k];
             }
         }
-        for (int kb = lo + 16; kb < hi; ++kb) {
+        for (int kb = lo + kColPer; kb < hi; ++kb) {
             const int t = hist[(size_t)kb * nbins + bin];
             hist[(size_t)kb * nbins + bin] = run;
             run += t;
@@ -625,8 +626,8 @@ __global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ x, const 
                     pts, subbeg, sub);
 }
 
-// Pass 3 with the group scan folded in (grids of at most kMaxLdsBins bins,
-// i.e. at most kScatThreads groups): every block scans the group sums gsum
+// Pass 3 with the group scan folded in (at most kScatThreads groups of
+// kColBins bins, i.e. 32,768 bins): every block scans the group sums gsum
 // itself into LDS (at most 8 KiB of loads per block), so no launch separates
 // k_colscan from the scatter.  A block scatters kScatThreads * kScatPer points
 // and, if its bin range [blk * kScatThreads, ...) holds bins, writes their

@@ -239,6 +239,32 @@ def test_shapes(N1, N2):
     assert oracle.rel_l2(F[i1, i2], oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2)) < TOL32
 
 
+@pytest.mark.parametrize("N1,N2,M", [(2048, 1024, 300000), (2048, 2048, 200000)])
+def test_large_grids_sort_paths(N1, N2, M):
+    """Oversampled grids above 16,384 bins: the point histogram goes through
+    global atomics (k_hist_global); at 2048 x 1024 (32,768 bins) the group
+    scan stays folded into k_scatter_scan, at 2048 x 2048 (65,536 bins) it runs
+    as the one-block k_scan_groups + k_scatter.  Type-2 on 256 seeded points
+    and type-1 on 256 seeded modes against FP64 direct sums, both with half
+    the points clustered into a few bins (many subproblems per bin)."""
+    rng = np.random.default_rng(N1 + N2)
+    h = M // 2
+    x1 = np.concatenate([(rng.random(h) - 0.5) * N1, rng.normal(0.0, 3.0, M - h)]).astype(np.float32)
+    x2 = np.concatenate([(rng.random(h) - 0.5) * N2, rng.normal(0.0, 3.0, M - h)]).astype(np.float32)
+    f = cn01(rng, (N1, N2))
+    d = cn01(rng, M)
+    print(f"case large grid {N1}x{N2} M={M}", flush=True)
+    y, plan = run_type2(f, x1, x2)
+    sel = rng.choice(M, 256, replace=False)
+    e2 = oracle.rel_l2(y[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel]))
+    F = plan.type1(T(d)).cpu().numpy()
+    i1 = rng.integers(0, N1, 256)
+    i2 = rng.integers(0, N2, 256)
+    e1 = oracle.rel_l2(F[i1, i2], oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2))
+    print(f"large grid rel-L2 type-2 {e2:.3e} type-1 {e1:.3e}")
+    assert e2 < TOL32 and e1 < TOL32
+
+
 def test_nonfinite_coordinate_is_contained():
     rng = np.random.default_rng(5)
     N, M = 32, 500
