@@ -202,12 +202,10 @@ def bench_grad(args, dev):
     def loss_now() -> float:                       # eager, outside the graph
         return float(((plan.type2(f) - y_obs).abs() ** 2).sum().item())
 
-    r = torch.empty(M, dtype=torch.complex64, device=dev)
-
     def step():
-        # forward, residual r = 2 (y - y_obs) and dL/dx in one gather
-        plan.type2_lsq(f, y_obs, r=r)
-        plan.adjoint(r, out=gf)                    # dL/df
+        # forward, residual r = 2 (y - y_obs) (kept in the plan) and dL/dx in one gather
+        plan.type2_lsq(f, y_obs, keep_r=False)
+        plan.lsq_adjoint(out=gf)                   # dL/df = A^H r
         f.add_(gf, alpha=-lr)
 
     g_step = capture(step)
@@ -554,7 +552,7 @@ def main():
     if gr is not None:
         out["grad"] = {"steps_per_s": 1.0 / gr["s"], "ms_per_step": gr["s"] * 1e3,
                        "config": "256x256 modes, M=262144 uniform, loss ||A(x)f - y||^2, grad f and x, GD update; "
-                                 "step = type2_lsq (forward, residual and grad x in one gather) + adjoint + update",
+                                 "step = type2_lsq (forward, residual and grad x in one gather; the residual stays in the plan) + lsq_adjoint (its adjoint, spread in tile order) + update",
                        "hbm_frac": gr["bytes"] / gr["s"] / (HBM_PEAK_GBS * 1e9),
                        "loss_first": gr["loss0"], "loss_after_steps": gr["loss_final"], "gd_steps": gr["steps"],
                        "gd_lr": gr["lr"]}

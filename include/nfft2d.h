@@ -144,12 +144,23 @@ int nfft2d_type2_grad_x(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, 
 int nfft2d_type2_grad_x_grid(nfft2d_plan *plan, void *gx1, void *gx2, const void *g, void *stream);
 /* Least squares L = ||A(x) data - y_obs||^2 in one pass over the points:
  *   r[l] = 2 (y_l - y_obs[l])  (M complex; dL/d(conj y) times 2, the input of
- *          nfft2d_adjoint for dL/d data),
+ *          nfft2d_adjoint for dL/d data; r may be NULL),
  *   gx1[l], gx2[l] = Re(conj(r[l]) dy_l/dx_l), Re(conj(r[l]) dy_l/dy_l)  (M reals),
  * with the value and derivative sums sharing their tile reads.  Leaves the
- * oversampled grid of `data` in the plan (as nfft2d_type2 does). */
+ * oversampled grid of `data` in the plan (as nfft2d_type2 does), and r in the
+ * plan, for nfft2d_lsq_adjoint.  The first call on a plan (and the first after
+ * nfft2d_setpts with more points than before) allocates that residual buffer:
+ * make it outside a graph capture. */
 int nfft2d_type2_lsq(nfft2d_plan *plan, void *r, void *gx1, void *gx2, const void *y_obs, const void *data,
                      void *stream);
+/* dL/d data of the same loss: the adjoint of the residual r that the last
+ * nfft2d_type2_lsq left in the plan (result: N1*N2): nfft2d_adjoint(plan,
+ * result, r), bit for bit for complex64 plans (the complex128 spread sums with
+ * LDS atomics, in an order that varies between runs).  The spread reads the
+ * held residual in the plan's tile order instead of gathering r by point
+ * index.
+ * NFFT2D_ERR_STATE if no nfft2d_type2_lsq ran since the last nfft2d_setpts. */
+int nfft2d_lsq_adjoint(nfft2d_plan *plan, void *result, void *stream);
 
 /* The stages of the transforms, for callers that time them or reuse the
  * oversampled grid (which lives in the plan between the calls):

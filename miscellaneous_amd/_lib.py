@@ -58,6 +58,7 @@ _SIGS = {
     "nfft2d_type2_grad_x": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "nfft2d_type2_grad_x_grid": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "nfft2d_type2_lsq": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nfft2d_lsq_adjoint": (_i, [_vp, _vp, _vp]),
     "nfft2d_type2_grid": (_i, [_vp, _vp, _vp]),
     "nfft2d_type2_interp": (_i, [_vp, _vp, _vp]),
     "nfft2d_type1_spread": (_i, [_vp, _vp, _vp]),

@@ -197,6 +197,9 @@ struct nfft2d_plan_s {
     int2 *gexcl = nullptr, *gsum = nullptr;  // bin-scan prefixes inside / of groups of kColBins bins
     void *scratch = nullptr;              // MFMA spread tiles, 28x28 per subproblem
     int scratchcap = 0;
+    void *resid = nullptr;                // residual of the last nfft2d_type2_lsq, sorted point order
+    int residcap = 0;
+    bool resid_valid = false;             // resid belongs to the current point set
     BinGeom geo;
     WinArgs wa;
     Fork fk;                              // setpts beside the grid passes (nfft2d_setpts_type2)
@@ -245,6 +248,7 @@ static void free_plan(nfft2d_plan *p)
     free_points(p);
     free_hist(p);
     free_scratch(p);
+    hfree(p->resid);
     p->fk.destroy();
     hfree(p->subbeg);
     hfree(p->gexcl); hfree(p->gsum);
@@ -338,6 +342,7 @@ extern "C" int nfft2d_plan_set_maxpts(nfft2d_plan *p, int maxpts)
         return nfft2d_fail(NFFT2D_ERR_ARG, "set_maxpts: maxpts must be in [32, %d]", kGatherThreads);
     p->maxpts = maxpts;
     p->M = -1;   // subproblems must be rebuilt
+    p->resid_valid = false;
     return NFFT2D_OK;
 }
 
@@ -456,6 +461,7 @@ static int setpts_impl(nfft2d_plan *p, const T *x, const T *y, int M, hipStream_
     p->nblk = nblk;
     p->chunk = chunk;
     p->M = M;
+    p->resid_valid = false;
     return NFFT2D_OK;
 }
 
@@ -596,8 +602,10 @@ static bool mfma_spread_ok(const nfft2d_plan *p)
     return p->dtype == NFFT2D_C64 && p->nb1 >= 3 && p->nb2 >= 3 && p->L1 >= 44 && p->L2 >= 44;
 }
 
+// data: M samples in the callers' point order, or (sorted) in the plan's
+// sorted point-record order.
 template <typename T>
-static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
+static int spread_impl(nfft2d_plan *p, const void *data, bool sorted, hipStream_t s)
 {
     using C = cplx<T>;
     if constexpr (std::is_same<T, float>::value) {
@@ -609,7 +617,7 @@ static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
                 p->scratchcap = p->subcap;
             }
             if (p->M > 0) {
-                hipLaunchKernelGGL(k_spread_halo, dim3(sub_blocks(p)), dim3(256), 0, s, (const PtRec<float> *)p->pts,
+                hipLaunchKernelGGL((sorted ? k_spread_halo<true> : k_spread_halo<false>), dim3(sub_blocks(p)), dim3(256), 0, s, (const PtRec<float> *)p->pts,
                                    (const float2 *)data, p->sub, p->nsub, p->geo, p->wa,
                                    (float2 *)p->scratch);
                 NFFT2D_LAUNCH_CHECK();
@@ -623,7 +631,7 @@ static int spread_impl(nfft2d_plan *p, const void *data, hipStream_t s)
     int rc = zero_fill<T>(p->grid, 2LL * p->L1 * p->L2, s);
     if (rc) return rc;
     if (p->M > 0) {
-        hipLaunchKernelGGL((k_spread<T>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->geo,
+        hipLaunchKernelGGL((sorted ? k_spread<T, true> : k_spread<T, false>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->geo,
                            (const PtRec<T> *)p->pts, (const C *)data, p->sub, p->nsub, p->wa);
         NFFT2D_LAUNCH_CHECK();
     }
@@ -662,7 +670,7 @@ extern "C" int nfft2d_type1_spread(nfft2d_plan *p, const void *data, void *strea
     int rc = check_exec(p, data, data);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    return p->dtype == NFFT2D_C64 ? spread_impl<float>(p, data, s) : spread_impl<double>(p, data, s);
+    return p->dtype == NFFT2D_C64 ? spread_impl<float>(p, data, false, s) : spread_impl<double>(p, data, false, s);
 }
 
 extern "C" int nfft2d_type1_finish(nfft2d_plan *p, void *result, int sign, void *stream)
@@ -725,20 +733,54 @@ extern "C" int nfft2d_type2_dx(nfft2d_plan *p, void *y, void *dy1, void *dy2, co
 
 // Least squares in one gather: y = A(x) data, r = 2 (y - y_obs) and the
 // coordinate gradient of |A(x) data - y_obs|^2, Re(conj(r) dy/dx) (the value
-// and derivative sums share their tile reads).
+// and derivative sums share their tile reads).  r also goes, in the plan's
+// sorted point order, into the plan's residual buffer for nfft2d_lsq_adjoint;
+// the caller's r may be null.
 extern "C" int nfft2d_type2_lsq(nfft2d_plan *p, void *r, void *gx1, void *gx2, const void *y_obs, const void *data,
                                 void *stream)
 {
-    int rc = check_exec(p, r, data);
+    int rc = check_exec(p, data, data);
     if (rc || p->M == 0) return rc;
     if (!gx1 || !gx2 || !y_obs) return nfft2d_fail(NFFT2D_ERR_ARG, "type2_lsq: null argument");
     hipStream_t s = (hipStream_t)stream;
-    if (p->dtype == NFFT2D_C64) {
-        if ((rc = t2_grid<float>(p, data, s))) return rc;
-        return gather<float, 3>(p, r, nullptr, nullptr, y_obs, gx1, gx2, s);
+    if (p->M > p->residcap) {
+        hfree(p->resid);
+        p->resid = nullptr;
+        p->residcap = 0;
+        p->resid_valid = false;
+        if (hipMalloc(&p->resid, p->esz * (size_t)p->Mcap) != hipSuccess)
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "type2_lsq: hipMalloc of the residual (%d points) failed", p->Mcap);
+        p->residcap = p->Mcap;
     }
-    if ((rc = t2_grid<double>(p, data, s))) return rc;
-    return gather<double, 3>(p, r, nullptr, nullptr, y_obs, gx1, gx2, s);
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = t2_grid<float>(p, data, s)) || (rc = gather<float, 3>(p, r, p->resid, nullptr, y_obs, gx1, gx2, s)))
+            return rc;
+    } else {
+        if ((rc = t2_grid<double>(p, data, s)) || (rc = gather<double, 3>(p, r, p->resid, nullptr, y_obs, gx1, gx2, s)))
+            return rc;
+    }
+    p->resid_valid = true;
+    return NFFT2D_OK;
+}
+
+// The adjoint of the residual that the last nfft2d_type2_lsq left in the
+// plan: dL/d(data) of the least-squares loss.  The spread reads the residual
+// in the plan's sorted point order (no gather by original index); otherwise
+// it is nfft2d_adjoint(r) (the same bits for complex64).
+extern "C" int nfft2d_lsq_adjoint(nfft2d_plan *p, void *result, void *stream)
+{
+    int rc = check_exec(p, result, result);
+    if (rc) return rc;
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "lsq_adjoint: null result pointer");
+    if (p->M > 0 && !p->resid_valid)
+        return nfft2d_fail(NFFT2D_ERR_STATE, "lsq_adjoint: no residual for the current points (call nfft2d_type2_lsq)");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == NFFT2D_C64) {
+        if ((rc = spread_impl<float>(p, p->resid, true, s))) return rc;
+        return t1_passes<float, +1>(p, result, s);
+    }
+    if ((rc = spread_impl<double>(p, p->resid, true, s))) return rc;
+    return t1_passes<double, +1>(p, result, s);
 }
 
 // The derivative gather of nfft2d_type2_grad_x on the oversampled grid the

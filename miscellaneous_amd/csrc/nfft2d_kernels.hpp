@@ -832,7 +832,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_warp)
 //   MODE 0: out[l] = y_l;
 //   MODE 1: out = y, out_d1 = dy/dx1, out_d2 = dy/dx2;
 //   MODE 2: gradient Re(conj(g_l) dy_l/dx) into gx1 / gx2;
-//   MODE 3: least squares, g = y_obs: out = r = 2 (y - y_obs), gx = Re(conj(r) dy/dx).
+//   MODE 3: least squares, g = y_obs: out = r = 2 (y - y_obs), gx = Re(conj(r) dy/dx);
+//           out_d1 receives r once more in the plan's sorted point order
+//           (out_d1[p] for the point record p), for a spread that reads its
+//           samples in that order; out may be null, out_d1 may not.
 // y_l = sum_{m2} w2[m2] sum_{m1} w1[m1] G[(mu2+m2-K) mod L2][(mu1+m1-K) mod L1]
 // (BP_FBP_FFBP/NFFT1_2D.m:61-67 on the unshifted grid).  The block first
 // counting-sorts its points by cell in LDS so that neighbouring lanes read
@@ -851,6 +854,7 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
     __shared__ int s_warp[kGatherThreads / 64];
     __shared__ T s_f1[kGatherThreads], s_f2[kGatherThreads];
     __shared__ int s_cell[kGatherThreads], s_idx[kGatherThreads];
+    __shared__ int s_pos[MODE == 3 ? kGatherThreads : 1];
     const int4 d = sub[s];
     const int b1 = d.x % geo.nb1, b2 = d.x / geo.nb1;
     const int O1 = b1 * kBin, O2 = b2 * kBin;
@@ -881,6 +885,7 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
         s_f2[dst] = f2;
         s_cell[dst] = cell;
         s_idx[dst] = o;
+        if constexpr (MODE == 3) s_pos[dst] = tid;
     }
     __syncthreads();
     if (!have) return;         // (no barrier follows in this function)
@@ -888,6 +893,7 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
     f2 = s_f2[tid];
     cell = s_cell[tid];
     o = s_idx[tid];
+    const int ps = MODE == 3 ? d.y + s_pos[tid] : 0;   // this point's record position
     const int r0 = cell / kBin, c0 = cell % kBin;
     T w1[kW], w2[kW];
     taps(f1, wa, w1);
@@ -943,7 +949,11 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
         } else {                       // MODE 3: g holds y_obs; r = 2 (y - y_obs) and its MODE 2 gradient
             const C yo = g[o];
             const C r = mk((T)2 * (ax - yo.x), (T)2 * (ay - yo.y));
-            out[o] = r;
+            // both stores unconditional (a branch around them makes the
+            // compiler hold the whole tile window in registers); without a
+            // caller's r the first one repeats the second
+            *(out ? out + o : out_d1 + ps) = r;
+            out_d1[ps] = r;
             gx1[o] = cc * (r.x * bx + r.y * by);
             gx2[o] = cc * (r.x * cx + r.y * cy);
         }
@@ -1200,7 +1210,9 @@ __device__ __forceinline__ void gather_wave_pairs(const int4& d, WavePairLds& L,
 
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates
 // the bin + K-halo in LDS, then adds it into the grid with global atomics.
-template <typename T>
+// SORTED: data[p] is the sample of point record p (the plan's sorted order)
+// instead of data[original index].
+template <typename T, bool SORTED>
 __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeom geo, const PtRec<T>* __restrict__ pts,
                                                 const cplx<T>* __restrict__ data,
                                                 const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa)
@@ -1222,7 +1234,7 @@ __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeo
     for (int p = d.y + threadIdx.x; p < d.y + d.z; p += blockDim.x) {
         const PtRec<T> q = pts[p];
         const int pc = q.cell;
-        const C v = data[q.idx];
+        const C v = data[SORTED ? p : q.idx];
         const T f1 = q.t1, f2 = q.t2;
         const int c0 = min(max((pc & 0xffff) - O1, 0), kBin - 1);
         const int r0 = min(max((pc >> 16) - O2, 0), kBin - 1);
@@ -1274,6 +1286,8 @@ __global__ __launch_bounds__(256) void k_spread(cplx<T>* __restrict__ Gt, BinGeo
 // of a zeroed column of 44, so tile row i of point p reads column row
 // i - r_p + 15, always inside the column, with no per-element test (rows
 // 28..31 of the 32 x 32 product read row 27's entries; they are not stored).
+// SORTED: data[p] is the sample of point record p (the plan's sorted order,
+// one coalesced load issued with the record's) instead of data[original index].
 constexpr int kHalo = kBin + 2 * kK;          // 28 used rows / columns of a tile
 constexpr int kTileCells = kHalo * kHalo;     // 784
 typedef float f16v __attribute__((ext_vector_type(16)));
@@ -1298,6 +1312,7 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo)
     lo = (_Float16)(x - (float)hi);
 }
 
+template <bool SORTED>
 __global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restrict__ pts,
                                                      const float2* __restrict__ data, const int4* __restrict__ sub,
                                                      const int* __restrict__ nsub, BinGeom geo, WinArgs wa,
@@ -1325,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restr
     int rl = 0, cl = 0;
     if (q < d.z) {
         const PtRec<float> rec = pts[d.y + q];
-        v = data[rec.idx];
+        v = data[SORTED ? d.y + q : rec.idx];
         taps(rec.t1, wa, w1);
         taps(rec.t2, wa, w2);
         rl = min(max((rec.cell >> 16) - O2, 0), kBin - 1);

@@ -239,15 +239,19 @@ class Plan:
             self.grid_version += 1
         return gx1, gx2
 
-    def type2_lsq(self, f: torch.Tensor, y_obs: torch.Tensor, r: torch.Tensor | None = None):
+    def type2_lsq(self, f: torch.Tensor, y_obs: torch.Tensor, r: torch.Tensor | None = None, keep_r: bool = True):
         """Least squares ||A(x) f - y_obs||^2 in one gather: returns r = 2 (A(x) f - y_obs) (the
-        adjoint of r is the gradient w.r.t. f) and the coordinate gradient (gx1, gx2)."""
+        adjoint of r is the gradient w.r.t. f) and the coordinate gradient (gx1, gx2).  The plan
+        keeps r for ``lsq_adjoint``; with keep_r=False r is not written out (returned as None)."""
         self._need_points()
         _require(f, self.dtype, self.N1 * self.N2, "f", self.device)
         _require(y_obs, self.dtype, self.M, "y_obs", self.device)
-        if r is None:
-            r = torch.empty(self.M, dtype=self.dtype, device=self.device)
-        _require(r, self.dtype, self.M, "r", self.device)
+        if keep_r:
+            if r is None:
+                r = torch.empty(self.M, dtype=self.dtype, device=self.device)
+            _require(r, self.dtype, self.M, "r", self.device)
+        elif r is not None:
+            raise ValueError("type2_lsq: r given with keep_r=False")
         gx1 = torch.empty(self.M, dtype=self.rdtype, device=self.device)
         gx2 = torch.empty_like(gx1)
         with torch.cuda.device(self.device):
@@ -255,6 +259,18 @@ class Plan:
                                              _stream(self.device)), "nfft2d_type2_lsq")
             self.grid_version += 1
         return r, gx1, gx2
+
+    def lsq_adjoint(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """dL/df of the least-squares loss of the last ``type2_lsq``: the adjoint of the residual the
+        plan holds (``adjoint(r)``, bit for bit for complex64), spread in the plan's tile order."""
+        self._need_points()
+        if out is None:
+            out = torch.empty(self.N1, self.N2, dtype=self.dtype, device=self.device)
+        _require(out, self.dtype, self.N1 * self.N2, "out", self.device)
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_lsq_adjoint(self._h, _ptr(out), _stream(self.device)), "nfft2d_lsq_adjoint")
+            self.grid_version += 1
+        return out
 
     def type2_grad_x_grid(self, g: torch.Tensor):
         """type2_grad_x on the oversampled grid the plan holds: valid while grid_version is the one

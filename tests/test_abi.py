@@ -66,6 +66,7 @@ def test_null_arguments_rejected(lib):
     assert lib.nfft2d_setpts_type2(None, None, None, None, None, 10, None) == -1
     assert lib.nfft2d_type2_grad_x_grid(None, None, None, None, None) == -1
     assert lib.nfft2d_type2_lsq(None, None, None, None, None, None, None) == -1
+    assert lib.nfft2d_lsq_adjoint(None, None, None) == -1
     assert lib.nfft2d_type1(None, None, None, None) == -1
     assert lib.nfft2d_plan_num_points(None) == -1
     assert lib.NFFT2_2D_GPU_c64(None, None, None, None, 64, 64, 10) == -1

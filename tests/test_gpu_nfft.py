@@ -431,6 +431,40 @@ def test_least_squares_gather_matches_the_separate_calls(dtype):
     g1, g2 = plan.type2_grad_x_grid(r_ref)
     assert torch.equal(r, r_ref)
     assert torch.equal(gx1, g1) and torch.equal(gx2, g2)
+    # the residual the plan keeps: its adjoint is adjoint(r), with or without r
+    # written out in point order -- the same bits for complex64 (the MFMA
+    # spread sums in a fixed order); the complex128 spread accumulates with LDS
+    # atomics, whose order varies from run to run
+    gf = plan.adjoint(r)
+    _, hx1, hx2 = plan.type2_lsq(f, y_obs, keep_r=False)
+    assert torch.equal(hx1, gx1) and torch.equal(hx2, gx2)
+    for _ in range(2):                                 # twice: the held residual is reusable
+        ga = plan.lsq_adjoint()
+        if dtype == torch.complex64:
+            assert torch.equal(ga, gf)
+        else:
+            assert ((ga - gf).norm() / gf.norm()).item() < 1e-13
+
+
+def test_lsq_adjoint_state():
+    """lsq_adjoint needs a type2_lsq on the current point set: refused before
+    the first one and again after setpts; an empty point set gives zeros."""
+    rng = np.random.default_rng(16)
+    N, M = 64, 3000
+    x1 = T(((rng.random(M) - 0.5) * N).astype(np.float32))
+    x2 = T(((rng.random(M) - 0.5) * N).astype(np.float32))
+    plan = m.Plan(N, N, device=DEV).setpts(x1, x2)
+    with pytest.raises(m.NFFT2DError, match="no residual"):
+        plan.lsq_adjoint()
+    f = T(cn01(rng, (N, N)))
+    y_obs = T(cn01(rng, M))
+    r, _, _ = plan.type2_lsq(f, y_obs)
+    assert torch.equal(plan.lsq_adjoint(), plan.adjoint(r))
+    plan.setpts(x2, x1)
+    with pytest.raises(m.NFFT2DError, match="no residual"):
+        plan.lsq_adjoint()
+    plan.setpts(torch.empty(0, device=DEV), torch.empty(0, device=DEV))
+    assert torch.count_nonzero(plan.lsq_adjoint()).item() == 0
 
 
 def test_autograd_gradients_vs_fp64_and_finite_differences():
