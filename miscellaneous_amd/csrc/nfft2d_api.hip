@@ -617,9 +617,9 @@ static int spread_impl(nfft2d_plan *p, const void *data, bool sorted, hipStream_
                 p->scratchcap = p->subcap;
             }
             if (p->M > 0) {
-                hipLaunchKernelGGL((sorted ? k_spread_halo<true> : k_spread_halo<false>), dim3(sub_blocks(p)), dim3(256), 0, s, (const PtRec<float> *)p->pts,
-                                   (const float2 *)data, p->sub, p->nsub, p->geo, p->wa,
-                                   (float2 *)p->scratch);
+                hipLaunchKernelGGL((sorted ? k_spread_halo<true> : k_spread_halo<false>), dim3(sub_blocks(p)), dim3(256),
+                                   0, s, (const PtRec<float> *)p->pts, (const float2 *)data, p->sub, p->nsub,
+                                   p->subbeg, p->tot, p->maxpts, p->geo, p->wa, (float2 *)p->scratch);
                 NFFT2D_LAUNCH_CHECK();
             }
             hipLaunchKernelGGL(k_spread_merge, dim3(p->nbins), dim3(256), 0, s, (const float2 *)p->scratch, p->subbeg,

@@ -1312,11 +1312,19 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo)
     lo = (_Float16)(x - (float)hi);
 }
 
+// A block spreads up to kSpUnit consecutive subproblems of one bin (up to
+// kSpUnit * maxpts points, 256 per pass) into one tile, stored under the first
+// of them: blocks s with (s - first subproblem of the bin) % kSpUnit != 0
+// return at once, and k_spread_merge reads every kSpUnit-th tile of a bin.
+// (A bin just over maxpts points then costs one block, not a second, nearly
+// empty one.)
+constexpr int kSpUnit = 2;
 template <bool SORTED>
-__global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restrict__ pts,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_spread_halo(const PtRec<float>* __restrict__ pts,
                                                      const float2* __restrict__ data, const int4* __restrict__ sub,
-                                                     const int* __restrict__ nsub, BinGeom geo, WinArgs wa,
-                                                     float2* __restrict__ scratch)
+                                                     const int* __restrict__ nsub, const int* __restrict__ subbeg,
+                                                     const int* __restrict__ tot, int maxpts, BinGeom geo,
+                                                     WinArgs wa, float2* __restrict__ scratch)
 {
     __shared__ __attribute__((aligned(16))) union {
         SpreadWaveLds w[4];
@@ -1325,6 +1333,9 @@ __global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restr
     const int s = blockIdx.x;
     if (s >= *nsub) return;
     const int4 d = sub[s];
+    const int k = s - subbeg[d.x];                          // index of subproblem s in its bin
+    if (k % kSpUnit != 0) return;
+    const int cnt = min(kSpUnit * maxpts, tot[d.x] - k * maxpts);   // points of the unit from d.y on
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
     SpreadWaveLds& L = sm.w[w];
@@ -1333,78 +1344,94 @@ __global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restr
         u4v* z = reinterpret_cast<u4v*>(&L);
         for (int e = lane; e < n16; e += 64) z[e] = u4v{0u, 0u, 0u, 0u};
     }
-    // this lane's point (zeros beyond the subproblem)
-    const int q = 64 * w + lane;
-    float w1[kW], w2[kW];
-    float2 v = make_float2(0.f, 0.f);
-    int rl = 0, cl = 0;
-    if (q < d.z) {
-        const PtRec<float> rec = pts[d.y + q];
-        v = data[SORTED ? d.y + q : rec.idx];
-        taps(rec.t1, wa, w1);
-        taps(rec.t2, wa, w2);
-        rl = min(max((rec.cell >> 16) - O2, 0), kBin - 1);
-        cl = min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
-    } else {
+    // this lane's point of every pass (zeros beyond the unit), loads in flight together
+    constexpr int kPass = (kSpUnit * kGatherThreads + 255) / 256;
+    PtRec<float> rec[kPass];
+    float2 v[kPass];
 #pragma unroll
-        for (int m = 0; m < kW; ++m) w1[m] = w2[m] = 0.f;
+    for (int ps = 0; ps < kPass; ++ps) {
+        const int q = 256 * ps + 64 * w + lane;
+        rec[ps].t1 = rec[ps].t2 = 0.f;
+        rec[ps].cell = rec[ps].idx = 0;
+        v[ps] = make_float2(0.f, 0.f);
+        if (q < cnt) {
+            rec[ps] = pts[d.y + q];
+            v[ps] = data[SORTED ? d.y + q : rec[ps].idx];
+        }
     }
     // the wave's scale 2^(14 - e), e = exponent of its largest data component
-    float mx = fmaxf(fabsf(v.x), fabsf(v.y));
+    float mx = 0.f;
+#pragma unroll
+    for (int ps = 0; ps < kPass; ++ps) mx = fmaxf(mx, fmaxf(fabsf(v[ps].x), fabsf(v[ps].y)));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     const int ex = min(max((int)((__float_as_uint(mx) >> 23) & 0xff) - 127, -100), 100);
     const float sc = __uint_as_float((unsigned)(127 + 14 - ex) << 23);
     const float inv_sc = __uint_as_float((unsigned)(127 - 14 + ex) << 23);
-    const float vx = v.x * sc, vy = v.y * sc;
-    uint2 aw[kW];
-    unsigned bw[kW];
-#pragma unroll
-    for (int m = 0; m < kW; ++m) {
-        _Float16 rh, rlo, ih, ilo, bh, blo;
-        split_h(vx * w2[m], rh, rlo);
-        split_h(vy * w2[m], ih, ilo);
-        split_h(w1[m], bh, blo);
-        aw[m] = make_uint2(pack_h2(rh, ih), pack_h2(rlo, ilo));
-        bw[m] = pack_h2(bh, blo);
-    }
-    L.rc[lane] = (unsigned)(kSpTap0 - rl) | ((unsigned)(kSpTap0 - cl) << 16);
-    wave_sync_lds();
     f16v acc_re = {}, acc_im = {};
     const int i = lane & 31, kg = lane >> 5, ic = min(i, kHalo - 1);
-    const int nw = min(64, max(0, d.z - 64 * w));     // points of this wave
-    for (int g = 0; g < nw; g += kSpGrp) {
-        if ((lane >> 4) == (g >> 4)) {
-            const int pl = lane & (kSpGrp - 1);
 #pragma unroll
-            for (int m = 0; m < kW; ++m) {
-                L.a[pl][kSpTap0 + m] = aw[m];
-                L.b[pl][kSpTap0 + m] = bw[m];
+    for (int ps = 0; ps < kPass; ++ps) {
+        const int nw = min(64, max(0, cnt - 256 * ps - 64 * w));     // points of this wave in this pass
+        if (nw == 0) continue;
+        float w1[kW], w2[kW];
+        int rl = 0, cl = 0;
+        if (64 * w + lane < cnt - 256 * ps) {
+            taps(rec[ps].t1, wa, w1);
+            taps(rec[ps].t2, wa, w2);
+            rl = min(max((rec[ps].cell >> 16) - O2, 0), kBin - 1);
+            cl = min(max((rec[ps].cell & 0xffff) - O1, 0), kBin - 1);
+        } else {
+#pragma unroll
+            for (int m = 0; m < kW; ++m) w1[m] = w2[m] = 0.f;
+        }
+        const float vx = v[ps].x * sc, vy = v[ps].y * sc;
+        uint2 aw[kW];
+        unsigned bw[kW];
+#pragma unroll
+        for (int m = 0; m < kW; ++m) {
+            _Float16 rh, rlo, ih, ilo, bh, blo;
+            split_h(vx * w2[m], rh, rlo);
+            split_h(vy * w2[m], ih, ilo);
+            split_h(w1[m], bh, blo);
+            aw[m] = make_uint2(pack_h2(rh, ih), pack_h2(rlo, ilo));
+            bw[m] = pack_h2(bh, blo);
+        }
+        L.rc[lane] = (unsigned)(kSpTap0 - rl) | ((unsigned)(kSpTap0 - cl) << 16);
+        wave_sync_lds();
+        for (int g = 0; g < nw; g += kSpGrp) {
+            if ((lane >> 4) == (g >> 4)) {
+                const int pl = lane & (kSpGrp - 1);
+#pragma unroll
+                for (int m = 0; m < kW; ++m) {
+                    L.a[pl][kSpTap0 + m] = aw[m];
+                    L.b[pl][kSpTap0 + m] = bw[m];
+                }
             }
-        }
-        wave_sync_lds();
-        u4v ahr, ahi, alr, ali, bh, bl;
+            wave_sync_lds();
+            u4v ahr, ahi, alr, ali, bh, bl;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int p0 = 8 * kg + 2 * t, p1 = p0 + 1;
-            const unsigned r0 = L.rc[g + p0], r1 = L.rc[g + p1];
-            const uint2 a0 = L.a[p0][ic + (int)(r0 & 0xffffu)], a1 = L.a[p1][ic + (int)(r1 & 0xffffu)];
-            const unsigned b0 = L.b[p0][ic + (int)(r0 >> 16)], b1 = L.b[p1][ic + (int)(r1 >> 16)];
-            ahr[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x05040100u);
-            ahi[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x07060302u);
-            alr[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x05040100u);
-            ali[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x07060302u);
-            bh[t] = __builtin_amdgcn_perm(b1, b0, 0x05040100u);
-            bl[t] = __builtin_amdgcn_perm(b1, b0, 0x07060302u);
+            for (int t = 0; t < 4; ++t) {
+                const int p0 = 8 * kg + 2 * t, p1 = p0 + 1;
+                const unsigned r0 = L.rc[g + p0], r1 = L.rc[g + p1];
+                const uint2 a0 = L.a[p0][ic + (int)(r0 & 0xffffu)], a1 = L.a[p1][ic + (int)(r1 & 0xffffu)];
+                const unsigned b0 = L.b[p0][ic + (int)(r0 >> 16)], b1 = L.b[p1][ic + (int)(r1 >> 16)];
+                ahr[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x05040100u);
+                ahi[t] = __builtin_amdgcn_perm(a1.x, a0.x, 0x07060302u);
+                alr[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x05040100u);
+                ali[t] = __builtin_amdgcn_perm(a1.y, a0.y, 0x07060302u);
+                bh[t] = __builtin_amdgcn_perm(b1, b0, 0x05040100u);
+                bl[t] = __builtin_amdgcn_perm(b1, b0, 0x07060302u);
+            }
+            const h8v Bh = __builtin_bit_cast(h8v, bh), Bl = __builtin_bit_cast(h8v, bl);
+            acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, alr), Bh, acc_re, 0, 0, 0);
+            acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bl, acc_re, 0, 0, 0);
+            acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bh, acc_re, 0, 0, 0);
+            acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ali), Bh, acc_im, 0, 0, 0);
+            acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bl, acc_im, 0, 0, 0);
+            acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bh, acc_im, 0, 0, 0);
+            wave_sync_lds();
         }
-        const h8v Bh = __builtin_bit_cast(h8v, bh), Bl = __builtin_bit_cast(h8v, bl);
-        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, alr), Bh, acc_re, 0, 0, 0);
-        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bl, acc_re, 0, 0, 0);
-        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahr), Bh, acc_re, 0, 0, 0);
-        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ali), Bh, acc_im, 0, 0, 0);
-        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bl, acc_im, 0, 0, 0);
-        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, ahi), Bh, acc_im, 0, 0, 0);
-        wave_sync_lds();
     }
     // unscale, reduce the 4 waves' tiles in a fixed order, store the 28x28 used part
     __syncthreads();
@@ -1430,8 +1457,9 @@ __global__ __launch_bounds__(256) void k_spread_halo(const PtRec<float>* __restr
 }
 
 // Grid cell (n2, n1) = sum over the neighbouring bins b' (3x3, fixed order) and
-// their subproblems (in order) of tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1]
-// where that index lies in [0, 28).  One block per bin writes the bin's cells.
+// their tiles (every kSpUnit-th subproblem, in order) of
+// tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1] where that index lies
+// in [0, 28).  One block per bin writes the bin's cells.
 __global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__ scratch,
                                                       const int* __restrict__ subbeg, BinGeom geo, int nb2,
                                                       float2* __restrict__ Gt)
@@ -1461,17 +1489,17 @@ __global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         int s = beg[j];
-        for (; s + 8 <= end[j]; s += 8) {
+        for (; s + 7 * kSpUnit < end[j]; s += 8 * kSpUnit) {
             float2 v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = scratch[(size_t)(s + q) * kTileCells + off[j]];
+            for (int q = 0; q < 8; ++q) v[q] = scratch[(size_t)(s + q * kSpUnit) * kTileCells + off[j]];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 sr += v[q].x;
                 si += v[q].y;
             }
         }
-        for (; s < end[j]; ++s) {
+        for (; s < end[j]; s += kSpUnit) {
             const float2 v = scratch[(size_t)s * kTileCells + off[j]];
             sr += v.x;
             si += v.y;
