@@ -87,12 +87,12 @@ static double knab_fprime(double P, double a)
 static double host_phihat(double d) { return knab_f(kK * kK - d * d, host_alpha()) / kPi; }
 static double host_phihat_d(double d) { return -2.0 * d * knab_fprime(kK * kK - d * d, host_alpha()) / kPi; }
 
-// Interpolating polynomial of degree kPolyDeg at the Chebyshev nodes of
+// Interpolating polynomial of degree Deg at the Chebyshev nodes of
 // [-1/2, 1/2], monomial coefficients, solved in long double.
-template <typename F>
+template <int Deg, typename F>
 static void fit_poly(F f, double *coef)
 {
-    constexpr int n = kPolyDeg + 1;
+    constexpr int n = Deg + 1;
     long double A[n][n + 1];
     for (int i = 0; i < n; ++i) {
         long double t = 0.5L * std::cos(kPi * (i + 0.5) / n);
@@ -120,16 +120,29 @@ static void make_window_args(WinArgs &wa)
     std::memset(&wa, 0, sizeof(wa));
     double cf[kPolyDeg + 1];
     for (int j = 0; j <= kK; ++j) {
-        fit_poly([&](double t) { return host_phihat(t - j) / S; }, cf);
+        fit_poly<kPolyDeg>([&](double t) { return host_phihat(t - j) / S; }, cf);
         for (int q = 0; q < kNE; ++q) wa.val.e[j][q] = (float)cf[2 * q];
         for (int q = 0; q < kNO; ++q) wa.val.o[j][q] = j == 0 ? 0.f : (float)cf[2 * q + 1];
-        fit_poly([&](double t) { return host_phihat_d(t - j) / S; }, cf);
+        fit_poly<kPolyDeg>([&](double t) { return host_phihat_d(t - j) / S; }, cf);
         for (int q = 0; q < kNE; ++q) wa.der.e[j][q] = j == 0 ? 0.f : (float)cf[2 * q];
         for (int q = 0; q < kNO; ++q) wa.der.o[j][q] = (float)cf[2 * q + 1];
     }
-    wa.ex.alpha = host_alpha();
-    wa.ex.inv_pi_S = 1.0 / (kPi * S);
-    wa.ex.atK = host_alpha() / (kPi * S);
+}
+
+// The FP64 tables: the same fits at degree kPolyDeg64.
+static void make_window_tables64(WinTables64 &wt)
+{
+    const double S = host_i0(kK * host_alpha());
+    std::memset(&wt, 0, sizeof(wt));
+    double cf[kPolyDeg64 + 1];
+    for (int j = 0; j <= kK; ++j) {
+        fit_poly<kPolyDeg64>([&](double t) { return host_phihat(t - j) / S; }, cf);
+        for (int q = 0; q < kNE64; ++q) wt.val.e[j][q] = cf[2 * q];
+        for (int q = 0; q < kNO64; ++q) wt.val.o[j][q] = j == 0 ? 0.0 : cf[2 * q + 1];
+        fit_poly<kPolyDeg64>([&](double t) { return host_phihat_d(t - j) / S; }, cf);
+        for (int q = 0; q < kNE64; ++q) wt.der.e[j][q] = j == 0 ? 0.0 : cf[2 * q];
+        for (int q = 0; q < kNO64; ++q) wt.der.o[j][q] = cf[2 * q + 1];
+    }
 }
 
 // ============================================================== plan
@@ -195,6 +208,7 @@ struct nfft2d_plan_s {
     int4 *sub = nullptr;
     int *subbeg = nullptr;                // first subproblem of each bin (nbins + 1)
     int2 *gexcl = nullptr, *gsum = nullptr;  // bin-scan prefixes inside / of groups of kColBins bins
+    void *w64 = nullptr;                  // FP64 plans: WinTables64 in device memory
     void *scratch = nullptr;              // MFMA spread tiles, 28x28 per subproblem
     int scratchcap = 0;
     void *resid = nullptr;                // residual of the last nfft2d_type2_lsq, sorted point order
@@ -243,7 +257,7 @@ static void free_plan(nfft2d_plan *p)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(p->device);
     hfree(p->tw1); hfree(p->tw2); hfree(p->D1); hfree(p->D2);
-    hfree(p->grid); hfree(p->tmp);
+    hfree(p->grid); hfree(p->tmp); hfree(p->w64);
     hfree(p->tot); hfree(p->nsub);
     free_points(p);
     free_hist(p);
@@ -315,6 +329,19 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
     make_window_args(p->wa);
     int rc = dtype == NFFT2D_C64 ? upload_tables<float>(p) : upload_tables<double>(p);
     if (rc) { free_plan(p); return rc; }
+    if (dtype == NFFT2D_C128) {
+        WinTables64 wt;
+        make_window_tables64(wt);
+        if (hipMalloc(&p->w64, sizeof(wt)) != hipSuccess) {
+            free_plan(p);
+            return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the FP64 window tables failed");
+        }
+        if (hipMemcpy(p->w64, &wt, sizeof(wt), hipMemcpyHostToDevice) != hipSuccess) {
+            free_plan(p);
+            return nfft2d_fail(NFFT2D_ERR_HIP, "plan_create: upload of the FP64 window tables failed");
+        }
+        p->wa.w64 = (const WinTables64 *)p->w64;
+    }
     if (hipMalloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
         hipMalloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||

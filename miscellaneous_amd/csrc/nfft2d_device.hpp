@@ -1,7 +1,7 @@
 // nfft2d_device.hpp -- device building blocks for the gfx950 2D NFFT kernels.
 //
-// Complex arithmetic on float2/double2, the Knab window (polynomial form for
-// FP32, closed form for FP64), and the register/LDS radix-2/4/8 Stockham
+// Complex arithmetic on float2/double2, the Knab window (per-tap polynomials:
+// degree 8 for FP32, 14 for FP64), and the register/LDS radix-2/4/8 Stockham
 // butterflies used by the FFT passes.
 #pragma once
 
@@ -73,45 +73,62 @@ __device__ __forceinline__ void window_taps(float t, const WinPoly& wp, float (&
     }
 }
 
-// FP64 window: closed form (NFFT/NFFT2_2D.cu:139-149), scaled by 1/S.
-struct WinExact {
-    double alpha;
-    double inv_pi_S;   // 1 / (pi * S)
-    double atK;        // alpha / (pi * S), the P == 0 branch
+// FP64 window: the same split, degree 14 (max error ~4e-15 of the peak, the
+// rounding level of the FP64 closed form NFFT/NFFT2_2D.cu:139-149 it
+// replaces), with the value and derivative tables in device memory (one
+// 1680-byte copy per FP64 plan, read through uniform loads), so the FP32
+// kernels' arguments do not grow.
+constexpr int kPolyDeg64 = 14;
+constexpr int kNE64 = kPolyDeg64 / 2 + 1;
+constexpr int kNO64 = kPolyDeg64 / 2;
+struct WinPoly64 {
+    double e[kK + 1][kNE64];
+    double o[kK + 1][kNO64];
+};
+struct WinTables64 {
+    WinPoly64 val;    // phi_hat / S
+    WinPoly64 der;    // phi_hat' / S
 };
 
-__device__ __forceinline__ double window_exact(double d, const WinExact& we)
+__device__ __forceinline__ void window_taps64(double t, const WinPoly64& wp, double (&w)[kW])
 {
-    double P = (double)(kK * kK) - d * d;
-    if (P > 0.0) { double s = sqrt(P); return sinh(we.alpha * s) / s * we.inv_pi_S; }
-    if (P < 0.0) { double s = sqrt(-P); return sin(we.alpha * s) / s * we.inv_pi_S; }
-    return we.atK;
+    const double u = t * t;
+#pragma unroll
+    for (int j = 0; j <= kK; ++j) {
+        double e = wp.e[j][kNE64 - 1];
+#pragma unroll
+        for (int q = kNE64 - 2; q >= 0; --q) e = fma(e, u, wp.e[j][q]);
+        if (j == 0) {
+            w[kK] = e;
+        } else {
+            double o = wp.o[j][kNO64 - 1];
+#pragma unroll
+            for (int q = kNO64 - 2; q >= 0; --q) o = fma(o, u, wp.o[j][q]);
+            w[kK + j] = fma(t, o, e);
+            w[kK - j] = fma(-t, o, e);
+        }
+    }
 }
 
-// Derivative of the FP64 window: -d (a s cosh(a s) - sinh(a s)) / (pi s^3) for
-// P > 0, d (a s cos(a s) - sin(a s)) / (pi s^3) for P < 0, and the series
-// -2 d f'(P) / pi near P = 0 (|P| < 1).
-__device__ __forceinline__ double window_exact_deriv(double d, const WinExact& we)
+// Derivative taps d/dt phi_hat(t - (m - K)) / S: tap -j is -E'_j(t^2) + t O'_j(t^2).
+__device__ __forceinline__ void window_dtaps64(double t, const WinPoly64& wp, double (&w)[kW])
 {
-    const double a = we.alpha;
-    double P = (double)(kK * kK) - d * d;
-    double fp;
-    if (fabs(P) < 1.0) {
-        double a2 = a * a, c = a * a2 / 6.0, Pp = 1.0, sum = c;
-        for (int j = 2; j < 40; ++j) {
-            c *= a2 / ((2.0 * j) * (2.0 * j + 1.0));
-            Pp *= P;
-            sum += (double)j * c * Pp;
+    const double u = t * t;
+#pragma unroll
+    for (int j = 0; j <= kK; ++j) {
+        double o = wp.o[j][kNO64 - 1];
+#pragma unroll
+        for (int q = kNO64 - 2; q >= 0; --q) o = fma(o, u, wp.o[j][q]);
+        if (j == 0) {
+            w[kK] = t * o;
+        } else {
+            double e = wp.e[j][kNE64 - 1];
+#pragma unroll
+            for (int q = kNE64 - 2; q >= 0; --q) e = fma(e, u, wp.e[j][q]);
+            w[kK + j] = fma(t, o, e);
+            w[kK - j] = fma(t, o, -e);
         }
-        fp = sum;
-    } else if (P > 0.0) {
-        double s = sqrt(P);
-        fp = (a * s * cosh(a * s) - sinh(a * s)) / (2.0 * s * s * s);
-    } else {
-        double s = sqrt(-P);
-        fp = -(a * s * cos(a * s) - sin(a * s)) / (2.0 * s * s * s);
     }
-    return -2.0 * d * fp * (we.inv_pi_S);
 }
 
 // ---------------------------------------------------------------- point -> (mu, t)

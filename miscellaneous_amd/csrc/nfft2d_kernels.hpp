@@ -706,14 +706,13 @@ __global__ __launch_bounds__(kScatThreads) void k_scatter_scan(
 struct WinArgs {
     WinPoly val;    // phi_hat / S
     WinPoly der;    // phi_hat' / S
-    WinExact ex;    // FP64 closed form
+    const WinTables64* __restrict__ w64;   // FP64 plans: degree-14 tables in device memory
 };
 
 __device__ __forceinline__ void taps(float t, const WinArgs& wa, float (&w)[kW]) { window_taps(t, wa.val, w); }
 __device__ __forceinline__ void taps(double t, const WinArgs& wa, double (&w)[kW])
 {
-#pragma unroll
-    for (int m = 0; m < kW; ++m) w[m] = window_exact(t - (double)(m - kK), wa.ex);
+    window_taps64(t, wa.w64->val, w);
 }
 
 // Derivative taps d/dt phi_hat(t - (m - K)) / S.  phi_hat' is odd, so tap -j
@@ -739,8 +738,7 @@ __device__ __forceinline__ void dtaps(float t, const WinArgs& wa, float (&w)[kW]
 }
 __device__ __forceinline__ void dtaps(double t, const WinArgs& wa, double (&w)[kW])
 {
-#pragma unroll
-    for (int m = 0; m < kW; ++m) w[m] = window_exact_deriv(t - (double)(m - kK), wa.ex);
+    window_dtaps64(t, wa.w64->der, w);
 }
 
 // Geometry shared by the gather and spread kernels.
