@@ -430,7 +430,8 @@ def bench_type1(args, dev):
 
 
 def bench_fp64(args, dev):
-    """The complex128 type-2 path on the polar set (512 x 512, M = 1048576): setpts + type-2, graph-replayed."""
+    """The complex128 paths on the polar set (512 x 512, M = 1048576), graph-replayed: setpts + type-2,
+    and setpts + type-1 (the reference's double NFFT2_2D_GPU)."""
     import miscellaneous_amd as m
 
     N = 512
@@ -452,8 +453,24 @@ def bench_fp64(args, dev):
     sel = torch.from_numpy(np.random.default_rng(SEED + 8).choice(M, 4096, replace=False)).to(dev)
     ref = direct_type2_subset(f, x1[sel], x2[sel])
     rel = ((y[sel] - ref).norm() / ref.norm()).item()
+    d = cn01(gen, (M,), dev).to(torch.complex128)
+    F = torch.empty(N, N, dtype=torch.complex128, device=dev)
+
+    def step1():
+        plan.setpts(x1, x2)
+        plan.type1(d, out=F)
+
+    g1 = capture(step1)
+    s1 = time_graph(g1, max(args.steps // 4, 10), 3)
+    rng = np.random.default_rng(SEED + 9)
+    selk = torch.from_numpy(rng.choice(N * N, 256, replace=False)).to(dev)
+    ref1 = direct_type1_subset(d, x1, x2, selk // N - N // 2, selk % N - N // 2, N, N)
+    rel1 = ((F.reshape(-1)[selk] - ref1).norm() / ref1.norm()).item()
     return {"config": "complex128 type-2 NFFT, 512x512 modes, M=1048576 polar points, K=6, c=2; step = setpts + type-2",
-            "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3, "rel_l2": rel}
+            "value": M / s / 1e6, "unit": "Mpts/s", "ms_per_step": s * 1e3, "rel_l2": rel,
+            "type1": {"config": "complex128 type-1 (the double NFFT2_2D_GPU), same points; step = setpts + type-1",
+                      "value": M / s1 / 1e6, "unit": "Mpts/s", "ms_per_step": s1 * 1e3,
+                      "rel_l2_256_modes": rel1}}
 
 
 def cpu_baseline(t2, seconds: float):

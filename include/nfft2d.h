@@ -155,10 +155,10 @@ int nfft2d_type2_lsq(nfft2d_plan *plan, void *r, void *gx1, void *gx2, const voi
                      void *stream);
 /* dL/d data of the same loss: the adjoint of the residual r that the last
  * nfft2d_type2_lsq left in the plan (result: N1*N2): nfft2d_adjoint(plan,
- * result, r), bit for bit for complex64 plans (the complex128 spread sums with
- * LDS atomics, in an order that varies between runs).  The spread reads the
- * held residual in the plan's tile order instead of gathering r by point
- * index.
+ * result, r), bit for bit when the oversampled grid is at least 44 x 44 (the
+ * spread of smaller grids sums with LDS atomics, in an order that varies
+ * between runs).  The spread reads the held residual in the plan's tile order
+ * instead of gathering r by point index.
  * NFFT2D_ERR_STATE if no nfft2d_type2_lsq ran since the last nfft2d_setpts. */
 int nfft2d_lsq_adjoint(nfft2d_plan *plan, void *result, void *stream);
 

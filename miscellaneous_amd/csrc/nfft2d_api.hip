@@ -626,7 +626,7 @@ static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, 
 
 static bool mfma_spread_ok(const nfft2d_plan *p)
 {
-    return p->dtype == NFFT2D_C64 && p->nb1 >= 3 && p->nb2 >= 3 && p->L1 >= 44 && p->L2 >= 44;
+    return p->nb1 >= 3 && p->nb2 >= 3 && p->L1 >= 44 && p->L2 >= 44;
 }
 
 // data: M samples in the callers' point order, or (sorted) in the plan's
@@ -635,31 +635,34 @@ template <typename T>
 static int spread_impl(nfft2d_plan *p, const void *data, bool sorted, hipStream_t s)
 {
     using C = cplx<T>;
-    if constexpr (std::is_same<T, float>::value) {
-        if (mfma_spread_ok(p)) {
-            if (p->subcap > p->scratchcap) {
-                free_scratch(p);
-                if (hipMalloc(&p->scratch, sizeof(float2) * kTileCells * (size_t)p->subcap) != hipSuccess)
-                    return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of %d tiles failed", p->subcap);
-                p->scratchcap = p->subcap;
-            }
-            if (p->M > 0) {
-                hipLaunchKernelGGL((sorted ? k_spread_halo<true> : k_spread_halo<false>), dim3(sub_blocks(p)), dim3(256),
-                                   0, s, (const PtRec<float> *)p->pts, (const float2 *)data, p->sub, p->nsub,
-                                   p->subbeg, p->tot, p->maxpts, p->geo, p->wa, (float2 *)p->scratch);
-                NFFT2D_LAUNCH_CHECK();
-            }
-            hipLaunchKernelGGL(k_spread_merge, dim3(p->nbins), dim3(256), 0, s, (const float2 *)p->scratch, p->subbeg,
-                               p->geo, p->nb2, (float2 *)p->grid);
-            NFFT2D_LAUNCH_CHECK();
-            return NFFT2D_OK;
+    if (mfma_spread_ok(p)) {
+        if (p->subcap > p->scratchcap) {
+            free_scratch(p);
+            if (hipMalloc(&p->scratch, sizeof(C) * kTileCells * (size_t)p->subcap) != hipSuccess)
+                return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of %d tiles failed", p->subcap);
+            p->scratchcap = p->subcap;
         }
+        if (p->M > 0) {
+            if constexpr (std::is_same<T, float>::value)
+                hipLaunchKernelGGL((sorted ? k_spread_halo<true> : k_spread_halo<false>), dim3(sub_blocks(p)),
+                                   dim3(256), 0, s, (const PtRec<float> *)p->pts, (const float2 *)data, p->sub,
+                                   p->nsub, p->subbeg, p->tot, p->maxpts, p->geo, p->wa, (float2 *)p->scratch);
+            else
+                hipLaunchKernelGGL((sorted ? k_spread_halo64<true> : k_spread_halo64<false>), dim3(sub_blocks(p)),
+                                   dim3(256), 0, s, (const PtRec<double> *)p->pts, (const double2 *)data, p->sub,
+                                   p->nsub, p->subbeg, p->tot, p->maxpts, p->geo, p->wa, (double2 *)p->scratch);
+            NFFT2D_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_spread_merge<T>, dim3(p->nbins), dim3(256), 0, s, (const C *)p->scratch, p->subbeg,
+                           p->geo, p->nb2, (C *)p->grid);
+        NFFT2D_LAUNCH_CHECK();
+        return NFFT2D_OK;
     }
     int rc = zero_fill<T>(p->grid, 2LL * p->L1 * p->L2, s);
     if (rc) return rc;
     if (p->M > 0) {
-        hipLaunchKernelGGL((sorted ? k_spread<T, true> : k_spread<T, false>), dim3(sub_blocks(p)), dim3(256), 0, s, (C *)p->grid, p->geo,
-                           (const PtRec<T> *)p->pts, (const C *)data, p->sub, p->nsub, p->wa);
+        hipLaunchKernelGGL((sorted ? k_spread<T, true> : k_spread<T, false>), dim3(sub_blocks(p)), dim3(256), 0, s,
+                           (C *)p->grid, p->geo, (const PtRec<T> *)p->pts, (const C *)data, p->sub, p->nsub, p->wa);
         NFFT2D_LAUNCH_CHECK();
     }
     return NFFT2D_OK;
@@ -793,7 +796,8 @@ extern "C" int nfft2d_type2_lsq(nfft2d_plan *p, void *r, void *gx1, void *gx2, c
 // The adjoint of the residual that the last nfft2d_type2_lsq left in the
 // plan: dL/d(data) of the least-squares loss.  The spread reads the residual
 // in the plan's sorted point order (no gather by original index); otherwise
-// it is nfft2d_adjoint(r) (the same bits for complex64).
+// it is nfft2d_adjoint(r) (the same bits, except on grids under 44 x 44,
+// whose spread sums with LDS atomics).
 extern "C" int nfft2d_lsq_adjoint(nfft2d_plan *p, void *result, void *stream)
 {
     int rc = check_exec(p, result, result);

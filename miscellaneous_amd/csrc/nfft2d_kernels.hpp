@@ -1454,14 +1454,136 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
 }
 
+// The complex128 form of k_spread_halo: the same tile
+//     T[i][j] = sum_p (d_p w2_p[i - r_p]) * w1_p[j - c_p],
+// the same units of up to kSpUnit subproblems of one bin and the same tile
+// store, from v_mfma_f64_16x16x4f64 products (4 points per instruction, the
+// 32 x 32 tile as 2 x 2 blocks of 16 x 16, real and imaginary parts apart;
+// FP64 operands need no split and no scaling).  A wave stages its 64 points'
+// 13 taps of d w2 and of w1 in LDS (tap-major, point fastest); in a 4-point
+// step lane l takes point 4 g + l / 16 and row / column 16 h + l % 16 of block
+// h, reading tap (row - r_p) of that point, or zero outside its window.
+// Accumulator element r of lane l holds row 4 r + l / 16, column l % 16 of
+// its block (measured with a known product).  The 4 waves' tiles are summed
+// in a fixed order, as in k_spread_halo.
+typedef double d4v __attribute__((ext_vector_type(4)));
+struct SpreadWaveLds64 {
+    double2 a[kW][64];    // d_p w2_p[m]
+    double b[kW][64];     // w1_p[m]
+    int r[64], c[64];     // r_p, c_p: the point's cell in the bin
+};
+
+template <bool SORTED>
+__global__ __launch_bounds__(256) void k_spread_halo64(const PtRec<double>* __restrict__ pts,
+                                                       const double2* __restrict__ data, const int4* __restrict__ sub,
+                                                       const int* __restrict__ nsub, const int* __restrict__ subbeg,
+                                                       const int* __restrict__ tot, int maxpts, BinGeom geo,
+                                                       WinArgs wa, double2* __restrict__ scratch)
+{
+    __shared__ __attribute__((aligned(16))) union {
+        SpreadWaveLds64 w[4];
+        double red[4 * 32 * 32 * 2];   // [w][cell(32x32)][re, im], after the products
+    } sm;
+    const int s = blockIdx.x;
+    if (s >= *nsub) return;
+    const int4 d = sub[s];
+    const int k = s - subbeg[d.x];                          // index of subproblem s in its bin
+    if (k % kSpUnit != 0) return;
+    const int cnt = min(kSpUnit * maxpts, tot[d.x] - k * maxpts);   // points of the unit from d.y on
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
+    SpreadWaveLds64& L = sm.w[w];
+    const int col = lane & 15, q4 = lane >> 4;
+    d4v acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc_re[bi][bj] = acc_im[bi][bj] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int base = 64 * w; base < cnt; base += 256) {
+        const int nw = min(64, cnt - base);                 // points of this wave in this pass
+        double w1[kW], w2[kW];
+        double2 v = make_double2(0.0, 0.0);
+        int rl = 0, cl = 0;
+        if (lane < nw) {
+            const PtRec<double> rec = pts[d.y + base + lane];
+            v = data[SORTED ? d.y + base + lane : rec.idx];
+            taps(rec.t1, wa, w1);
+            taps(rec.t2, wa, w2);
+            rl = min(max((rec.cell >> 16) - O2, 0), kBin - 1);
+            cl = min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
+        } else {
+#pragma unroll
+            for (int m = 0; m < kW; ++m) w1[m] = w2[m] = 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < kW; ++m) {
+            L.a[m][lane] = make_double2(v.x * w2[m], v.y * w2[m]);
+            L.b[m][lane] = w1[m];
+        }
+        L.r[lane] = rl;
+        L.c[lane] = cl;
+        wave_sync_lds();
+        for (int g = 0; g < nw; g += 4) {
+            const int p = g + q4;
+            const int rp = L.r[p], cp = L.c[p];
+            double ar[2], ai[2], bb[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int mi = 16 * h + col - rp, mj = 16 * h + col - cp;
+                const bool ina = (unsigned)mi < (unsigned)kW, inb = (unsigned)mj < (unsigned)kW;
+                const double2 av = L.a[ina ? mi : 0][p];
+                const double bv = L.b[inb ? mj : 0][p];
+                ar[h] = ina ? av.x : 0.0;
+                ai[h] = ina ? av.y : 0.0;
+                bb[h] = inb ? bv : 0.0;
+            }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    acc_re[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[bi], bb[bj], acc_re[bi][bj], 0, 0, 0);
+                    acc_im[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[bi], bb[bj], acc_im[bi][bj], 0, 0, 0);
+                }
+        }
+        wave_sync_lds();                                    // staging reused by the next pass
+    }
+    // reduce the 4 waves' tiles in a fixed order and store the 28x28 used part
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ti = 16 * bi + 4 * r + q4, tj = 16 * bj + col;
+                sm.red[(w * 1024 + ti * 32 + tj) * 2] = acc_re[bi][bj][r];
+                sm.red[(w * 1024 + ti * 32 + tj) * 2 + 1] = acc_im[bi][bj][r];
+            }
+    __syncthreads();
+    double2* out = scratch + (size_t)s * kTileCells;
+    for (int e = threadIdx.x; e < kTileCells; e += 256) {
+        const int ti = e / kHalo, tj = e - ti * kHalo;
+        const int c = ti * 32 + tj;
+        double sr = sm.red[c * 2], si = sm.red[c * 2 + 1];
+#pragma unroll
+        for (int ww = 1; ww < 4; ++ww) {
+            sr += sm.red[(ww * 1024 + c) * 2];
+            si += sm.red[(ww * 1024 + c) * 2 + 1];
+        }
+        out[e] = make_double2(sr, si);
+    }
+}
+
 // Grid cell (n2, n1) = sum over the neighbouring bins b' (3x3, fixed order) and
 // their tiles (every kSpUnit-th subproblem, in order) of
 // tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1] where that index lies
 // in [0, 28).  One block per bin writes the bin's cells.
-__global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__ scratch,
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict__ scratch,
                                                       const int* __restrict__ subbeg, BinGeom geo, int nb2,
-                                                      float2* __restrict__ Gt)
+                                                      cplx<T>* __restrict__ Gt)
 {
+    using C = cplx<T>;
     const int nb1 = geo.nb1;
     const int bin = blockIdx.x;
     const int b1 = bin % nb1, b2 = bin / nb1;
@@ -1483,12 +1605,12 @@ __global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__
     // tiles are summed in (neighbour, subproblem) order; the loads go out in
     // groups of 8 so that a bin with many subproblems does not pay the memory
     // latency once per tile
-    float sr = 0.f, si = 0.f;
+    T sr = 0, si = 0;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         int s = beg[j];
         for (; s + 7 * kSpUnit < end[j]; s += 8 * kSpUnit) {
-            float2 v[8];
+            C v[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] = scratch[(size_t)(s + q * kSpUnit) * kTileCells + off[j]];
 #pragma unroll
@@ -1498,12 +1620,12 @@ __global__ __launch_bounds__(256) void k_spread_merge(const float2* __restrict__
             }
         }
         for (; s < end[j]; s += kSpUnit) {
-            const float2 v = scratch[(size_t)s * kTileCells + off[j]];
+            const C v = scratch[(size_t)s * kTileCells + off[j]];
             sr += v.x;
             si += v.y;
         }
     }
-    Gt[(size_t)n2 * geo.L1 + n1] = make_float2(sr, si);
+    Gt[(size_t)n2 * geo.L1 + n1] = mk(sr, si);
 }
 
 }  // namespace nfft2d

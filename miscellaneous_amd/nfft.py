@@ -262,7 +262,7 @@ class Plan:
 
     def lsq_adjoint(self, out: torch.Tensor | None = None) -> torch.Tensor:
         """dL/df of the least-squares loss of the last ``type2_lsq``: the adjoint of the residual the
-        plan holds (``adjoint(r)``, bit for bit for complex64), spread in the plan's tile order."""
+        plan holds (``adjoint(r)``, bit for bit on grids of at least 44 x 44), spread in the plan's tile order."""
         self._need_points()
         if out is None:
             out = torch.empty(self.N1, self.N2, dtype=self.dtype, device=self.device)
