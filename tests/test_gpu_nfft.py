@@ -205,8 +205,10 @@ def test_plan_state_and_size_errors():
     assert lib.nfft2d_plan_num_points(h) == -5
 
 
-def test_clustered_points_and_dense_tiles():
-    """All points in a few cells (many subproblems per tile) and exact ties."""
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+def test_clustered_points_and_dense_tiles(dtype):
+    """All points in a few cells (many subproblems per tile, so the spread's
+    two-subproblem units and their second pass) and exact ties."""
     rng = np.random.default_rng(4)
     N, M = 64, 50000
     x1 = np.concatenate([rng.normal(0, 0.3, M // 2), np.full(M // 4, 0.25), rng.uniform(-32, 32, M - M // 2 - M // 4)])
@@ -215,11 +217,14 @@ def test_clustered_points_and_dense_tiles():
     x2 = x2.astype(np.float32)
     f = cn01(rng, (N, N))
     d = cn01(rng, M)
-    y, plan = run_type2(f, x1, x2)
+    wide = dtype == torch.complex128
+    tol = TOL64 if wide else TOL32
+    y, plan = run_type2(f.astype(np.complex128) if wide else f, x1.astype(np.float64) if wide else x1,
+                        x2.astype(np.float64) if wide else x2, dtype)
     sel = rng.choice(M, 3000, replace=False)
-    assert oracle.rel_l2(y[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel])) < TOL32
-    F = plan.type1(T(d)).cpu().numpy()
-    assert oracle.rel_l2(F, oracle.ndft_type1(d.astype(np.complex128), x1, x2, N, N)) < TOL32
+    assert oracle.rel_l2(y[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel])) < tol
+    F = plan.type1(T(d).to(dtype)).cpu().numpy()
+    assert oracle.rel_l2(F, oracle.ndft_type1(d.astype(np.complex128), x1, x2, N, N)) < tol
 
 
 @pytest.mark.parametrize("N1,N2", [(8, 8), (16, 64), (256, 32), (2048, 16), (4096, 8), (6, 10), (88, 88)])
