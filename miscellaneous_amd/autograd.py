@@ -23,8 +23,9 @@ from .nfft import Plan
 class NFFT2DType2(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor, plan: Plan):
-        f = f.contiguous()
-        x1c, x2c = x1.detach().contiguous(), x2.detach().contiguous()
+        # lazy conjugate / negative views are resolved: the library reads memory
+        f = f.resolve_conj().resolve_neg().contiguous()
+        x1c, x2c = x1.detach().resolve_neg().contiguous(), x2.detach().resolve_neg().contiguous()
         plan.setpts(x1c, x2c)
         y = plan.type2(f)
         ctx.plan = plan
@@ -40,7 +41,7 @@ class NFFT2DType2(torch.autograd.Function):
         if plan.pts_version != ctx.version:      # the plan was re-pointed since the forward
             plan.setpts(x1, x2)
             ctx.version = plan.pts_version
-        g = g.contiguous()
+        g = g.resolve_conj().resolve_neg().contiguous()
         grad_f = grad_x1 = grad_x2 = None
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
             # before the adjoint, which overwrites the grid

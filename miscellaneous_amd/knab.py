@@ -1,0 +1,164 @@
+"""Band-limited 2-D interpolation of polar-grid samples by the type-2 NFFT.
+
+The reference's ``knabNFFT2D`` (``BP_FBP_FFBP/knabNFFT2D.m:1-79``), the step
+above ``NFFT1_2D`` in its filtered and fast-factorised backprojection
+(``BP_FBP_FFBP/FBP.m:95``, ``FFBP_POLAR_NFFT.m:89,141``): samples on a regular
+(rho, cos alpha) grid are interpolated at arbitrary output points with
+J. Selva's convolution-based trigonometric interpolation
+(``NFFT_Based_Selvas_Approach_for_Signal_Interp_2D.m:4-28``):
+
+    sF  = fft2(samples), repeated periodically to 2 kg + 1 frequencies per
+          dimension and multiplied by the window G = Gn (x) Gm
+          (``KnabWindow.m``: a pass band of 1/N, tapered by the convolution of
+          a rectangle with an approximate prolate window, ``ConvWindow.m``);
+    pad sF with zeros to a N x b M (oversampling a, b detected from the
+          output sampling density);
+    out = conj(type2(conj(sF), rho / (T1 / a), cos / (T2 / b))) / (a b N M),
+
+then output points outside the input grid get ``ExtrapVal``.
+
+Here the type-2 runs on the library (``Plan.setpts_type2``: tile sort, grid
+passes and gather, complex64).  The input spectrum is a plain 2-D FFT of the
+caller's samples (``torch.fft.fft2``, as the reference calls MATLAB's
+``fft2``), and the window tables are computed on the host in FP64 once per
+(T, B, P, N): the reference integrates ``ConvWindow`` with MATLAB's adaptive
+``quad``; here the substitution f = B/2 sin(theta) removes the square-root
+edge of the integrand and a 64-point Gauss-Legendre rule integrates the
+remaining smooth function to FP64 rounding.
+
+Sizes: a N and b M must be even (the library's even-size plans; the reference
+also has an odd branch), and at most 4096 (2048 when not a power of two).
+"""
+from __future__ import annotations
+
+import functools
+import math
+
+import numpy as np
+import torch
+
+from .nfft import Plan
+
+__all__ = ["knabNFFT2D", "knab_window"]
+
+_GL_X, _GL_W = np.polynomial.legendre.leggauss(64)
+
+
+def _conv_window_integral(lo: float, hi: float, B: float, T: float) -> float:
+    """int_lo^hi W(f) df for W(f) = I0(pi B T/2 sqrt(1 - (2f/B)^2)) / (B sinh(pi B T/2)/(pi B T/2)),
+    [lo, hi] inside [-B/2, B/2]; with f = B/2 sin(theta): W df = I0(a cos theta) cos theta B/2 dtheta."""
+    a = math.pi * B * T / 2
+    t0 = math.asin(max(-1.0, min(1.0, 2 * lo / B)))
+    t1 = math.asin(max(-1.0, min(1.0, 2 * hi / B)))
+    th = 0.5 * (t1 - t0) * _GL_X + 0.5 * (t1 + t0)
+    vals = np.i0(a * np.cos(th)) * np.cos(th)
+    return float(0.5 * (t1 - t0) * np.dot(_GL_W, vals) * (B / 2) / (B * math.sinh(a) / a))
+
+
+@functools.lru_cache(maxsize=64)
+def knab_window(T: float, B: float, P: int, N: int):
+    """The spectral window of KnabWindow.m:11-47 for N samples at step T and bilateral bandwidth B:
+    returns (G, kg), G of length 2 kg + 1 (FP64, host)."""
+    df = 1.0 / (N * T)
+    kg = int(math.floor((1.0 / T - B / 2) / df))
+    f = -df * kg + df * np.arange(kg + 1)
+    B0 = 1.0 / T
+    Bw = B0 - B                       # bandwidth of the prolate window
+    inside = Bw / 2 <= f + B0 / 2
+    G = np.zeros(2 * kg + 1)
+    G[:kg + 1][inside] = 1.0
+    out = np.nonzero(~inside)[0]
+    for k in range(out[-1] + 1 if out.size else 0):
+        lo, hi = max(-Bw / 2, f[k] - B0 / 2), min(Bw / 2, f[k] + B0 / 2)
+        if lo <= hi:
+            G[k] = _conv_window_integral(lo, hi, Bw, 2 * P * T)
+    G = G + G[::-1]
+    G[kg] -= 1.0
+    return G / N, kg
+
+
+def _round(x: float) -> int:
+    """MATLAB round (half away from zero)."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+_plans: dict = {}
+
+
+def _plan(n1: int, n2: int, device: torch.device) -> Plan:
+    key = (n1, n2, device)
+    p = _plans.get(key)
+    if p is None:
+        p = _plans[key] = Plan(n1, n2, torch.complex64, device=device)
+    return p
+
+
+def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT_DATA: torch.Tensor,
+               RHO_OUT: torch.Tensor, COSALPHA_OUT: torch.Tensor, B1: float, B2: float, P: int = 18) -> torch.Tensor:
+    """knabNFFT2D.m:1-79 on the GPU.  RHO_IN, COSALPHA_IN: the input meshgrid (rows: cos alpha,
+    columns: rho), INPUT_DATA: complex samples on it (same shape), RHO_OUT, COSALPHA_OUT: the
+    output meshgrid; B1, B2: bilateral bandwidths in rho and cos alpha; returns complex64 of
+    RHO_OUT's shape.  All tensors on one ROCm device."""
+    dev = INPUT_DATA.device
+    if dev.type != "cuda":
+        raise ValueError("knabNFFT2D: tensors must be on a ROCm device")
+    if INPUT_DATA.dim() != 2 or INPUT_DATA.shape != RHO_IN.shape or RHO_IN.shape != COSALPHA_IN.shape:
+        raise ValueError("The lengths of the X and Y vectors must match Z.")
+    if min(INPUT_DATA.shape) < 2:
+        raise ValueError("Z must be at least 2-by-2.")
+    if RHO_OUT.shape != COSALPHA_OUT.shape or RHO_OUT.dim() != 2:
+        raise ValueError("RHO_OUT and COSALPHA_OUT must be matrices of one shape")
+    f64 = torch.float64
+    rho_in, cos_in = RHO_IN.to(f64), COSALPHA_IN.to(f64)
+    T1 = float(torch.diff(rho_in[0, :]).mean())
+    T2 = float(torch.diff(cos_in[:, 0]).mean())
+    rho_s = RHO_OUT.to(f64) - rho_in[0, 0]
+    cos_s = COSALPHA_OUT.to(f64) - cos_in[0, 0]
+    len1, len2 = rho_s.shape
+    Nn = float(rho_s.max() - rho_s.min()) / (T1 * (1 - 1 / len2))
+    Mm = float(cos_s.max() - cos_s.min()) / (T2 * (1 - 1 / len1))
+    if Nn < 1 or math.isnan(Nn):
+        Nn = 1.0
+    if Mm < 1 or math.isnan(Mm):
+        Mm = 1.0
+    a, b = _round(len2 / Nn), _round(len1 / Mm)
+    if a < 2:                          # extra output points at the origin, cropped at the end
+        a = 2
+        ii = _round(a * Nn) - rho_s.shape[1]
+        rho_s = torch.nn.functional.pad(rho_s, (0, ii))
+        cos_s = torch.nn.functional.pad(cos_s, (0, ii))
+    if b < 2:
+        b = 2
+        ii = _round(b * Mm) - rho_s.shape[0]
+        rho_s = torch.nn.functional.pad(rho_s, (0, 0, 0, ii))
+        cos_s = torch.nn.functional.pad(cos_s, (0, 0, 0, ii))
+    data = INPUT_DATA.to(torch.complex64).t()        # N (rho) x M (cos alpha)
+    N, M = data.shape
+    if (a * N) % 2 or (b * M) % 2:
+        raise ValueError(f"knabNFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
+    Gn, kgn = knab_window(T1, float(B1), int(P), N)
+    Gm, kgm = knab_window(T2, float(B2), int(P), M)
+    if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M:
+        raise ValueError("knabNFFT2D: the window is wider than the padded spectrum")
+    # the spectrum, repeated periodically to 2 kg + 1 frequencies and windowed (Selva eq. 25)
+    sF = torch.fft.fft2(data)
+    i_n = torch.remainder(torch.arange(-kgn, kgn + 1, device=dev), N)
+    i_m = torch.remainder(torch.arange(-kgm, kgm + 1, device=dev), M)
+    G = torch.from_numpy(np.outer(Gn, Gm) * float(N * a) * float(M * b)).to(dev)
+    sF = (sF.index_select(0, i_n).index_select(1, i_m).to(torch.complex128) * G).to(torch.complex64)
+    # zero padding to a N x b M, frequency 0 at (a N / 2, b M / 2): the library's centred layout
+    F = torch.zeros(a * N, b * M, dtype=torch.complex64, device=dev)
+    F[a * N // 2 - kgn:a * N // 2 + kgn + 1, b * M // 2 - kgm:b * M // 2 + kgm + 1] = sF
+    # points in units of the padded grid, wrapped to one period in FP64 before the FP32 cast
+    x1 = rho_s.reshape(-1) / (T1 / a)
+    x2 = cos_s.reshape(-1) / (T2 / b)
+    x1 = (x1 - a * N * torch.round(x1 / (a * N))).to(torch.float32)
+    x2 = (x2 - b * M * torch.round(x2 / (b * M))).to(torch.float32)
+    y = _plan(a * N, b * M, dev).setpts_type2(x1, x2, torch.conj_physical(F))
+    res = (y.conj() / float(a * b * N * M)).reshape(rho_s.shape)[:len1, :len2].clone()
+    rho_axis, cos_axis = rho_in[0, :], cos_in[:, 0]
+    xout = (RHO_OUT[0, :].to(f64) < rho_axis.min()) | (RHO_OUT[0, :].to(f64) > rho_axis.max())
+    yout = (COSALPHA_OUT[:, 0].to(f64) < cos_axis.min()) | (COSALPHA_OUT[:, 0].to(f64) > cos_axis.max())
+    res[:, xout] = ExtrapVal
+    res[yout, :] = ExtrapVal
+    return res

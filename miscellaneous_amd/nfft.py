@@ -56,6 +56,9 @@ def _require(t: torch.Tensor, dtype, numel: int, name: str, device=None) -> None
         raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
+    if t.is_conj() or t.is_neg():
+        # torch's lazy conjugate / negative views leave the memory unchanged
+        raise ValueError(f"{name} is a lazy conjugate or negative view: pass t.resolve_conj().resolve_neg()")
     if t.numel() != numel:
         raise ValueError(f"{name} must have {numel} elements, got {t.numel()}")
 

@@ -205,6 +205,21 @@ def test_plan_state_and_size_errors():
     assert lib.nfft2d_plan_num_points(h) == -5
 
 
+def test_lazy_conjugate_views_are_refused_or_resolved():
+    """torch's .conj() only sets a flag; the library reads memory, so the plan API
+    refuses such views, and the autograd wrapper resolves them."""
+    rng = np.random.default_rng(17)
+    N, M = 32, 500
+    x1 = T(((rng.random(M) - 0.5) * N).astype(np.float32))
+    x2 = T(((rng.random(M) - 0.5) * N).astype(np.float32))
+    f = T(cn01(rng, (N, N)))
+    plan = m.Plan(N, N, device=DEV).setpts(x1, x2)
+    with pytest.raises(ValueError, match="lazy conjugate"):
+        plan.type2(f.conj())
+    y = m.nfft2d_type2(f.conj(), x1, x2, plan)
+    assert torch.equal(y, plan.type2(f.conj().resolve_conj()))
+
+
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
 def test_clustered_points_and_dense_tiles(dtype):
     """All points in a few cells (many subproblems per tile, so the spread's
