@@ -19,7 +19,9 @@ descent update), also graph-replayed.
 Also reported (skip with --no-extra): the batch of 256 uneven type-2 NFFTs
 (batched setpts + type-2 in one graph, against the same transforms one plan at
 a time), the type-2 NDFT as an MFMA contraction in FP32 / BF16x3 / FP16 with
-its MFMA utilisation, and the 1024^2 / M=4M uniform type-2.
+its MFMA utilisation, the 1024^2 / M=4M uniform type-2, the type-1 and the
+complex128 forms on the polar set, and the band-limited polar interpolation
+(knabNFFT2D) built on the type-2.
 
 cpu_baseline: the CPU restatement of the reference algorithm (FP32,
 OpenMP over the host cores) on the same type-2 workload, timed for a bounded
@@ -473,6 +475,54 @@ def bench_fp64(args, dev):
                       "rel_l2_256_modes": rel1}}
 
 
+def bench_knab(args, dev):
+    """knabNFFT2D (BP_FBP_FFBP/knabNFFT2D.m, the band-limited polar interpolation above NFFT1_2D in
+    FBP.m:95): 512 x 512 (cos alpha x rho) samples of a band-limited test signal at half its Nyquist
+    rate, interpolated at 1024 x 1024 points inside the input grid; eager calls (the wrapper reads
+    the grid steps and extents on the host).  Accuracy: rel-L2 against the signal itself."""
+    import miscellaneous_amd as m
+
+    n, T1, T2 = 512, 0.05, 0.002
+    f64 = torch.float64
+    rho = 100 + T1 * torch.arange(n, dtype=f64, device=dev)
+    cs = -0.1 + T2 * torch.arange(n, dtype=f64, device=dev)
+    RHO, COS = torch.meshgrid(rho, cs, indexing="xy")
+    B1, B2 = 0.5 / T1, 0.5 / T2
+    rng = np.random.default_rng(SEED + 10)
+    u = torch.from_numpy((rng.random(6) - 0.5) * B1 * 0.9).to(dev)
+    v = torch.from_numpy((rng.random(6) - 0.5) * B2 * 0.9).to(dev)
+    c = torch.from_numpy(rng.standard_normal(6) + 1j * rng.standard_normal(6)).to(dev)
+
+    def sig(R, C):
+        ph = 2 * np.pi * (R[..., None] * u + C[..., None] * v)
+        return (torch.polar(torch.ones_like(ph), ph) * c).sum(-1)
+
+    data = sig(RHO, COS).to(torch.complex64)
+    RO, CO = torch.meshgrid(torch.linspace(float(rho[20]), float(rho[-21]), 1024, dtype=f64, device=dev),
+                            torch.linspace(float(cs[20]), float(cs[-21]), 1024, dtype=f64, device=dev),
+                            indexing="xy")
+
+    def call():
+        return m.knabNFFT2D(0.0, RHO, COS, data, RO, CO, B1, B2)
+
+    for _ in range(3):
+        out = call()
+    torch.cuda.synchronize()
+    reps = max(args.steps // 10, 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = call()
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t0) / reps
+    ref = sig(RO, CO)
+    rel = ((out.to(torch.complex128) - ref).norm() / ref.norm()).item()
+    return {"config": "knabNFFT2D: 512x512 polar samples (B = 0.5/T in both variables, P = 18), 1024x1024 "
+                      "output points; spectrum FFT + window + zero padding (a = b = 2) + type-2 NFFT on "
+                      "1024x1024 modes; eager calls",
+            "value": RO.numel() / s / 1e6, "unit": "Mpts/s", "ms_per_call": s * 1e3,
+            "rel_l2_vs_bandlimited_signal": rel}
+
+
 def cpu_baseline(t2, seconds: float):
     import oracle
 
@@ -529,6 +579,7 @@ def main():
         extra["large_1024"] = bench_large(args, dev)
         extra["type1_polar512"] = bench_type1(args, dev)
         extra["fp64_polar512"] = bench_fp64(args, dev)
+        extra["knab_polar_interp"] = bench_knab(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]

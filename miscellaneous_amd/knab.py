@@ -83,6 +83,19 @@ def _round(x: float) -> int:
 
 
 _plans: dict = {}
+_windows: dict = {}
+
+
+def _window_tensor(wn: tuple, wm: tuple, scale: float, device: torch.device) -> torch.Tensor:
+    """scale * Gn (x) Gm as a complex64 device tensor, kept per (knab_window arguments, scale, device)."""
+    key = (wn, wm, scale, device)
+    g = _windows.get(key)
+    if g is None:
+        if len(_windows) >= 16:
+            _windows.clear()
+        Gn, Gm = knab_window(*wn)[0], knab_window(*wm)[0]
+        g = _windows[key] = torch.from_numpy(np.outer(Gn, Gm) * scale).to(device).to(torch.complex64)
+    return g
 
 
 def _plan(n1: int, n2: int, device: torch.device) -> Plan:
@@ -109,14 +122,16 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
     if RHO_OUT.shape != COSALPHA_OUT.shape or RHO_OUT.dim() != 2:
         raise ValueError("RHO_OUT and COSALPHA_OUT must be matrices of one shape")
     f64 = torch.float64
-    rho_in, cos_in = RHO_IN.to(f64), COSALPHA_IN.to(f64)
-    T1 = float(torch.diff(rho_in[0, :]).mean())
-    T2 = float(torch.diff(cos_in[:, 0]).mean())
-    rho_s = RHO_OUT.to(f64) - rho_in[0, 0]
-    cos_s = COSALPHA_OUT.to(f64) - cos_in[0, 0]
-    len1, len2 = rho_s.shape
-    Nn = float(rho_s.max() - rho_s.min()) / (T1 * (1 - 1 / len2))
-    Mm = float(cos_s.max() - cos_s.min()) / (T2 * (1 - 1 / len1))
+    rho_axis, cos_axis = RHO_IN[0, :].to(f64), COSALPHA_IN[:, 0].to(f64)
+    ro, co = RHO_OUT.to(f64), COSALPHA_OUT.to(f64)
+    # the grid steps, origins and output extents in one transfer to the host
+    hs = torch.stack([torch.diff(rho_axis).mean(), torch.diff(cos_axis).mean(), rho_axis[0], cos_axis[0],
+                      ro.max() - ro.min(), co.max() - co.min()]).cpu().tolist()
+    T1, T2, r0, c0, rext, cext = hs
+    len1, len2 = ro.shape
+    Nn = rext / (T1 * (1 - 1 / len2))
+    Mm = cext / (T2 * (1 - 1 / len1))
+    rho_s, cos_s = ro - r0, co - c0
     if Nn < 1 or math.isnan(Nn):
         Nn = 1.0
     if Mm < 1 or math.isnan(Mm):
@@ -136,16 +151,16 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
     N, M = data.shape
     if (a * N) % 2 or (b * M) % 2:
         raise ValueError(f"knabNFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
-    Gn, kgn = knab_window(T1, float(B1), int(P), N)
-    Gm, kgm = knab_window(T2, float(B2), int(P), M)
+    wn, wm = (T1, float(B1), int(P), N), (T2, float(B2), int(P), M)
+    kgn, kgm = knab_window(*wn)[1], knab_window(*wm)[1]
     if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M:
         raise ValueError("knabNFFT2D: the window is wider than the padded spectrum")
     # the spectrum, repeated periodically to 2 kg + 1 frequencies and windowed (Selva eq. 25)
     sF = torch.fft.fft2(data)
     i_n = torch.remainder(torch.arange(-kgn, kgn + 1, device=dev), N)
     i_m = torch.remainder(torch.arange(-kgm, kgm + 1, device=dev), M)
-    G = torch.from_numpy(np.outer(Gn, Gm) * float(N * a) * float(M * b)).to(dev)
-    sF = (sF.index_select(0, i_n).index_select(1, i_m).to(torch.complex128) * G).to(torch.complex64)
+    G = _window_tensor(wn, wm, float(N * a) * float(M * b), dev)
+    sF = sF.index_select(0, i_n).index_select(1, i_m) * G
     # zero padding to a N x b M, frequency 0 at (a N / 2, b M / 2): the library's centred layout
     F = torch.zeros(a * N, b * M, dtype=torch.complex64, device=dev)
     F[a * N // 2 - kgn:a * N // 2 + kgn + 1, b * M // 2 - kgm:b * M // 2 + kgm + 1] = sF
@@ -156,9 +171,8 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
     x2 = (x2 - b * M * torch.round(x2 / (b * M))).to(torch.float32)
     y = _plan(a * N, b * M, dev).setpts_type2(x1, x2, torch.conj_physical(F))
     res = (y.conj() / float(a * b * N * M)).reshape(rho_s.shape)[:len1, :len2].clone()
-    rho_axis, cos_axis = rho_in[0, :], cos_in[:, 0]
-    xout = (RHO_OUT[0, :].to(f64) < rho_axis.min()) | (RHO_OUT[0, :].to(f64) > rho_axis.max())
-    yout = (COSALPHA_OUT[:, 0].to(f64) < cos_axis.min()) | (COSALPHA_OUT[:, 0].to(f64) > cos_axis.max())
+    xout = (ro[0, :] < rho_axis.min()) | (ro[0, :] > rho_axis.max())
+    yout = (co[:, 0] < cos_axis.min()) | (co[:, 0] > cos_axis.max())
     res[:, xout] = ExtrapVal
     res[yout, :] = ExtrapVal
     return res
