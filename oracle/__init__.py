@@ -16,6 +16,8 @@ algorithm:
   closed forms every fast result is checked against).
 * ``window``, ``window_deriv``, ``bessel_i0``, ``spectral_window``, ``fft2`` --
   the pieces, exposed so that each can be pinned on its own.
+* ``oracle.knab`` (its own module) -- the band-limited polar interpolation
+  ``BP_FBP_FFBP/knabNFFT2D.m`` and its helpers, on ``nfft1_2d``.
 
 Pins (what checks each function, in ``tests/test_oracle.py``):
 
