@@ -18,8 +18,8 @@ from .nfft import (Plan, NFFT2_2D_GPU, NFFT1_2D_GPU, NDFT2_2D_GPU, Calculate_cuF
 from .autograd import NFFT2DType2, nfft2d_type2
 from .ndft import NdftPlan
 from .batch import Batch
-from .knab import knabNFFT2D, knab_window
+from .knab import knabNFFT2D, rectNFFT2D, knab_window
 
 __all__ = ["NFFT2DError", "load_library", "LIB_PATH", "Plan", "NFFT2_2D_GPU", "NFFT1_2D_GPU", "NDFT2_2D_GPU",
            "Calculate_cuFFT_plan_C2C_NFFT2_2D", "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D",
-           "NFFT2DType2", "nfft2d_type2", "NdftPlan", "Batch", "knabNFFT2D", "knab_window"]
+           "NFFT2DType2", "nfft2d_type2", "NdftPlan", "Batch", "knabNFFT2D", "rectNFFT2D", "knab_window"]

@@ -15,7 +15,8 @@ J. Selva's convolution-based trigonometric interpolation
           output sampling density);
     out = conj(type2(conj(sF), rho / (T1 / a), cos / (T2 / b))) / (a b N M),
 
-then output points outside the input grid get ``ExtrapVal``.
+then output points outside the input grid get ``ExtrapVal``.  ``rectNFFT2D``
+(``BP_FBP_FFBP/rectNFFT2D.m:1-77``) is the same with a rectangular window.
 
 Here the type-2 runs on the library (``Plan.setpts_type2``: tile sort, grid
 passes and gather, complex64).  The input spectrum is a plain 2-D FFT of the
@@ -39,7 +40,7 @@ import torch
 
 from .nfft import Plan
 
-__all__ = ["knabNFFT2D", "knab_window"]
+__all__ = ["knabNFFT2D", "rectNFFT2D", "knab_window"]
 
 _GL_X, _GL_W = np.polynomial.legendre.leggauss(64)
 
@@ -112,9 +113,21 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
     columns: rho), INPUT_DATA: complex samples on it (same shape), RHO_OUT, COSALPHA_OUT: the
     output meshgrid; B1, B2: bilateral bandwidths in rho and cos alpha; returns complex64 of
     RHO_OUT's shape.  All tensors on one ROCm device."""
+    return _interp("knab", ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, COSALPHA_OUT, B1, B2, P)
+
+
+def rectNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT_DATA: torch.Tensor,
+               RHO_OUT: torch.Tensor, COSALPHA_OUT: torch.Tensor, B1: float, B2: float) -> torch.Tensor:
+    """rectNFFT2D.m:1-77 on the GPU: the same steps with a rectangular spectral window (plain
+    trigonometric interpolation of the samples' periodic extension), whose half-width kg comes, as
+    in the reference, from the output extent in input samples rather than from the input size."""
+    return _interp("rect", ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, COSALPHA_OUT, B1, B2, 0)
+
+
+def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, COSALPHA_OUT, B1, B2, P):
     dev = INPUT_DATA.device
     if dev.type != "cuda":
-        raise ValueError("knabNFFT2D: tensors must be on a ROCm device")
+        raise ValueError(f"{window}NFFT2D: tensors must be on a ROCm device")
     if INPUT_DATA.dim() != 2 or INPUT_DATA.shape != RHO_IN.shape or RHO_IN.shape != COSALPHA_IN.shape:
         raise ValueError("The lengths of the X and Y vectors must match Z.")
     if min(INPUT_DATA.shape) < 2:
@@ -150,17 +163,24 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
     data = INPUT_DATA.to(torch.complex64).t()        # N (rho) x M (cos alpha)
     N, M = data.shape
     if (a * N) % 2 or (b * M) % 2:
-        raise ValueError(f"knabNFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
-    wn, wm = (T1, float(B1), int(P), N), (T2, float(B2), int(P), M)
-    kgn, kgm = knab_window(*wn)[1], knab_window(*wm)[1]
-    if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M:
-        raise ValueError("knabNFFT2D: the window is wider than the padded spectrum")
+        raise ValueError(f"{window}NFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
+    if window == "knab":
+        wn, wm = (T1, float(B1), int(P), N), (T2, float(B2), int(P), M)
+        kgn, kgm = knab_window(*wn)[1], knab_window(*wm)[1]
+    else:   # rectNFFT2D.m:62-65: kg from the output-extent Nn, Mm
+        kgn = int(math.floor((1 / T1 - float(B1) / 2) / (1 / (Nn * T1))))
+        kgm = int(math.floor((1 / T2 - float(B2) / 2) / (1 / (Mm * T2))))
+    if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M or kgn < 0 or kgm < 0:
+        raise ValueError(f"{window}NFFT2D: the window is wider than the padded spectrum")
     # the spectrum, repeated periodically to 2 kg + 1 frequencies and windowed (Selva eq. 25)
     sF = torch.fft.fft2(data)
     i_n = torch.remainder(torch.arange(-kgn, kgn + 1, device=dev), N)
     i_m = torch.remainder(torch.arange(-kgm, kgm + 1, device=dev), M)
-    G = _window_tensor(wn, wm, float(N * a) * float(M * b), dev)
-    sF = sF.index_select(0, i_n).index_select(1, i_m) * G
+    sF = sF.index_select(0, i_n).index_select(1, i_m)
+    if window == "knab":
+        sF = sF * _window_tensor(wn, wm, float(N * a) * float(M * b), dev)
+    else:   # G = 1 and the samples divided by N M (rectNFFT2D.m:70)
+        sF = sF * (float(a) * float(b))
     # zero padding to a N x b M, frequency 0 at (a N / 2, b M / 2): the library's centred layout
     F = torch.zeros(a * N, b * M, dtype=torch.complex64, device=dev)
     F[a * N // 2 - kgn:a * N // 2 + kgn + 1, b * M // 2 - kgm:b * M // 2 + kgm + 1] = sF

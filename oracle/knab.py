@@ -12,7 +12,9 @@ It restates, step by step and in FP64 numpy:
   tolerance: both are adaptive and reach the integral to ~1e-12);
 * ``selva_interp_2d``        -- ``BP_FBP_FFBP/NFFT_Based_Selvas_Approach_for_Signal_Interp_2D.m:4-28``,
   with the oracle's ``nfft1_2d`` (``NFFT1_2D.m``) as its NUFFT;
-* ``knab_nfft2d``            -- ``BP_FBP_FFBP/knabNFFT2D.m:1-79``.
+* ``knab_nfft2d``            -- ``BP_FBP_FFBP/knabNFFT2D.m:1-79``;
+* ``rect_nfft2d``            -- ``BP_FBP_FFBP/rectNFFT2D.m:1-77`` (the same steps
+  with a rectangular window).
 
 Pins (``tests/test_knab.py``):
 
@@ -22,7 +24,9 @@ Pins (``tests/test_knab.py``):
   substitution f = B/2 sin(theta)), and its pass band equal to 1/N;
 * ``knab_nfft2d`` reproduces band-limited signals (sums of 2-D exponentials
   with frequencies inside the bands B1, B2) at off-grid points, and marks
-  points outside the input grid with ExtrapVal.
+  points outside the input grid with ExtrapVal;
+* ``rect_nfft2d`` reproduces signals periodic on the input grid (frequencies on
+  its DFT grid, inside the bands) at off-grid points.
 
 Only the even-size branch of the zero padding (a N and b M even) is restated;
 odd products raise, as in the GPU form.
@@ -100,10 +104,11 @@ def _round(x: float) -> int:
     return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
 
 
-def knab_nfft2d(extrap_val, rho_in: np.ndarray, cos_in: np.ndarray, data: np.ndarray, rho_out: np.ndarray,
-                cos_out: np.ndarray, B1: float, B2: float, P: int = 18) -> np.ndarray:
-    """knabNFFT2D.m:1-79.  rho_in, cos_in: input meshgrid (rows: cos alpha, columns: rho);
-    data: the samples on it; rho_out, cos_out: output meshgrid (len1 x len2)."""
+def _interp_nfft2d(window: str, extrap_val, rho_in: np.ndarray, cos_in: np.ndarray, data: np.ndarray,
+                   rho_out: np.ndarray, cos_out: np.ndarray, B1: float, B2: float, P: int = 18) -> np.ndarray:
+    """knabNFFT2D.m:1-79 (window "knab") and rectNFFT2D.m:1-77 (window "rect"; they differ only in
+    knabNFFT2D.m:62-70 / rectNFFT2D.m:62-70).  rho_in, cos_in: input meshgrid (rows: cos alpha,
+    columns: rho); data: the samples on it; rho_out, cos_out: output meshgrid (len1 x len2)."""
     T1 = float(np.mean(np.diff(rho_in[0, :])))
     T2 = float(np.mean(np.diff(cos_in[:, 0])))
     rho_s = rho_out - rho_in[0, 0]
@@ -128,9 +133,15 @@ def knab_nfft2d(extrap_val, rho_in: np.ndarray, cos_in: np.ndarray, data: np.nda
         rho_s = np.pad(rho_s, ((0, ii), (0, 0)))
         cos_s = np.pad(cos_s, ((0, ii), (0, 0)))
     dataT = data.T
-    Gn, kgn = knab_window(T1, B1, P, dataT.shape[0])
-    Gm, kgm = knab_window(T2, B2, P, dataT.shape[1])
-    G = np.outer(Gn, Gm)
+    if window == "knab":
+        Gn, kgn = knab_window(T1, B1, P, dataT.shape[0])
+        Gm, kgm = knab_window(T2, B2, P, dataT.shape[1])
+        G = np.outer(Gn, Gm)
+    else:   # rectangular: G = 1, the samples divided by their count; kg from the output-extent N, M
+        kgn = int(math.floor((1 / T1 - B1 / 2) / (1 / (N * T1))))
+        kgm = int(math.floor((1 / T2 - B2 / 2) / (1 / (M * T2))))
+        G = 1.0
+        dataT = dataT / dataT.shape[0] / dataT.shape[1]
     res = selva_interp_2d(dataT, T1, T2, a, b, G, kgn, kgm, rho_s, cos_s).reshape(rho_s.shape)
     res = res[:len1, :len2].copy()
     xout = (rho_out[0, :] < rho_in[0, :].min()) | (rho_out[0, :] > rho_in[0, :].max())
@@ -138,3 +149,13 @@ def knab_nfft2d(extrap_val, rho_in: np.ndarray, cos_in: np.ndarray, data: np.nda
     res[:, xout] = extrap_val
     res[yout, :] = extrap_val
     return res
+
+
+def knab_nfft2d(extrap_val, rho_in, cos_in, data, rho_out, cos_out, B1, B2, P: int = 18) -> np.ndarray:
+    """knabNFFT2D.m:1-79: the prolate-tapered window (KnabWindow.m)."""
+    return _interp_nfft2d("knab", extrap_val, rho_in, cos_in, data, rho_out, cos_out, B1, B2, P)
+
+
+def rect_nfft2d(extrap_val, rho_in, cos_in, data, rho_out, cos_out, B1, B2) -> np.ndarray:
+    """rectNFFT2D.m:1-77: the rectangular window (plain trigonometric interpolation)."""
+    return _interp_nfft2d("rect", extrap_val, rho_in, cos_in, data, rho_out, cos_out, B1, B2)

@@ -71,3 +71,22 @@ def test_knab_rejects_odd_padded_sizes():
     RO, CO = np.meshgrid(np.linspace(rho[0], rho[-1], 183), np.linspace(cs[0], cs[-1], 102))
     with pytest.raises(ValueError, match="must be even"):
         m.knabNFFT2D(0.0, T(RHO), T(COS), T(data), T(RO), T(CO), 10.0, 250.0)
+
+
+@pytest.mark.parametrize("nr,nc,lo,hi", [(96, 80, 150, 170), (50, 60, 40, 45)])
+def test_rect_matches_oracle(nr, nc, lo, hi):
+    """rectNFFT2D (rectangular window) against the oracle, dense and coarse outputs."""
+    rng = np.random.default_rng(nr * nc)
+    rho, cs, RHO, COS = grid(nr, nc)
+    T1, T2 = rho[1] - rho[0], cs[1] - cs[0]
+    data = ((rng.standard_normal((nr, nc)) + 1j * rng.standard_normal((nr, nc))) / np.sqrt(2))
+    RO, CO = np.meshgrid(np.linspace(rho[0] - 2 * T1, rho[-1], lo), np.linspace(cs[3], cs[-1] + T2, hi))
+    print(f"case rect {nr}x{nc} -> {hi}x{lo}", flush=True)
+    ref = oknab.rect_nfft2d(0.5, RHO, COS, data, RO, CO, 0.5 / T1, 0.5 / T2)
+    out = m.rectNFFT2D(0.5, T(RHO), T(COS), T(data, torch.complex64), T(RO), T(CO), 0.5 / T1, 0.5 / T2)
+    out = out.cpu().numpy()
+    inside = ref != 0.5
+    assert np.all(out[~inside] == 0.5)
+    e = rel(out[inside], ref[inside])
+    print(f"rect rel-L2 {e:.3e}")
+    assert e < TOL

@@ -80,3 +80,22 @@ def test_oracle_marks_extrapolation_and_coarse_outputs():
     inner = (ro > rho[20]) & (ro < rho[-21])
     ref = sig(RO, CO)[:, inner]
     assert np.linalg.norm(out[:, inner] - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_oracle_rect_reproduces_periodic_signals():
+    """rectNFFT2D.m is trigonometric interpolation of the samples' periodic extension: signals with
+    frequencies on the input grid's DFT grid, inside the kept band and clear of its aliases
+    (|k| < N - kg), are reproduced at off-grid points."""
+    rng = np.random.default_rng(2)
+    nr, nc = 96, 80
+    rho, cs, RHO, COS = polar_like(rng, nr, nc)
+    T1, T2 = rho[1] - rho[0], cs[1] - cs[0]
+    k1 = rng.integers(-int(0.2 * nc), int(0.2 * nc), 4)
+    k2 = rng.integers(-int(0.2 * nr), int(0.2 * nr), 4)
+    c = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+    sig = lambda R, C: sum(c[q] * np.exp(2j * np.pi * (k1[q] * (R - rho[0]) / (nc * T1) +
+                                                       k2[q] * (C - cs[0]) / (nr * T2))) for q in range(4))
+    RO, CO = np.meshgrid(np.linspace(rho[0], rho[-1], 150), np.linspace(cs[0], cs[-1], 170))
+    out = oknab.rect_nfft2d(0.0, RHO, COS, sig(RHO, COS), RO, CO, 0.5 / T1, 0.5 / T2)
+    ref = sig(RO, CO)
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-10
