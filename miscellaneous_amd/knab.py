@@ -134,6 +134,8 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
         raise ValueError("Z must be at least 2-by-2.")
     if RHO_OUT.shape != COSALPHA_OUT.shape or RHO_OUT.dim() != 2:
         raise ValueError("RHO_OUT and COSALPHA_OUT must be matrices of one shape")
+    if min(RHO_OUT.shape) < 2:   # the output spacing (knabNFFT2D.m:36-37) divides by len - 1
+        raise ValueError("RHO_OUT must be at least 2-by-2.")
     f64 = torch.float64
     rho_axis, cos_axis = RHO_IN[0, :].to(f64), COSALPHA_IN[:, 0].to(f64)
     ro, co = RHO_OUT.to(f64), COSALPHA_OUT.to(f64)
