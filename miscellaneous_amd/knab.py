@@ -19,7 +19,7 @@ then output points outside the input grid get ``ExtrapVal``.  ``rectNFFT2D``
 (``BP_FBP_FFBP/rectNFFT2D.m:1-77``) is the same with a rectangular window.
 
 Here the type-2 runs on the library (``Plan.setpts_type2``: tile sort, grid
-passes and gather, complex64).  The input spectrum is a plain 2-D FFT of the
+passes and gather; complex64, or complex128 for complex128 samples).  The input spectrum is a plain 2-D FFT of the
 caller's samples (``torch.fft.fft2``, as the reference calls MATLAB's
 ``fft2``), and the window tables are computed on the host in FP64 once per
 (T, B, P, N): the reference integrates ``ConvWindow`` with MATLAB's adaptive
@@ -87,23 +87,23 @@ _plans: dict = {}
 _windows: dict = {}
 
 
-def _window_tensor(wn: tuple, wm: tuple, scale: float, device: torch.device) -> torch.Tensor:
-    """scale * Gn (x) Gm as a complex64 device tensor, kept per (knab_window arguments, scale, device)."""
-    key = (wn, wm, scale, device)
+def _window_tensor(wn: tuple, wm: tuple, scale: float, dtype, device: torch.device) -> torch.Tensor:
+    """scale * Gn (x) Gm as a complex device tensor, kept per (knab_window arguments, scale, dtype, device)."""
+    key = (wn, wm, scale, dtype, device)
     g = _windows.get(key)
     if g is None:
         if len(_windows) >= 16:
             _windows.clear()
         Gn, Gm = knab_window(*wn)[0], knab_window(*wm)[0]
-        g = _windows[key] = torch.from_numpy(np.outer(Gn, Gm) * scale).to(device).to(torch.complex64)
+        g = _windows[key] = torch.from_numpy(np.outer(Gn, Gm) * scale).to(device).to(dtype)
     return g
 
 
-def _plan(n1: int, n2: int, device: torch.device) -> Plan:
-    key = (n1, n2, device)
+def _plan(n1: int, n2: int, dtype, device: torch.device) -> Plan:
+    key = (n1, n2, dtype, device)
     p = _plans.get(key)
     if p is None:
-        p = _plans[key] = Plan(n1, n2, torch.complex64, device=device)
+        p = _plans[key] = Plan(n1, n2, dtype, device=device)
     return p
 
 
@@ -111,8 +111,9 @@ def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT
                RHO_OUT: torch.Tensor, COSALPHA_OUT: torch.Tensor, B1: float, B2: float, P: int = 18) -> torch.Tensor:
     """knabNFFT2D.m:1-79 on the GPU.  RHO_IN, COSALPHA_IN: the input meshgrid (rows: cos alpha,
     columns: rho), INPUT_DATA: complex samples on it (same shape), RHO_OUT, COSALPHA_OUT: the
-    output meshgrid; B1, B2: bilateral bandwidths in rho and cos alpha; returns complex64 of
-    RHO_OUT's shape.  All tensors on one ROCm device."""
+    output meshgrid; B1, B2: bilateral bandwidths in rho and cos alpha; returns RHO_OUT's shape,
+    complex128 for complex128 samples (the library's FP64 path), complex64 otherwise.  All tensors
+    on one ROCm device."""
     return _interp("knab", ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, COSALPHA_OUT, B1, B2, P)
 
 
@@ -162,7 +163,9 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
         ii = _round(b * Mm) - rho_s.shape[0]
         rho_s = torch.nn.functional.pad(rho_s, (0, 0, 0, ii))
         cos_s = torch.nn.functional.pad(cos_s, (0, 0, 0, ii))
-    data = INPUT_DATA.to(torch.complex64).t()        # N (rho) x M (cos alpha)
+    cdt = torch.complex128 if INPUT_DATA.dtype == torch.complex128 else torch.complex64
+    rdt = torch.float64 if cdt == torch.complex128 else torch.float32
+    data = INPUT_DATA.to(cdt).t()                    # N (rho) x M (cos alpha)
     N, M = data.shape
     if (a * N) % 2 or (b * M) % 2:
         raise ValueError(f"{window}NFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
@@ -180,18 +183,18 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
     i_m = torch.remainder(torch.arange(-kgm, kgm + 1, device=dev), M)
     sF = sF.index_select(0, i_n).index_select(1, i_m)
     if window == "knab":
-        sF = sF * _window_tensor(wn, wm, float(N * a) * float(M * b), dev)
+        sF = sF * _window_tensor(wn, wm, float(N * a) * float(M * b), cdt, dev)
     else:   # G = 1 and the samples divided by N M (rectNFFT2D.m:70)
         sF = sF * (float(a) * float(b))
     # zero padding to a N x b M, frequency 0 at (a N / 2, b M / 2): the library's centred layout
-    F = torch.zeros(a * N, b * M, dtype=torch.complex64, device=dev)
+    F = torch.zeros(a * N, b * M, dtype=cdt, device=dev)
     F[a * N // 2 - kgn:a * N // 2 + kgn + 1, b * M // 2 - kgm:b * M // 2 + kgm + 1] = sF
-    # points in units of the padded grid, wrapped to one period in FP64 before the FP32 cast
+    # points in units of the padded grid, wrapped to one period in FP64 before any FP32 cast
     x1 = rho_s.reshape(-1) / (T1 / a)
     x2 = cos_s.reshape(-1) / (T2 / b)
-    x1 = (x1 - a * N * torch.round(x1 / (a * N))).to(torch.float32)
-    x2 = (x2 - b * M * torch.round(x2 / (b * M))).to(torch.float32)
-    y = _plan(a * N, b * M, dev).setpts_type2(x1, x2, torch.conj_physical(F))
+    x1 = (x1 - a * N * torch.round(x1 / (a * N))).to(rdt)
+    x2 = (x2 - b * M * torch.round(x2 / (b * M))).to(rdt)
+    y = _plan(a * N, b * M, cdt, dev).setpts_type2(x1, x2, torch.conj_physical(F))
     res = (y.conj() / float(a * b * N * M)).reshape(rho_s.shape)[:len1, :len2].clone()
     xout = (ro[0, :] < rho_axis.min()) | (ro[0, :] > rho_axis.max())
     yout = (co[:, 0] < cos_axis.min()) | (co[:, 0] > cos_axis.max())

@@ -90,3 +90,20 @@ def test_rect_matches_oracle(nr, nc, lo, hi):
     e = rel(out[inside], ref[inside])
     print(f"rect rel-L2 {e:.3e}")
     assert e < TOL
+
+
+@pytest.mark.parametrize("fn", ["knab", "rect"])
+def test_fp64_samples_take_the_fp64_path(fn):
+    """complex128 samples run on complex128 plans: the oracle's FP64 values to 1e-9."""
+    rng = np.random.default_rng(21)
+    nr, nc = 96, 80
+    rho, cs, RHO, COS = grid(nr, nc)
+    T1, T2 = rho[1] - rho[0], cs[1] - cs[0]
+    data = ((rng.standard_normal((nr, nc)) + 1j * rng.standard_normal((nr, nc))) / np.sqrt(2))
+    RO, CO = np.meshgrid(np.linspace(rho[2], rho[-3], 150), np.linspace(cs[3], cs[-4], 170))
+    ref = getattr(oknab, f"{fn}_nfft2d")(0.0, RHO, COS, data, RO, CO, 0.5 / T1, 0.5 / T2)
+    out = getattr(m, f"{fn}NFFT2D")(0.0, T(RHO), T(COS), T(data), T(RO), T(CO), 0.5 / T1, 0.5 / T2)
+    assert out.dtype == torch.complex128
+    e = rel(out.cpu().numpy(), ref)
+    print(f"{fn} fp64 rel-L2 {e:.3e}")
+    assert e < 1e-9
