@@ -613,7 +613,7 @@ def main():
         "stage_roofline": {"a_b_precomp_pad_fft": stage_roofline(168 * t2["N"] * t2["N"], t2["grid_ms"]),
                            "c_gather": stage_roofline(t2["gather_bytes"], t2["gather_ms"])},
         "type2_hbm_frac": t2["type2_bytes"] / t2["s_exec"] / (HBM_PEAK_GBS * 1e9),
-        "roofline": {"kernel": "k_gather_pairs (stage c)", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_gather_pairs<2> (stage c)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": t2["gather_bytes"]},
     }

@@ -399,7 +399,8 @@ __device__ __forceinline__ int wave_run_rank(int* counter, int bin, bool act)
 
 // Pass 1: block b counts the bins of points [b*chunk, (b+1)*chunk) in an LDS
 // histogram (integer LDS atomics, one per run of equal bins in a wave);
-// rank[i] is the point's rank inside its (block, bin).  hist is [nblk][nbins].
+// rank[i] is the point's rank inside its (block, bin), written unless rank is
+// null (k_scatter_lds recomputes it).  hist is [nblk][nbins].
 constexpr int kHistThreads = 1024;   // 16 waves per block: 4 per SIMD to hide the load latency
 template <typename T>
 __device__ __forceinline__ void hist_body(int blk, int* h, const T* __restrict__ x, const T* __restrict__ y, int M,
@@ -429,7 +430,7 @@ __device__ __forceinline__ void hist_body(int blk, int* h, const T* __restrict__
             T t1, t2;
             const int bin = bin_of(xv[q], yv[q], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
             const int r = wave_run_rank(h, bin, act);
-            if (act) rank[i] = r;
+            if (act && rank != nullptr) rank[i] = r;
         }
     }
     __syncthreads();
@@ -702,6 +703,98 @@ __global__ __launch_bounds__(kScatThreads) void k_scatter_scan(
     }
 }
 
+// Pass 3 for grids whose bins fit one LDS array (nbins <= kMaxLdsBins): block
+// b takes the points of histogram block b, [b*chunk, (b+1)*chunk), and builds
+// base[bin] = first sorted position of (block b, bin) in LDS from its row of
+// column offsets.  Each point then takes base[bin] + its rank by incrementing
+// base[bin] (one LDS atomic per run of equal bins in a wave), so pass 1 stores
+// no per-point rank.  Inside a (block, bin) the points are ranked in the order
+// the waves reach the counter, as in pass 1.  The bins' subproblem lists are
+// split evenly over the blocks.
+static_assert(kMaxLdsBins / kColBins <= kScatThreads, "k_scatter_lds scans at most kScatThreads groups");
+template <typename T>
+__global__ __launch_bounds__(kScatThreads) void k_scatter_lds(
+    const T* __restrict__ x, const T* __restrict__ y, int M, int chunk, int L1, int L2, T inv1, T inv2, int nb1,
+    int nbins, int maxpts, const int* __restrict__ off, const int* __restrict__ tot, const int2* __restrict__ gexcl,
+    const int2* __restrict__ gsum, int ngroups, PtRec<T>* __restrict__ pts, int* __restrict__ subbeg,
+    int4* __restrict__ sub, int* __restrict__ nsub)
+{
+    __shared__ int base[kMaxLdsBins];
+    __shared__ int2 pre[kScatThreads + 1];
+    __shared__ int w_a[kScatThreads / 64], w_b[kScatThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lo = blockIdx.x * chunk, hi = min(lo + chunk, M);
+    // the first batch of points: loads in flight during the scan
+    T xv[kScatPer], yv[kScatPer];
+#pragma unroll
+    for (int q = 0; q < kScatPer; ++q) {
+        const int i = lo + q * kScatThreads + t;
+        xv[q] = i < hi ? x[i] : (T)0;
+        yv[q] = i < hi ? y[i] : (T)0;
+    }
+    const int* __restrict__ orow = off + (size_t)blockIdx.x * nbins;
+    // exclusive scan of the (points, subproblems) group sums, one per thread
+    const int2 v = t < ngroups ? gsum[t] : make_int2(0, 0);
+    int ia = v.x, ib = v.y;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int na = __shfl_up(ia, o, 64), nb = __shfl_up(ib, o, 64);
+        if (lane >= o) { ia += na; ib += nb; }
+    }
+    if (lane == 63) { w_a[w] = ia; w_b[w] = ib; }
+    __syncthreads();
+    int pa = ia - v.x, pb = ib - v.y;
+    for (int k = 0; k < w; ++k) { pa += w_a[k]; pb += w_b[k]; }
+    pre[t] = make_int2(pa, pb);
+    if (t == kScatThreads - 1) pre[kScatThreads] = make_int2(pa + v.x, pb + v.y);
+    __syncthreads();
+    for (int b = t; b < nbins; b += kScatThreads) base[b] = pre[b / kColBins].x + gexcl[b].x + orow[b];
+    // subproblem lists: an even share of the bins per block
+    const int bpb = (nbins + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int blo = blockIdx.x * bpb, bhi = min(blo + bpb, nbins);
+    for (int bi = blo + t; bi < bhi; bi += kScatThreads) {
+        const int c = tot[bi];
+        const int2 e = gexcl[bi], gb = pre[bi / kColBins];
+        const int pos = gb.x + e.x, sp = gb.y + e.y;
+        subbeg[bi] = sp;
+        const int ns = (c + maxpts - 1) / maxpts;
+        for (int k = 0; k < ns; ++k) sub[sp + k] = make_int4(bi, pos + k * maxpts, min(maxpts, c - k * maxpts), 0);
+    }
+    if (blockIdx.x == 0 && t == 0) {
+        const int total = pre[kScatThreads].y;
+        *nsub = total;
+        subbeg[nbins] = total;
+    }
+    __syncthreads();
+    for (int b0 = lo; b0 < hi; b0 += kScatPer * kScatThreads) {
+        if (b0 != lo) {
+#pragma unroll
+            for (int q = 0; q < kScatPer; ++q) {
+                const int i = b0 + q * kScatThreads + t;
+                xv[q] = i < hi ? x[i] : (T)0;
+                yv[q] = i < hi ? y[i] : (T)0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kScatPer; ++q) {
+            const int i = b0 + q * kScatThreads + t;
+            const bool act = i < hi;
+            int m1, m2;
+            T t1, t2;
+            const int bin = bin_of(xv[q], yv[q], L1, L2, inv1, inv2, nb1, m1, m2, t1, t2);
+            const int p = wave_run_rank(base, bin, act);
+            if (act) {
+                PtRec<T> rec;
+                rec.t1 = t1;
+                rec.t2 = t2;
+                rec.cell = (m2 << 16) | m1;
+                rec.idx = i;
+                pts[p] = rec;
+            }
+        }
+    }
+}
+
 // ================================================================= window taps
 struct WinArgs {
     WinPoly val;    // phi_hat / S
@@ -970,30 +1063,43 @@ __device__ __forceinline__ void gather_body(int s, const cplx<T>* __restrict__ G
 // the same order as in gather_body (plus one FMA with a zero weight):
 // the results are bitwise the same.
 constexpr int kPairs = kBin * kBin / 2;                     // cell pairs of a bin
+// A block takes one subproblem (NSUB = 1) or, with NSUB = 2, also the last
+// subproblem of the same bin when that one holds at most kTailMax points.  A
+// cell pair with n points takes ceil(n / 2) lanes, so at most (points +
+// kPairs) / 2 lanes in all: one lane per thread for up to 384 points.
+constexpr int kTailMax = kGatherThreads / 2;
+template <int NSUB> constexpr int pair_lanes() { return (kGatherThreads + (NSUB - 1) * kTailMax + kPairs) / 2; }
+static_assert(pair_lanes<2>() <= kGatherThreads, "one lane round per block");
+template <int NSUB>
 struct __attribute__((aligned(16))) PairLds {
     float2 tile[kTileW * kTileLd];            // 16-byte aligned: the window reads are 16-byte loads
     int cnt[kPairs];
     int s_warp[kGatherThreads / 64];
-    float t1[2][kGatherThreads], t2[2][kGatherThreads];
-    int cell[2][kGatherThreads], idx[2][kGatherThreads];
+    float t1[2][pair_lanes<NSUB>()], t2[2][pair_lanes<NSUB>()];
+    int cell[2][pair_lanes<NSUB>()], idx[2][pair_lanes<NSUB>()];
     int nlanes;
 };
 
-// Sort the subproblem's points (thread t holds point t in rec when t < d.z)
-// by cell pair into the lane slots of L; returns the number of lanes.  The
-// tile may be stored into L before the call (the first barrier publishes it).
-__device__ __forceinline__ int pairs_stage(PairLds& L, const int4& d, int O1, int O2, const PtRec<float>& rec)
+// Sort the block's points (thread t holds point t of subproblem q in rec[q]
+// when t < n[q]) by cell pair into the lane slots of L; returns the number of
+// lanes.  The tile may be stored into L before the call (the first barrier
+// publishes it).
+template <int NSUB>
+__device__ __forceinline__ int pairs_stage(PairLds<NSUB>& L, const int (&n)[NSUB], int O1, int O2,
+                                           const PtRec<float> (&rec)[NSUB])
 {
     const int tid = threadIdx.x;
-    const bool have = tid < d.z;
-    const int cell = have ? min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin +
-                                min(max((rec.cell & 0xffff) - O1, 0), kBin - 1)
-                          : 0;
-    const int pr = cell >> 1;                                // cell pair: row * 8 + column / 2
+    int cell[NSUB], rk[NSUB];
+#pragma unroll
+    for (int q = 0; q < NSUB; ++q)
+        cell[q] = tid < n[q] ? min(max((rec[q].cell >> 16) - O2, 0), kBin - 1) * kBin +
+                                   min(max((rec[q].cell & 0xffff) - O1, 0), kBin - 1)
+                             : 0;                            // cell pair: cell >> 1 = row * 8 + column / 2
     if (tid < kPairs) L.cnt[tid] = 0;
-    L.idx[1][tid] = -1;
+    for (int l = tid; l < pair_lanes<NSUB>(); l += kGatherThreads) L.idx[1][l] = -1;
     __syncthreads();
-    const int rk = have ? atomicAdd(&L.cnt[pr], 1) : 0;
+#pragma unroll
+    for (int q = 0; q < NSUB; ++q) rk[q] = tid < n[q] ? atomicAdd(&L.cnt[cell[q] >> 1], 1) : 0;
     __syncthreads();
     const int lanes = tid < kPairs ? (L.cnt[tid] + 1) >> 1 : 0;
     const int start = block_exclusive_scan(lanes, L.s_warp);
@@ -1001,12 +1107,15 @@ __device__ __forceinline__ int pairs_stage(PairLds& L, const int4& d, int O1, in
     if (tid < kPairs) L.cnt[tid] = start;
     if (tid == kGatherThreads - 1) L.nlanes = start + lanes;
     __syncthreads();
-    if (have) {
-        const int l = L.cnt[pr] + (rk >> 1), k = rk & 1;
-        L.t1[k][l] = rec.t1;
-        L.t2[k][l] = rec.t2;
-        L.cell[k][l] = cell;
-        L.idx[k][l] = rec.idx;
+#pragma unroll
+    for (int q = 0; q < NSUB; ++q) {
+        if (tid < n[q]) {
+            const int l = L.cnt[cell[q] >> 1] + (rk[q] >> 1), k = rk[q] & 1;
+            L.t1[k][l] = rec[q].t1;
+            L.t2[k][l] = rec[q].t2;
+            L.cell[k][l] = cell[q];
+            L.idx[k][l] = rec[q].idx;
+        }
     }
     __syncthreads();
     return L.nlanes;
@@ -1065,32 +1174,55 @@ __device__ __forceinline__ void pairs_mac(const Lds& L, int l, const WinArgs& wa
     out[two ? ob : oa] = make_float2(xb, yb);
 }
 
-__device__ __forceinline__ void gather_pairs_body(int s, const float2* __restrict__ Gt, const BinGeom& geo,
+// NSUB = 2: a bin's last subproblem with at most kTailMax points is taken by
+// the block of the subproblem before it (which holds more), and its own block
+// returns at once.  The two share one tile and one staging pass instead of
+// paying both for a small tail (polar 512^2 / M = 1M: 4,545 blocks with points
+// instead of 5,456).
+template <int NSUB>
+__device__ __forceinline__ void gather_pairs_body(int s, int ns, const float2* __restrict__ Gt, const BinGeom& geo,
                                                   const PtRec<float>* __restrict__ pts, const int4* __restrict__ sub,
                                                   const WinArgs& wa, float2* __restrict__ out)
 {
-    __shared__ PairLds L;
+    __shared__ PairLds<NSUB> L;
     const int4 d = sub[s];
+    int n[NSUB], at[NSUB];
+    n[0] = d.z;
+    at[0] = d.y;
+    if constexpr (NSUB == 2) {
+        // entries of a bin are consecutive, all but its last hold maxpts > kTailMax points
+        const int4 a = s > 0 ? sub[s - 1] : make_int4(-1, 0, 0, 0);
+        const int4 e = s + 1 < ns ? sub[s + 1] : make_int4(-1, 0, 0, 0);
+        if (d.z <= kTailMax && a.x == d.x && a.z > kTailMax) return;
+        n[1] = e.x == d.x && e.z <= kTailMax && d.z > kTailMax ? e.z : 0;
+        at[1] = e.y;
+    }
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
-    PtRec<float> rec;
-    rec.t1 = rec.t2 = 0.f;
-    rec.cell = rec.idx = 0;
-    if ((int)threadIdx.x < d.z) rec = pts[d.y + threadIdx.x];
+    PtRec<float> rec[NSUB];
+#pragma unroll
+    for (int q = 0; q < NSUB; ++q) {
+        rec[q].t1 = rec[q].t2 = 0.f;
+        rec[q].cell = rec[q].idx = 0;
+        if ((int)threadIdx.x < n[q]) rec[q] = pts[at[q] + threadIdx.x];
+    }
     load_tile(L.tile, Gt, geo.L1, geo.L2, O1 - kK, O2 - kK);
-    const int nl = pairs_stage(L, d, O1, O2, rec);
+    const int nl = pairs_stage<NSUB>(L, n, O1, O2, rec);
     if ((int)threadIdx.x < nl) pairs_mac(L, threadIdx.x, wa, out);
 }
 
 #ifndef NFFT2D_GATHER_WAVES
 #define NFFT2D_GATHER_WAVES 6
 #endif
-// The complex64 type-2 gather of a plan: one block per subproblem.
+// The complex64 type-2 gather of a plan: one block per subproblem, with
+// small tails merged (NSUB = 2) or not (NSUB = 1).
+template <int NSUB>
 __global__ __launch_bounds__(kGatherThreads) __attribute__((amdgpu_waves_per_eu(NFFT2D_GATHER_WAVES, 8))) void
 k_gather_pairs(const float2* __restrict__ Gt, BinGeom geo, const PtRec<float>* __restrict__ pts,
                const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, float2* __restrict__ out)
 {
-    if ((int)blockIdx.x >= *nsub) return;
-    gather_pairs_body(blockIdx.x, Gt, geo, pts, sub, wa, out);
+    const int ns = *nsub;
+    if ((int)blockIdx.x >= ns) return;
+    gather_pairs_body<NSUB>(blockIdx.x, ns, Gt, geo, pts, sub, wa, out);
 }
 
 template <typename T, int MODE>

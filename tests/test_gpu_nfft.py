@@ -242,6 +242,35 @@ def test_clustered_points_and_dense_tiles(dtype):
     assert oracle.rel_l2(F, oracle.ndft_type1(d.astype(np.complex128), x1, x2, N, N)) < tol
 
 
+def test_small_last_subproblems_join_the_one_before():
+    """Bins of 256 q + k points, k on both sides of 128: the complex64 gather lets
+    a bin's last subproblem of at most 128 points share the block (tile and
+    staging) of the full one before it.  Its values must be the same bits as
+    the one-point-per-thread gather behind type2_dx, and match the direct sum."""
+    rng = np.random.default_rng(12)
+    N = 64                                       # 128^2 grid, 8 x 8 bins of 16 x 16 cells
+    extra = [0, 1, 2, 64, 127, 128, 129, 200, 255, 256]
+    xs, ys = [], []
+    for b in range(64):
+        n = 256 * (1 + b % 3) + extra[b % len(extra)]
+        mu1 = 16 * (b % 8) + rng.integers(0, 16, n)
+        mu2 = 16 * (b // 8) + rng.integers(0, 16, n)
+        # cell mu (mod 128) holds the x with rint(2 x) = mu - 128 [mu >= 64]
+        xs.append((np.where(mu1 >= 64, mu1 - 128, mu1) + rng.uniform(-0.45, 0.45, n)) / 2)
+        ys.append((np.where(mu2 >= 64, mu2 - 128, mu2) + rng.uniform(-0.45, 0.45, n)) / 2)
+    x1 = np.concatenate(xs).astype(np.float32)
+    x2 = np.concatenate(ys).astype(np.float32)
+    perm = rng.permutation(len(x1))
+    x1, x2 = x1[perm], x2[perm]
+    f = cn01(rng, (N, N))
+    plan = m.Plan(N, N, device=DEV).setpts(T(x1), T(x2))
+    y = plan.type2(T(f))
+    y1, _, _ = plan.type2_dx(T(f))
+    assert torch.equal(y, y1)
+    sel = rng.choice(len(x1), 2000, replace=False)
+    assert oracle.rel_l2(y.cpu().numpy()[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel])) < TOL32
+
+
 @pytest.mark.parametrize("N1,N2", [(8, 8), (16, 64), (256, 32), (2048, 16), (4096, 8), (6, 10), (88, 88)])
 def test_shapes(N1, N2):
     rng = np.random.default_rng(N1 * 7 + N2)
