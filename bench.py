@@ -243,7 +243,7 @@ def batch_points(rng, N, M):
 
 
 def bench_batch(args, dev):
-    """256 type-2 NFFTs of uneven sizes: one graph replay of the batched setpts + type-2 (8 launches),
+    """256 type-2 NFFTs of uneven sizes: one graph replay of the batched setpts + type-2 (9 launches),
     against the same transforms run one plan at a time (also graph-replayed)."""
     import miscellaneous_amd as m
 
@@ -303,7 +303,7 @@ def bench_batch(args, dev):
     del plans, g2
     return {"config": "256 type-2 NFFTs, N uniform in {64,128,256,512} (square), M log-uniform in [4096, 262144], "
                       "half uniform / half Gaussian (sigma 0.05, wrapped) points, complex64, K=6, c=2; "
-                      "step = batched setpts + type-2 (8 launches, the tile sort beside the grid passes) replayed from one graph",
+                      "step = batched setpts + type-2 (9 launches, the tile sort beside the grid passes) replayed from one graph",
             "transforms": len(sizes), "total_points": total_m, "value": total_m / s_batch / 1e6, "unit": "Mpts/s",
             "ms_per_step": s_batch * 1e3, "one_stream_ms_per_step": s_serial * 1e3, "plain_ms_per_step": s_plain * 1e3, "speedup_vs_plain": s_plain / s_batch,
             "identical_to_plain": same, "rel_l2_max_16x256": worst,

@@ -218,7 +218,7 @@ int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c
 
 /* Many complex64 type-2 transforms of different sizes (K = 6, c = 2; N1, N2
  * powers of two in [8, 1024]; any M), run as ONE kernel launch per stage over
- * all of them: nfft2d_batch_setpts is 5 launches and nfft2d_batch_type2 is 3,
+ * all of them: nfft2d_batch_setpts is 5 launches and nfft2d_batch_type2 is 4,
  * whatever the number of transforms.  Neither allocates nor records a
  * hipMemset, so both can be captured into one hipGraph and replayed.  There is
  * no reference counterpart: the reference runs one NFFT2_2D_GPU call per

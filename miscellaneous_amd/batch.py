@@ -3,7 +3,7 @@
 ``Batch(sizes, device)`` with ``sizes = [(N1, N2, M), ...]`` builds one plan
 per transform (complex64, K = 6, c = 2; N1, N2 powers of two in [8, 1024]).
 Every stage then runs as one kernel launch over all transforms:
-``setpts()`` is 5 launches and ``type2()`` 3, so both can be captured into
+``setpts()`` is 5 launches and ``type2()`` 4, so both can be captured into
 one graph.  Arrays are kept by reference (``set_points`` / ``set_io``): keep
 the tensors alive while the batch uses them.
 """

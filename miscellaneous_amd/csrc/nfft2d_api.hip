@@ -1169,6 +1169,9 @@ static int batch_gather(nfft2d_batch *b, hipStream_t s)
     hipLaunchKernelGGL(kb_gather_w, dim3(kBatchGatherWaveBlocks), dim3(256), 0, s, b->ddesc, b->bsub,
                        b->base + b->count, b->plans[0]->wa);
     NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_gather_direct, dim3((unsigned)((b->bsubcap + 255 / kDirectMax) / (256 / kDirectMax))),
+                       dim3(256), 0, s, b->ddesc, b->bsub, b->base + b->count, b->plans[0]->wa);
+    NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }
 

@@ -1174,6 +1174,56 @@ __device__ __forceinline__ void pairs_mac(const Lds& L, int l, const WinArgs& wa
     out[two ? ob : oa] = make_float2(xb, yb);
 }
 
+// One point of a subproblem of the bin with origin (O1, O2), its 13 x 14
+// window read from the grid instead of an LDS tile: the fmaf sequence of
+// pairs_mac for a point (columns c0..c0+13 with the weights shifted for an odd
+// cell, then the 13 rows), so the same bits.  For subproblems of a few points,
+// where a tile, a sort and a wave's sums per subproblem cost more than the
+// 182 cached reads per point.
+__device__ __forceinline__ float2 direct_point(const PtRec<float>& rec, int O1, int O2, const float2* __restrict__ Gt,
+                                               const BinGeom& geo, const WinArgs& wa)
+{
+    const int cell = min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin + min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
+    const int r0 = cell / kBin, c0 = (cell % kBin) & ~1;
+    const bool odd = (cell & 1) != 0;
+    float w1[kW], w2[kW];
+    taps(rec.t1, wa, w1);
+    taps(rec.t2, wa, w2);
+    float e[kW + 1];
+#pragma unroll
+    for (int j = 0; j <= kW; ++j) {
+        const float a0 = j < kW ? w1[j] : 0.f, a1 = j > 0 ? w1[j - 1] : 0.f;
+        e[j] = odd ? a1 : a0;
+    }
+    const bool big = geo.L1 >= kTileW && geo.L2 >= kTileW;
+    int g1[kW + 1];
+#pragma unroll
+    for (int j = 0; j <= kW; ++j) {
+        const int o1 = O1 - kK + c0 + j;
+        g1[j] = big ? wrap1(o1, geo.L1) : wrap(o1, geo.L1);
+    }
+    // one row (14 loads) in flight at a time; the row weights shift down so
+    // that the current one is w2[0] (register arrays take static indices only)
+    float x = 0.f, y = 0.f;
+#pragma unroll 1
+    for (int b = 0; b < kW; ++b) {
+        const int o2 = O2 - kK + r0 + b;
+        const float2* __restrict__ row = Gt + (size_t)(big ? wrap1(o2, geo.L2) : wrap(o2, geo.L2)) * geo.L1;
+        float sx = 0.f, sy = 0.f;
+#pragma unroll
+        for (int j = 0; j <= kW; ++j) {
+            const float2 u = row[g1[j]];
+            sx = fmaf(e[j], u.x, sx);
+            sy = fmaf(e[j], u.y, sy);
+        }
+        x = fmaf(w2[0], sx, x);
+        y = fmaf(w2[0], sy, y);
+#pragma unroll
+        for (int m = 0; m + 1 < kW; ++m) w2[m] = w2[m + 1];
+    }
+    return make_float2(x, y);
+}
+
 // NSUB = 2: a bin's last subproblem with at most kTailMax points is taken by
 // the block of the subproblem before it (which holds more), and its own block
 // returns at once.  The two share one tile and one staging pass instead of
