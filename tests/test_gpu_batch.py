@@ -172,3 +172,48 @@ def test_batch_autotune_keeps_results():
     torch.cuda.synchronize()
     for t in range(len(SIZES)):
         assert torch.equal(to[t], before[t]), t
+
+
+def points_per_bin(rng, N, counts):
+    """Points of an N x N transform (grid 2N, bins of 16 x 16 cells) with counts[b]
+    points in bin b, in shuffled order."""
+    L, nb = 2 * N, (2 * N) // 16
+    xs, ys = [], []
+    for b, n in enumerate(counts):
+        mu1 = 16 * (b % nb) + rng.integers(0, 16, n)
+        mu2 = 16 * (b // nb) + rng.integers(0, 16, n)
+        # cell mu (mod L) holds the x with rint(2 x) = mu - L [mu >= L / 2]
+        xs.append((np.where(mu1 >= L // 2, mu1 - L, mu1) + rng.uniform(-0.45, 0.45, n)) / 2)
+        ys.append((np.where(mu2 >= L // 2, mu2 - L, mu2) + rng.uniform(-0.45, 0.45, n)) / 2)
+    x1 = np.concatenate(xs).astype(np.float32)
+    x2 = np.concatenate(ys).astype(np.float32)
+    perm = rng.permutation(len(x1))
+    return x1[perm], x2[perm]
+
+
+def test_batch_small_subproblems_lane_per_point():
+    """Bins of 0..12 points on both sides of the batch's one-lane-per-point limit
+    (subproblems of at most 8 points skip the waves' tile and sort): every
+    transform bitwise equal to its own plan, and close to the direct sum."""
+    rng = np.random.default_rng(21)
+    sizes, fs, xs, ys = [], [], [], []
+    for N, shift in ((64, 0), (32, 5), (128, 3)):
+        nbins = ((2 * N) // 16) ** 2
+        counts = [(b + shift) % 13 for b in range(nbins)]
+        x1, x2 = points_per_bin(rng, N, counts)
+        sizes.append((N, N, len(x1)))
+        fs.append(cn01(rng, (N, N)))
+        xs.append(x1)
+        ys.append(x2)
+    b = m.Batch(sizes, device=DEV)
+    to = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in sizes]
+    b.set_points([T(x) for x in xs], [T(y) for y in ys]).set_io([T(f) for f in fs], to)
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    for t, (N1, N2, M) in enumerate(sizes):
+        yb = to[t].cpu().numpy()
+        assert np.array_equal(yb, per_plan(fs[t], xs[t], ys[t])), f"transform {t} {sizes[t]}"
+        sel = np.sort(rng.choice(M, min(M, 500), replace=False))
+        ref = oracle.ndft_type2(fs[t].astype(np.complex128), xs[t][sel], ys[t][sel])
+        assert oracle.rel_l2(yb[sel], ref) <= 1e-5, t
