@@ -1069,7 +1069,7 @@ constexpr int kPairs = kBin * kBin / 2;                     // cell pairs of a b
 // kPairs) / 2 lanes in all: one lane per thread for up to 384 points.
 constexpr int kTailMax = kGatherThreads / 2;
 template <int NSUB> constexpr int pair_lanes() { return (kGatherThreads + (NSUB - 1) * kTailMax + kPairs) / 2; }
-static_assert(pair_lanes<2>() <= kGatherThreads, "one lane round per block");
+static_assert(pair_lanes<2>() <= kGatherThreads, "at most one lane slot per thread");
 template <int NSUB>
 struct __attribute__((aligned(16))) PairLds {
     float2 tile[kTileW * kTileLd];            // 16-byte aligned: the window reads are 16-byte loads
