@@ -92,11 +92,11 @@ void Calculate_cuFFT_plan_C2C_NFFT2_2D(int N1, int N2);
 void Calculate_cuFFT_plan_Z2Z_NFFT2_2D(int N1, int N2);
 void Destroy_cuFFT_plan_NFFT2_2D(void);
 
-/* Direct NED sum, NFFT/NDFT2_2D.cu:50 (decl. NFFT/NDFT2_2D.cuh:4-6):
+/* Direct NED sum, NFFT/NDFT2_2D.cu:37 (decl. NFFT/NDFT2_2D.cuh:4-6):
  *   out[n] = sum_{m<M} in[m] e^{-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)},  n < N,
  * in FP64.  `handle` took a cublasHandle_t; it is accepted and ignored.  The
  * reference materialises the M x N kernel matrix and writes it to a text file
- * (:57,69); neither happens here. */
+ * (:44,56) before its gemv (:67); neither happens here. */
 int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, const double *u, const double *v,
                  const nfft2d_c128 *in, nfft2d_c128 *out, int Nu, int Nv, int M, int N);
 

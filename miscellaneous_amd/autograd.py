@@ -7,7 +7,9 @@ backward:  grad_f = A^H g                       (nfft2d_adjoint, the adjoint NFF
 with g the incoming gradient in PyTorch's convention for complex outputs.
 The backward uses the oversampled grid the forward left in the plan when
 nothing has rewritten it since (the usual forward-then-backward order), and
-recomputes it from f otherwise; it never keeps a copy of its own.
+recomputes it from f otherwise; it never keeps a copy of its own.  The
+forward sorts the points only when they are not the ones the plan already
+holds (Plan.setpts_if_changed).
 
 The reference never differentiates its NFFT (its only optimisers are
 derivative-free: fminsearch and PSO, e.g. Synthesis/Aper_Synth_1D/Aper_Synth_1D.m:161,
@@ -26,7 +28,9 @@ class NFFT2DType2(torch.autograd.Function):
         # lazy conjugate / negative views are resolved: the library reads memory
         f = f.resolve_conj().resolve_neg().contiguous()
         x1c, x2c = x1.detach().resolve_neg().contiguous(), x2.detach().resolve_neg().contiguous()
-        plan.setpts(x1c, x2c)
+        # the tile sort is skipped when the plan already holds these coordinates (same tensors,
+        # unmodified): repeated forwards at fixed points cost the grid passes and the gather only
+        plan.setpts_if_changed(x1, x2)
         y = plan.type2(f)
         ctx.plan = plan
         ctx.version = plan.pts_version

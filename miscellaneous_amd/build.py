@@ -46,5 +46,29 @@ def build_library(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+ROOT = os.path.dirname(PKG_DIR)
+REFCALLS_SRC = os.path.join(ROOT, "tests", "cpp", "reference_calls.cpp")
+REFCALLS_BIN = os.path.join(ROOT, "tests", "cpp", "reference_calls")
+
+
+def build_reference_calls(verbose: bool = False) -> str:
+    """Compile tests/cpp/reference_calls.cpp (the reference's call syntax through
+    include/nfft2d_overloads.hpp) against libnfft2d.so, run-path set to the package."""
+    tmp = REFCALLS_BIN + f".tmp{os.getpid()}"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           "-Wno-unused-command-line-argument", "-o", tmp, REFCALLS_SRC, "-L", PKG_DIR, "-lnfft2d",
+           "-Wl,-rpath," + PKG_DIR, "-Wl,-rpath,$ORIGIN/../../miscellaneous_amd"]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise RuntimeError("hipcc failed on reference_calls.cpp:\n" + res.stdout[-8000:])
+    os.replace(tmp, REFCALLS_BIN)
+    return REFCALLS_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_reference_calls(verbose=True))

@@ -1,8 +1,8 @@
 // ndft2d.hip -- the 2D NDFT.  Part 1: NDFT2_2D_GPU, the reference's direct
-// sum in FP64 (NFFT/NDFT2_2D.cu:50-85):
+// sum in FP64 (NFFT/NDFT2_2D.cu:15-32,37-72):
 //   out[n] = sum_{m<M} in[m] exp(-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)).
-// The reference fills an explicit M x N FP64 kernel matrix (:57-63; 64 GiB at
-// M = N = 65536), writes it to a text file (:69) and calls cublasZgemv (:80).
+// The reference fills an explicit M x N FP64 kernel matrix (:44-50; 64 GiB at
+// M = N = 65536), writes it to a text file (:56) and calls cublasZgemv (:67).
 // Here the kernel is evaluated on the fly: each thread owns one output n and
 // streams the points through LDS in chunks, accumulating in FP64.
 #include <hip/hip_runtime.h>

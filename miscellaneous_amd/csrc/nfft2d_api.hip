@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <map>
 #include <mutex>
 #include <string>
@@ -312,10 +313,13 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
         return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: N1, N2 must be even and in [2, %d] (got %d, %d)", maxN, N1, N2);
     if ((!fast_path(N1, dtype) && N1 > 2048) || (!fast_path(N2, dtype) && N2 > 2048))
         return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: non-power-of-two N above 2048 is not supported");
-    nfft2d_plan *p = new nfft2d_plan_s();
+    int device = 0;
+    NFFT2D_HIP_TRY(hipGetDevice(&device));      // before the allocation: nothing to free on failure
+    nfft2d_plan *p = new (std::nothrow) nfft2d_plan_s();
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: host allocation failed");
     p->N1 = N1; p->N2 = N2; p->L1 = kC * N1; p->L2 = kC * N2; p->dtype = dtype;
     p->esz = dtype == NFFT2D_C64 ? sizeof(float2) : sizeof(double2);
-    NFFT2D_HIP_TRY(hipGetDevice(&p->device));
+    p->device = device;
     p->nb1 = (p->L1 + kBin - 1) / kBin;
     p->nb2 = (p->L2 + kBin - 1) / kBin;
     p->nbins = p->nb1 * p->nb2;
@@ -667,8 +671,12 @@ static int spread_impl(nfft2d_plan *p, const void *data, bool sorted, hipStream_
                                    p->nsub, p->subbeg, p->tot, p->maxpts, p->geo, p->wa, (double2 *)p->scratch);
             NFFT2D_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_spread_merge<T>, dim3(p->nbins), dim3(256), 0, s, (const C *)p->scratch, p->subbeg,
-                           p->geo, p->nb2, (C *)p->grid);
+        // a dimension whose last bin is thinner than K needs the +-2 neighbours
+        const int r1 = merge_radius(p->L1), r2 = merge_radius(p->L2);
+        auto merge = r1 == 1 ? (r2 == 1 ? k_spread_merge<T, 1, 1> : k_spread_merge<T, 1, 2>)
+                             : (r2 == 1 ? k_spread_merge<T, 2, 1> : k_spread_merge<T, 2, 2>);
+        hipLaunchKernelGGL(merge, dim3(p->nbins), dim3(256), 0, s, (const C *)p->scratch, p->subbeg, p->geo, p->nb2,
+                           (C *)p->grid);
         NFFT2D_LAUNCH_CHECK();
         return NFFT2D_OK;
     }
@@ -967,6 +975,7 @@ struct nfft2d_batch_s {
     long long bsubcap = 0;
     int max_nbins = 0;
     bool have_points = false, have_io = false;
+    bool sorted = false;                  // the subproblem list matches the points and sizes (batch_setpts ran)
     Fork fk;                              // setpts beside the grid passes (nfft2d_batch_setpts_type2)
 };
 
@@ -1073,7 +1082,7 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_create: device allocation failed");
     }
     if (hipMemcpy(b->own, own.data(), sizeof(int2) * own.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        batch_upload(b) != NFFT2D_OK) {
+        hipMemset(b->base, 0, sizeof(int) * (count + 1)) != hipSuccess || batch_upload(b) != NFFT2D_OK) {
         batch_free(b);
         return nfft2d_fail(NFFT2D_ERR_HIP, "batch_create: upload of the batch tables failed");
     }
@@ -1100,6 +1109,7 @@ extern "C" int nfft2d_batch_set_points(nfft2d_batch *b, const float *const *x, c
         b->hdesc[t].y = y[t];
     }
     b->have_points = true;
+    b->sorted = false;
     return batch_upload(b);
 }
 
@@ -1124,6 +1134,7 @@ extern "C" int nfft2d_batch_set_maxpts(nfft2d_batch *b, int t, int maxpts)
                            kGatherThreads);
     b->plans[t]->maxpts = maxpts;
     b->hdesc[t].maxpts = maxpts;
+    b->sorted = false;                    // the subproblem list must be rebuilt
     return batch_upload(b);
 }
 
@@ -1152,6 +1163,7 @@ extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_scatter, dim3(b->n_scat), dim3(256), 0, s, b->ddesc, b->own + b->off_scat, b->base, b->bsub);
     NFFT2D_LAUNCH_CHECK();
+    b->sorted = true;
     return NFFT2D_OK;
 }
 
@@ -1179,6 +1191,9 @@ extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
 {
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type2: null batch");
     if (!b->have_io) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: nfft2d_batch_set_io was not called");
+    if (!b->sorted)
+        return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type2: the points are not sorted (call nfft2d_batch_setpts after "
+                                             "nfft2d_batch_set_points / nfft2d_batch_set_maxpts)");
     int rc = nfft2d_check_device(b->device);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;

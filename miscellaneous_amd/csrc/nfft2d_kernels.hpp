@@ -1756,30 +1756,42 @@ __global__ __launch_bounds__(256) void k_spread_halo64(const PtRec<double>* __re
     }
 }
 
-// Grid cell (n2, n1) = sum over the neighbouring bins b' (3x3, fixed order) and
+// Grid cell (n2, n1) = sum over the neighbouring bins b' (fixed order) and
 // their tiles (every kSpUnit-th subproblem, in order) of
 // tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1] where that index lies
 // in [0, 28).  One block per bin writes the bin's cells.
-template <typename T>
+// The neighbours are the bins at offsets -R..R per dimension.  R = 1 covers
+// every tile that reaches a bin when each bin is at least K cells wide; when
+// the last bin of a dimension is thinner than K (L mod 16 = 2 or 4), the
+// tiles of the bins on either side of it reach two bins across the wrap, and
+// that dimension needs R = 2 (merge_radius).  With 2R + 1 > nb the offsets
+// are 0..nb-1 instead, so that no bin is visited twice.
+__host__ __device__ constexpr int merge_radius(int L) { return (L % kBin != 0 && L % kBin < kK) ? 2 : 1; }
+
+template <typename T, int R1, int R2>
 __global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict__ scratch,
                                                       const int* __restrict__ subbeg, BinGeom geo, int nb2,
                                                       cplx<T>* __restrict__ Gt)
 {
     using C = cplx<T>;
+    constexpr int W1 = 2 * R1 + 1, W2 = 2 * R2 + 1, NN = W1 * W2;
     const int nb1 = geo.nb1;
     const int bin = blockIdx.x;
     const int b1 = bin % nb1, b2 = bin / nb1;
     const int n2 = b2 * kBin + threadIdx.x / kBin, n1 = b1 * kBin + threadIdx.x % kBin;
     if (n2 >= geo.L2 || n1 >= geo.L1) return;
-    int beg[9], end[9], off[9];
+    const int s1 = W1 > nb1 ? 0 : b1 - R1 + nb1, c1 = W1 > nb1 ? nb1 : W1;   // first offset, count
+    const int s2 = W2 > nb2 ? 0 : b2 - R2 + nb2, c2 = W2 > nb2 ? nb2 : W2;
+    int beg[NN], end[NN], off[NN];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        const int y2 = (b2 + j / 3 - 1 + nb2) % nb2, x1 = (b1 + j % 3 - 1 + nb1) % nb1;
+    for (int j = 0; j < NN; ++j) {
+        const int j2 = j / W1, j1 = j % W1;
+        const int y2 = (s2 + j2) % nb2, x1 = (s1 + j1) % nb1;
         int r = n2 - (y2 * kBin - kK), c = n1 - (x1 * kBin - kK);
         r = r < 0 ? r + geo.L2 : (r >= geo.L2 ? r - geo.L2 : r);
         c = c < 0 ? c + geo.L1 : (c >= geo.L1 ? c - geo.L1 : c);
         const int nb = y2 * nb1 + x1;
-        const bool in = r < kHalo && c < kHalo;
+        const bool in = j1 < c1 && j2 < c2 && r < kHalo && c < kHalo;
         beg[j] = in ? subbeg[nb] : 0;
         end[j] = in ? subbeg[nb + 1] : 0;
         off[j] = r * kHalo + c;
@@ -1789,7 +1801,7 @@ __global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict_
     // latency once per tile
     T sr = 0, si = 0;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
+    for (int j = 0; j < NN; ++j) {
         int s = beg[j];
         for (; s + 7 * kSpUnit < end[j]; s += 8 * kSpUnit) {
             C v[8];

@@ -99,12 +99,28 @@ def _window_tensor(wn: tuple, wm: tuple, scale: float, dtype, device: torch.devi
     return g
 
 
+_MAX_PLANS = 8
+
+
 def _plan(n1: int, n2: int, dtype, device: torch.device) -> Plan:
+    """The type-2 plan of one (n1, n2, dtype, device), from a small least-recently-used cache: a
+    plan holds an oversampled grid of up to 8192^2 x 8 B, so at most _MAX_PLANS are kept and an
+    evicted plan is closed (its device memory freed) at once."""
     key = (n1, n2, dtype, device)
-    p = _plans.get(key)
+    p = _plans.pop(key, None)
     if p is None:
-        p = _plans[key] = Plan(n1, n2, dtype, device=device)
+        while len(_plans) >= _MAX_PLANS:
+            _plans.pop(next(iter(_plans))).close()
+        p = Plan(n1, n2, dtype, device=device)
+    _plans[key] = p                    # most recently used last
     return p
+
+
+def clear_plan_cache() -> None:
+    """Close every cached interpolation plan and window (frees their device memory)."""
+    while _plans:
+        _plans.pop(next(iter(_plans))).close()
+    _windows.clear()
 
 
 def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT_DATA: torch.Tensor,

@@ -11,7 +11,7 @@ algorithm:
 * ``nfft1_2d``  -- type-2 (uniform -> non-uniform), ``BP_FBP_FFBP/NFFT1_2D.m``.
 * ``nfft2_2d``  -- type-1 (non-uniform -> uniform), ``NFFT/NFFT2_2D.cu:329-495``
   and its MATLAB twin ``NFFT/NFFT2/Matlab/NFFT2_2D.m``.
-* ``ndft2_2d``  -- the direct NED sum, ``NFFT/NDFT2_2D.cu:28-45,80``.
+* ``ndft2_2d``  -- the direct NED sum, ``NFFT/NDFT2_2D.cu:28-29,67``.
 * ``ndft_type2`` / ``ndft_type1`` -- direct FP64 sums of both directions (the
   closed forms every fast result is checked against).
 * ``window``, ``window_deriv``, ``bessel_i0``, ``spectral_window``, ``fft2`` --

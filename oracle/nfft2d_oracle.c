@@ -25,7 +25,7 @@
  *                        fftshift -> forward FFT -> fftshift :383-393,
  *                        Decimation_and_Scaling :74-100.  Also the MATLAB twin
  *                        NFFT/NFFT2/Matlab/NFFT2_2D.m.
- *   orc_ndft2_2d         direct NED sum of NFFT/NDFT2_2D.cu:28-45,80
+ *   orc_ndft2_2d         direct NED sum of NFFT/NDFT2_2D.cu:28-29,67
  *                        (Kernel_Matrix_Filling + cublasZgemv OP_T).
  *   orc_ndft_type2(_dx)  direct type-2 sum y_l = sum_k f_k e^{-2 pi i k.x_l/N}
  *                        (sign pinned by NFFT/NFFT1/1D/Matlab/NFFT1_1D_BLAS.m:5-16)
@@ -178,7 +178,7 @@ static void cis_neg(double t, double *re, double *im)
     *im = -sin(a);
 }
 
-/* NFFT/NDFT2_2D.cu:28-45,80: out[n] = sum_m in[m] exp(-2 pi i (u_n X_m / Nu + v_n Y_m / Nv)). */
+/* NFFT/NDFT2_2D.cu:28-29,67: out[n] = sum_m in[m] exp(-2 pi i (u_n X_m / Nu + v_n Y_m / Nv)). */
 void orc_ndft2_2d(const double *X, const double *Y, const double *u, const double *v,
                   const double *in, double *out, int Nu, int Nv, int M, int N)
 {

@@ -217,3 +217,85 @@ def test_batch_small_subproblems_lane_per_point():
         sel = np.sort(rng.choice(M, min(M, 500), replace=False))
         ref = oracle.ndft_type2(fs[t].astype(np.complex128), xs[t][sel], ys[t][sel])
         assert oracle.rel_l2(yb[sel], ref) <= 1e-5, t
+
+
+def test_type2_refused_before_setpts():
+    """The batch's subproblem list exists only after setpts: type2 before it (and after a change of
+    subproblem size or of the point arrays) is refused on the host, before any launch."""
+    rng = np.random.default_rng(21)
+    sizes = [(64, 64, 3000), (32, 32, 500)]
+    b = m.Batch(sizes, device=DEV)
+    xs, ys, fs, outs = [], [], [], []
+    for (N1, N2, M) in sizes:
+        x1, x2 = batch_points(rng, N1, N2, M)
+        xs.append(T(x1))
+        ys.append(T(x2))
+        fs.append(T(cn01(rng, (N1, N2))))
+        outs.append(torch.empty(M, dtype=torch.complex64, device=DEV))
+    b.set_points(xs, ys).set_io(fs, outs)
+    with pytest.raises(m.NFFT2DError, match="not sorted"):
+        b.type2()
+    b.setpts()
+    b.type2()
+    b.set_maxpts(0, 128)
+    with pytest.raises(m.NFFT2DError, match="not sorted"):
+        b.type2()
+    b.setpts()
+    b.type2()
+    b.set_points(ys, xs)
+    with pytest.raises(m.NFFT2DError, match="not sorted"):
+        b.type2()
+    torch.cuda.synchronize()
+
+
+def c3_sizes(seed: int, count: int = 256):
+    """The uneven-batch workload: N uniform in {64, 128, 256, 512} (square), M log-uniform in
+    [4096, 262144] (about 16M points in all)."""
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice([64, 128, 256, 512], size=count)
+    Ms = np.round(np.exp(rng.uniform(np.log(4096), np.log(262144), size=count))).astype(int)
+    return [(int(n), int(n), int(mm)) for n, mm in zip(Ns, Ms)], rng
+
+
+def test_c3_workload_graph_replay():
+    """The 256-transform uneven batch at full size: setpts + type-2 replayed from one graph, with
+    new coordinates written into the captured inputs between replays.  16 transforms bitwise equal
+    to their own plans, and 16 x 256 seeded outputs within 1e-5 of FP64 direct sums."""
+    sizes, rng = c3_sizes(20260)
+    total = sum(s[2] for s in sizes)
+    print(f"case C3 batch: {len(sizes)} transforms, {total} points", flush=True)
+    fs, xs, ys = [], [], []
+    for (N1, N2, M) in sizes:
+        fs.append(T(cn01(rng, (N1, N2))))
+        x1, x2 = batch_points(rng, N1, N2, M)
+        xs.append(T(x1))
+        ys.append(T(x2))
+    outs = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in sizes]
+    b = m.Batch(sizes, device=DEV).set_points(xs, ys).set_io(fs, outs)
+    b.setpts_type2()                               # creates the batch's second stream outside the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.setpts_type2()
+    torch.cuda.synchronize()
+    check = rng.choice(len(sizes), 16, replace=False)
+    for rep in range(2):
+        if rep == 1:                               # new points into the captured inputs
+            for t, (N1, N2, M) in enumerate(sizes):
+                x1, x2 = batch_points(rng, N1, N2, M)
+                xs[t].copy_(T(x1))
+                ys[t].copy_(T(x2))
+        g.replay()
+        torch.cuda.synchronize()
+        worst = 0.0
+        for t in check:
+            N1, N2, M = sizes[t]
+            plan = m.Plan(N1, N2, device=DEV).setpts(xs[t], ys[t])
+            assert torch.equal(plan.type2(fs[t]), outs[t]), f"transform {t} differs from its plan"
+            sel = rng.choice(M, 256, replace=False)
+            x1 = xs[t].cpu().numpy()[sel]
+            x2 = ys[t].cpu().numpy()[sel]
+            ref = oracle.ndft_type2(fs[t].cpu().numpy().astype(np.complex128), x1, x2)
+            worst = max(worst, oracle.rel_l2(outs[t].cpu().numpy()[sel], ref))
+        print(f"C3 replay {rep}: worst rel-L2 over 16 x 256 outputs {worst:.2e}", flush=True)
+        assert worst < 1e-5

@@ -288,6 +288,34 @@ def test_shapes(N1, N2):
     assert oracle.rel_l2(F[i1, i2], oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2)) < TOL32
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("N1,N2", [(26, 34), (34, 26), (42, 64), (50, 50), (88, 90)])
+def test_spread_thin_last_bin(N1, N2, dtype):
+    """Grids whose last 16-cell bin is 4 cells wide (2 N = 4 mod 16, i.e. N = 2 mod 8): the
+    spread tiles of the bins beside it reach two bins across the wrap, so the tile merge visits
+    the bins at offsets -2..2 in that dimension.  Type-1 and adjoint against FP64 direct sums,
+    with points crowded near the wrap and spread over the whole period."""
+    rng = np.random.default_rng(N1 * 100 + N2)
+    M = 3000
+    x1 = np.concatenate([rng.uniform(-N1 / 2, N1 / 2, M // 2), N1 / 2 - rng.uniform(0, 3, M - M // 2)])
+    x2 = np.concatenate([rng.uniform(-N2 / 2, N2 / 2, M // 2), -N2 / 2 + rng.uniform(0, 3, M - M // 2)])
+    x1 = x1.astype(np.float32)
+    x2 = x2.astype(np.float32)
+    wide = dtype == torch.complex128
+    rt = torch.float64 if wide else torch.float32
+    tol = TOL64 if wide else TOL32
+    d = cn01(rng, M, np.complex128 if wide else np.complex64)
+    print(f"case thin last bin {N1}x{N2} {dtype}", flush=True)
+    plan = m.Plan(N1, N2, dtype, device=DEV).setpts(T(x1).to(rt), T(x2).to(rt))
+    F = plan.type1(T(d)).cpu().numpy()
+    Fa = plan.adjoint(T(d)).cpu().numpy()
+    ref = oracle.ndft_type1(d.astype(np.complex128), x1, x2, N1, N2)
+    ref_a = np.conj(oracle.ndft_type1(np.conj(d.astype(np.complex128)), x1, x2, N1, N2))
+    e1, ea = oracle.rel_l2(F, ref), oracle.rel_l2(Fa, ref_a)
+    print(f"thin last bin {N1}x{N2}: type-1 {e1:.2e}, adjoint {ea:.2e}")
+    assert e1 < tol and ea < tol
+
+
 @pytest.mark.parametrize("N1,N2,M", [(2048, 1024, 300000), (2048, 2048, 200000)])
 def test_large_grids_sort_paths(N1, N2, M):
     """Oversampled grids above 16,384 bins: the point histogram goes through
@@ -549,4 +577,34 @@ def test_autograd_gradients_vs_fp64_and_finite_differences():
     xs2 = x2[sel].astype(np.float64)
     fd1 = (np.abs(oracle.ndft_type2(f0.astype(np.complex128), xs1 + h, xs2) - y_obs[sel]) ** 2 -
            np.abs(oracle.ndft_type2(f0.astype(np.complex128), xs1 - h, xs2) - y_obs[sel]) ** 2) / (2 * h)
-    assert oracle.rel_l2(gx1, fd1) < 1e-3
+    fd2 = (np.abs(oracle.ndft_type2(f0.astype(np.complex128), xs1, xs2 + h) - y_obs[sel]) ** 2 -
+           np.abs(oracle.ndft_type2(f0.astype(np.complex128), xs1, xs2 - h) - y_obs[sel]) ** 2) / (2 * h)
+    e1, e2 = oracle.rel_l2(gx1, fd1), oracle.rel_l2(gx2, fd2)
+    print(f"grad_x vs central differences: d/dx1 {e1:.2e}, d/dx2 {e2:.2e}")
+    assert e1 < 1e-3 and e2 < 1e-3
+
+
+def test_autograd_skips_the_sort_for_unchanged_points():
+    """Repeated forwards at the same coordinate tensors sort once; an in-place update of the
+    coordinates (an optimiser step) is seen through the tensors' version counters and re-sorts.
+    Results equal a fresh plan's in both cases."""
+    rng = np.random.default_rng(23)
+    N, M = 64, 5000
+    x1 = T(((rng.random(M) - 0.5) * N).astype(np.float32)).requires_grad_(True)
+    x2 = T(((rng.random(M) - 0.5) * N).astype(np.float32)).requires_grad_(True)
+    f = T(cn01(rng, (N, N))).requires_grad_(True)
+    plan = m.Plan(N, N, device=DEV)
+    y0 = m.nfft2d_type2(f, x1, x2, plan)
+    v0 = plan.pts_version
+    y1 = m.nfft2d_type2(f, x1, x2, plan)
+    assert plan.pts_version == v0 and torch.equal(y0, y1)
+    (y1.abs() ** 2).sum().backward()
+    with torch.no_grad():
+        x1.add_(0.25)                              # in place: the version counter moves
+    y2 = m.nfft2d_type2(f, x1, x2, plan)
+    assert plan.pts_version == v0 + 1
+    fresh = m.Plan(N, N, device=DEV).setpts(x1.detach().contiguous(), x2.detach().contiguous())
+    assert torch.equal(y2, fresh.type2(f.detach()))
+    plan.setpts(x2.detach().contiguous(), x1.detach().contiguous())   # re-pointed elsewhere
+    y3 = m.nfft2d_type2(f, x1, x2, plan)
+    assert torch.equal(y3, y2)

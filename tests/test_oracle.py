@@ -137,6 +137,38 @@ def test_direct_sums_vs_golden(golden):
         assert oracle.rel_l2(F, g[f"{tag}_F"]) < 1e-12
 
 
+def test_type1_modes_vs_golden(golden):
+    """ndft_type1_modes (the FP64 checker of the GPU type-1 at selected modes) against the committed
+    golden direct sums F (tests/golden/make_golden.py, numpy), at seeded mode indices, on square and
+    non-square cases: a swapped (i1, i2) or (N1, N2) reads another mode and fails."""
+    g = golden("tiny.npz")
+    c1 = golden("c1_128_16384.npz")
+    cases = [(g[f"{t}_d"], g[f"{t}_x1"], g[f"{t}_x2"], g[f"{t}_F"]) for t in "abc"]
+    cases.append((c1["d"], c1["x1"], c1["x2"], c1["F"]))
+    rng = np.random.default_rng(11)
+    for d, x1, x2, F in cases:
+        N1, N2 = F.shape
+        q = 256
+        i1 = rng.integers(0, N1, q)
+        i2 = rng.integers(0, N2, q)
+        got = oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2)
+        assert oracle.rel_l2(got, F[i1, i2]) < 1e-12
+        if N1 != N2:
+            # plausible slips fail: the two mode indices swapped, or the two periods swapped
+            sw = oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i2 % N1, i1 % N2)
+            assert oracle.rel_l2(sw, F[i1, i2]) > 1e-2
+            sn = oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N2, N1, i1 % N2, i2 % N1)
+            assert oracle.rel_l2(sn, F[i1 % N2, i2 % N1]) > 1e-2
+    # and against the full direct sum on a ragged set (one point, an empty mode list)
+    d, x1, x2, F = cases[1]
+    N1, N2 = F.shape
+    full = oracle.ndft_type1(d[:1].astype(np.complex128), x1[:1], x2[:1], N1, N2)
+    ii1, ii2 = np.meshgrid(np.arange(N1), np.arange(N2), indexing="ij")
+    one = oracle.ndft_type1_modes(d[:1].astype(np.complex128), x1[:1], x2[:1], N1, N2, ii1.ravel(), ii2.ravel())
+    assert oracle.rel_l2(one, full.ravel()) < 1e-14
+    assert oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, [], []).shape == (0,)
+
+
 def test_derivative_vs_central_differences():
     rng = np.random.default_rng(7)
     N1, N2, M = 16, 16, 64
