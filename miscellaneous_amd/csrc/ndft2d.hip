@@ -3,8 +3,10 @@
 //   out[n] = sum_{m<M} in[m] exp(-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)).
 // The reference fills an explicit M x N FP64 kernel matrix (:44-50; 64 GiB at
 // M = N = 65536), writes it to a text file (:56) and calls cublasZgemv (:67).
-// Here the kernel is evaluated on the fly: each thread owns one output n and
-// streams the points through LDS in chunks, accumulating in FP64.
+// Here nothing is materialised: on the integer meshgrid the reference's
+// caller passes, the sum is an FP64 complex GEMM on the matrix cores
+// (k_ndft_grid); for other (u, v) each thread owns one output n and streams
+// the points through LDS in chunks, accumulating in FP64 (k_ndft_direct).
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -17,11 +19,173 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// ---------------------------------------------------------------------------
+// Output grids.  The reference's only caller passes (u, v) = the integer
+// meshgrid of the far-field plane (Synthesis/PSO/Synthesis.cu:85-90,397):
+// N = nu nv outputs with u and v arithmetic progressions.  Then
+//     out[i, j] = sum_m in_m e^{-2 pi i a_i X_m / Nu} e^{-2 pi i b_j Y_m / Nv},
+//     a_i = u0 + i du, b_j = v0 + j dv,
+// is the complex product (nu x M) . (M x nv) of two phasor matrices: a GEMM on
+// the FP64 matrix cores instead of M N separate kernel evaluations.
+// k_grid_detect recognises the two flattenings of such a grid (u constant
+// over runs of nv outputs, or v constant over runs of nu) by exact FP64
+// comparison and leaves its finding in g_grid (one per device, as every
+// __device__ variable); k_ndft_grid runs when it found a grid, k_ndft_direct
+// (any u, v) when it did not.  Both read the flag on the device, so the call
+// stays asynchronous and capturable.  Two NDFT2_2D_GPU calls that overlap on
+// one device (different streams) race on g_grid.
+struct GridInfo {
+    int ok;          // 1: (u, v) is a grid of nu x nv outputs
+    int vmajor;      // 0: n = i nv + j (u constant over runs of nv); 1: n = j nu + i
+    int nu, nv;
+    double u0, du, v0, dv;
+};
+__device__ GridInfo g_grid;
+
+constexpr int kDetThreads = 1024;
+__global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__restrict__ u,
+                                                             const double *__restrict__ v, int N)
+{
+    __shared__ int s_run[2], s_bad;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        s_run[0] = N;
+        s_run[1] = N;
+        s_bad = 0;
+    }
+    __syncthreads();
+    const double u0 = u[0], v0 = v[0];
+    // first change of u and of v
+    for (int n = 1 + t; n < N; n += kDetThreads) {
+        if (u[n] != u0) atomicMin(&s_run[0], n);
+        if (v[n] != v0) atomicMin(&s_run[1], n);
+    }
+    __syncthreads();
+    const int ru = s_run[0], rv = s_run[1];
+    // u-major: v changes at once and u after nv = ru outputs; v-major: the converse
+    int vmajor = -1, nu = 1, nv = 1;
+    if (N == 1) {
+        vmajor = 0;
+    } else if (rv == 1 && ru > 1 && N % ru == 0) {
+        vmajor = 0;
+        nv = ru;
+        nu = N / ru;
+    } else if (ru == 1 && rv > 1 && N % rv == 0) {
+        vmajor = 1;
+        nu = rv;
+        nv = N / rv;
+    }
+    double du = 0.0, dv = 0.0;
+    if (vmajor == 0) {
+        du = nu > 1 ? u[nv] - u0 : 0.0;
+        dv = nv > 1 ? v[1] - v0 : 0.0;
+    } else if (vmajor == 1) {
+        du = nu > 1 ? u[1] - u0 : 0.0;
+        dv = nv > 1 ? v[nu] - v0 : 0.0;
+    }
+    if (vmajor >= 0) {
+        for (int n = t; n < N; n += kDetThreads) {
+            const int i = vmajor ? n % nu : n / nv, j = vmajor ? n / nu : n % nv;
+            if (u[n] != u0 + i * du || v[n] != v0 + j * dv) s_bad = 1;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        GridInfo g;
+        g.ok = vmajor >= 0 && !s_bad && nu * nv == N;
+        g.vmajor = vmajor > 0;
+        g.nu = nu;
+        g.nv = nv;
+        g.u0 = u0;
+        g.du = du;
+        g.v0 = v0;
+        g.dv = dv;
+        g_grid = g;
+    }
+}
+
+// One 16 x 16 tile of outputs per block; the 16 waves of the block split the
+// points and their partial tiles are summed in LDS in wave order
+// (deterministic).  v_mfma_f64_16x16x4f64 takes 4 points per instruction:
+// lane l supplies A[i][k] at i = l % 16, k = l / 16 and B[k][j] at
+// j = l % 16; accumulator element r of lane l is C[4 r + l / 16][l % 16].  The
+// complex product is four real ones: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br.
+// Operands are exact FP64 phasors: the phase reduced to [0, 1) turns by
+// sincospi.  Persistent over the tiles (their count is known on the device
+// only).
+constexpr int kGridWaves = 16;
+constexpr int kGridBlocks = 512;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 phasor(double t)   // e^{-2 pi i t}
+{
+    t -= floor(t);
+    double s, c;
+    sincospi(2.0 * t, &s, &c);
+    return make_double2(c, -s);
+}
+
+__global__ __launch_bounds__(64 * kGridWaves) void k_ndft_grid(const double *__restrict__ X,
+                                                             const double *__restrict__ Y,
+                                                             const double2 *__restrict__ in,
+                                                             double2 *__restrict__ out, double inv_Nu,
+                                                             double inv_Nv, int M)
+{
+    __shared__ double2 part[kGridWaves][16][16];
+    const GridInfo g = g_grid;
+    if (!g.ok) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = lane & 15, k = lane >> 4;
+    const int tu = (g.nu + 15) / 16, tv = (g.nv + 15) / 16;
+    const int per = (((M + kGridWaves - 1) / kGridWaves) + 3) & ~3;   // points per wave, a multiple of 4
+    const int m_lo = min(M, w * per), m_hi = min(M, m_lo + per);
+    for (int tile = blockIdx.x; tile < tu * tv; tile += gridDim.x) {
+        const int i0 = (tile / tv) * 16, j0 = (tile % tv) * 16;
+        const double a = g.u0 + (i0 + q) * g.du, b = g.v0 + (j0 + q) * g.dv;   // this lane's row / column
+        const bool row_in = i0 + q < g.nu, col_in = j0 + q < g.nv;
+        d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+        for (int m0 = m_lo; m0 < m_hi; m0 += 4) {
+            const int m = m0 + k;
+            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+            if (m < m_hi) {
+                const double2 d = in[m];
+                const double2 pa = phasor(a * X[m] * inv_Nu);
+                const double2 pb = phasor(b * Y[m] * inv_Nv);
+                ar = row_in ? d.x * pa.x - d.y * pa.y : 0.0;
+                ai = row_in ? d.x * pa.y + d.y * pa.x : 0.0;
+                br = col_in ? pb.x : 0.0;
+                bi = col_in ? pb.y : 0.0;
+            }
+            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[w][4 * r + k][q] = make_double2(cr[r], ci[r]);
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            const int ii = threadIdx.x / 16, jj = threadIdx.x % 16;
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < kGridWaves; ++ww) {
+                const double2 p = part[ww][ii][jj];
+                sr += p.x;
+                si += p.y;
+            }
+            const int i = i0 + ii, j = j0 + jj;
+            if (i < g.nu && j < g.nv) out[g.vmajor ? (size_t)j * g.nu + i : (size_t)i * g.nv + j] = make_double2(sr, si);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_ndft_direct(const double *__restrict__ X, const double *__restrict__ Y,
                                                           const double *__restrict__ u, const double *__restrict__ v,
                                                           const double2 *__restrict__ in, double2 *__restrict__ out,
                                                           double inv_Nu, double inv_Nv, int M, int N)
 {
+    if (g_grid.ok) return;         // k_ndft_grid wrote the outputs
     __shared__ double sX[kThreads], sY[kThreads];
     __shared__ double2 sIn[kThreads];
     const int n = blockIdx.x * kThreads + threadIdx.x;
@@ -62,6 +226,13 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
     if (N == 0) return NFFT2D_OK;
     if (!u || !v || !out || (M > 0 && (!X || !Y || !in)))
         return nfft2d_fail(NFFT2D_ERR_ARG, "NDFT2_2D_GPU: null pointer");
+    // (u, v) a grid: FP64 MFMA contraction; otherwise the direct sum (each kernel
+    // reads the detector's finding and returns at once when it is not its case)
+    hipLaunchKernelGGL(k_grid_detect, dim3(1), dim3(kDetThreads), 0, nullptr, u, v, N);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ndft_grid, dim3(kGridBlocks), dim3(64 * kGridWaves), 0, nullptr, X, Y,
+                       reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), 1.0 / Nu, 1.0 / Nv, M);
+    NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_ndft_direct, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, X, Y, u, v,
                        reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), 1.0 / Nu, 1.0 / Nv, M,
                        N);

@@ -129,3 +129,75 @@ def test_ndft_empty_and_plan_reuse():
         y = plan.eval(T(x1), T(x2)).cpu().numpy()
         ref = oracle.ndft_type2(f.astype(np.complex128), x1, x2)
         assert oracle.rel_l2(y, ref) <= TOL["bf16x3"]
+
+
+# ------------------------------------------------------------------ the reference's NDFT2_2D_GPU
+def _meshgrid(nu, nv, u0, v0, vmajor):
+    n = np.arange(nu * nv)
+    i, j = (n % nu, n // nu) if vmajor else (n // nv, n % nv)
+    return (u0 + i).astype(np.float64), (v0 + j).astype(np.float64)
+
+
+@pytest.mark.parametrize("nu,nv,M,vmajor", [(88, 88, 1936, 0), (88, 88, 1936, 1), (16, 40, 777, 0), (40, 16, 5, 1),
+                                            (1, 33, 300, 0), (33, 1, 300, 1), (7, 9, 0, 0)])
+def test_ndft2_2d_dropin_on_grids(nu, nv, M, vmajor):
+    """NDFT2_2D_GPU on meshgrid outputs (both flattenings, ragged tiles, a single row or column, no
+    points): the FP64 matrix-core contraction against the oracle's direct sum (NFFT/NDFT2_2D.cu:28-29,67)."""
+    rng = np.random.default_rng(nu * 1000 + nv + M)
+    X = rng.uniform(-nu / 2, nu / 2, M)
+    Y = rng.uniform(-nv / 2, nv / 2, M)
+    d = (rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)
+    u, v = _meshgrid(nu, nv, -(nu // 2), -(nv // 2), vmajor)
+    out = torch.empty(nu * nv, dtype=torch.complex128, device=DEV)
+    print(f"case NDFT2_2D_GPU grid {nu}x{nv} M={M} vmajor={vmajor}", flush=True)
+    m.NDFT2_2D_GPU(0, T(X), T(Y), T(u), T(v), T(d), out, nu, nv, M, nu * nv)
+    torch.cuda.synchronize()
+    ref = oracle.ndft2_2d(X, Y, u, v, d, nu, nv)
+    got = out.cpu().numpy()
+    if M == 0:
+        assert np.all(got == 0)
+    else:
+        assert oracle.rel_l2(got, ref) < 1e-12
+
+
+def test_ndft2_2d_dropin_grid_detection_falls_back():
+    """Outputs that are not a grid (one value off, a permuted grid, a non-divisor run) take the
+    direct sum and still match; the same call on the true grid then takes the contraction."""
+    rng = np.random.default_rng(31)
+    nu, nv, M = 24, 20, 1000
+    X = rng.uniform(-12, 12, M)
+    Y = rng.uniform(-10, 10, M)
+    d = (rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)
+    u, v = _meshgrid(nu, nv, -12, -10, 0)
+    cases = []
+    u1 = u.copy(); u1[77] += 0.5
+    cases.append((u1, v))
+    p = rng.permutation(nu * nv)
+    cases.append((u[p], v[p]))
+    cases.append((u[:nu * nv - 3], v[:nu * nv - 3]))
+    cases.append((u, v))
+    for uu, vv in cases:
+        out = torch.empty(len(uu), dtype=torch.complex128, device=DEV)
+        m.NDFT2_2D_GPU(0, T(X), T(Y), T(uu), T(vv), T(d), out, nu, nv, M, len(uu))
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(out.cpu().numpy(), oracle.ndft2_2d(X, Y, uu, vv, d, nu, nv)) < 1e-12
+
+
+def test_ndft2_2d_dropin_256_grid_65536_points():
+    """256 x 256 meshgrid outputs from M = 65,536 points: 1024 seeded outputs against the oracle."""
+    rng = np.random.default_rng(32)
+    nu = nv = 256
+    M = 65536
+    X = rng.uniform(-128, 128, M)
+    Y = rng.uniform(-128, 128, M)
+    d = (rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)
+    u, v = _meshgrid(nu, nv, -128, -128, 0)
+    out = torch.empty(nu * nv, dtype=torch.complex128, device=DEV)
+    print("case NDFT2_2D_GPU grid 256x256 M=65536", flush=True)
+    m.NDFT2_2D_GPU(0, T(X), T(Y), T(u), T(v), T(d), out, nu, nv, M, nu * nv)
+    torch.cuda.synchronize()
+    sel = rng.choice(nu * nv, 1024, replace=False)
+    ref = oracle.ndft2_2d(X, Y, u[sel], v[sel], d, nu, nv)
+    e = oracle.rel_l2(out.cpu().numpy()[sel], ref)
+    print(f"NDFT2_2D_GPU 256^2 grid, M = 65536: rel-L2 {e:.2e} on 1024 outputs")
+    assert e < 1e-12
