@@ -84,6 +84,19 @@ int NFFT1_2D_GPU_c64(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x,
 int NFFT1_2D_GPU_c128(nfft2d_c128 *result, const nfft2d_c128 *data, const double *x, const double *y,
                       int N1, int N2, int M);
 
+/* NFFT2_2D_GPU_c64 that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as NFFT2_2D_GPU_c64 (the same kernels), plus the
+ * stream the work is ordered on (NULL: the legacy default stream).  Every
+ * buffer of the transform lives in one file-scope device array of 22 MiB that
+ * each GPU the library runs on holds until the process ends; hence the limits
+ * N1, N2 <= 256 (even) and M <= 262144, NFFT2D_ERR_SIZE beyond them.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (different
+ * streams or host threads without ordering between them) race on it. */
+int NFFT2_2D_GPU_c64_noalloc(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y,
+                             int N1, int N2, int M, void *stream);
+
 /* Plan warm-up hooks, NFFT/NFFT2_2D.cu:49-50,55 (decl. NFFT/NFFT2_2D.cuh:5-7).
  * The reference builds one process-global cuFFT plan of size (2 N1) x (2 N2)
  * that the next NFFT2_2D_GPU call silently assumes; here they only pre-build

@@ -41,6 +41,7 @@ _SIGS = {
     "nfft2d_version": (_i, []),
     "NFFT2_2D_GPU_c64": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
     "NFFT2_2D_GPU_c128": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "NFFT2_2D_GPU_c64_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "NFFT1_2D_GPU_c64": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
     "NFFT1_2D_GPU_c128": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
     "Calculate_cuFFT_plan_C2C_NFFT2_2D": (None, [_i, _i]),

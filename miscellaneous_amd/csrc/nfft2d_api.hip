@@ -958,6 +958,177 @@ extern "C" void Destroy_cuFFT_plan_NFFT2_2D(void)
 }
 
 // ============================================================================
+// NFFT2_2D_GPU_c64_noalloc: the complex64 type-1 drop-in for callers that
+// cannot allocate device memory (code under graph capture, serving loops).
+// Every buffer of the transform (tables, oversampled grid, sorted points,
+// histograms, subproblem lists, spread tiles) is carved out of one
+// file-scope device array, `scratch`, which each GPU holds from the moment
+// the library is loaded on it until the process ends.  Its device address is
+// taken once per device with hipGetSymbolAddress (the array's name is a host
+// stand-in, not a device address) and kept in g_noalloc_ptr.  The kernels
+// and their order are those of NFFT2_2D_GPU_c64 (the results agree to FP32
+// rounding: the bin sort orders a bin's points as the waves reach its
+// counter, which the spread's sums follow).  Every call on a GPU shares that GPU's array: calls that overlap on one
+// GPU (different streams, or threads) race on it.  Every buffer is written
+// before it is read within a call (the setpts passes write every histogram
+// row, offset and subproblem entry; the spread writes every tile it merges and
+// every grid cell), so nothing is cleared between calls.
+constexpr int kNoAllocMaxN = 256;                   // N1, N2 <= 256
+constexpr int kNoAllocMaxM = 1 << 18;               // M <= 262,144
+constexpr int kNoAllocMaxL = kC * kNoAllocMaxN;
+constexpr int kNoAllocMaxBins = (kNoAllocMaxL / kBin) * (kNoAllocMaxL / kBin);
+constexpr int kNoAllocMaxSub = kNoAllocMaxBins + kNoAllocMaxM / kGatherThreads;
+static_assert(kNoAllocMaxBins <= kMaxLdsBins, "the no-allocation path uses the LDS histogram (no rank array)");
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of the buffers inside `scratch` (all at their maxima).
+struct NoAllocLayout {
+    size_t tw1, tw2, D1, D2, grid, tmp, pts, hist, tot, nsub, subbeg, gexcl, gsum, sub, tiles, total;
+    constexpr NoAllocLayout() : tw1(0), tw2(0), D1(0), D2(0), grid(0), tmp(0), pts(0), hist(0), tot(0), nsub(0),
+                                subbeg(0), gexcl(0), gsum(0), sub(0), tiles(0), total(0) {}
+};
+static NoAllocLayout noalloc_layout()
+{
+    NoAllocLayout o;
+    size_t at = 0;
+    auto take = [&](size_t &field, size_t bytes) { field = at; at += align256(bytes); };
+    take(o.tw1, sizeof(float2) * 2 * kNoAllocMaxN);
+    take(o.tw2, sizeof(float2) * 2 * kNoAllocMaxN);
+    take(o.D1, sizeof(float) * kNoAllocMaxN);
+    take(o.D2, sizeof(float) * kNoAllocMaxN);
+    take(o.grid, sizeof(float2) * (size_t)kNoAllocMaxL * kNoAllocMaxL);
+    take(o.tmp, sizeof(float2) * (size_t)kNoAllocMaxN * kNoAllocMaxL);
+    take(o.pts, sizeof(PtRec<float>) * (size_t)kNoAllocMaxM);
+    take(o.hist, sizeof(int) * (size_t)256 * kNoAllocMaxBins);      // at most 256 histogram blocks
+    take(o.tot, sizeof(int) * kNoAllocMaxBins);
+    take(o.nsub, sizeof(int));
+    take(o.subbeg, sizeof(int) * (kNoAllocMaxBins + 1));
+    take(o.gexcl, sizeof(int2) * kNoAllocMaxBins);
+    take(o.gsum, sizeof(int2) * ((kNoAllocMaxBins + kColBins - 1) / kColBins));
+    take(o.sub, sizeof(int4) * kNoAllocMaxSub);
+    take(o.tiles, sizeof(float2) * kTileCells * (size_t)kNoAllocMaxSub);
+    o.total = at;
+    return o;
+}
+constexpr size_t kNoAllocBytes = (size_t)22 << 20;  // 22 MiB >= noalloc_layout().total (checked at set-up)
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};       // device address of `scratch` per device
+
+// Host copies of the complex64 twiddle / deconvolution tables per N (the
+// values upload_tables<float> computes), kept so that an asynchronous copy
+// never reads freed host memory.
+std::map<int, std::pair<std::vector<float2>, std::vector<float>>> g_noalloc_tables;
+WinArgs g_noalloc_wa;
+bool g_noalloc_wa_ready = false;
+
+const std::pair<std::vector<float2>, std::vector<float>> &noalloc_tables(int N)
+{
+    auto it = g_noalloc_tables.find(N);
+    if (it != g_noalloc_tables.end()) return it->second;
+    const double S = host_i0(kK * host_alpha()), a = host_alpha();
+    std::vector<float2> tw(2 * N);
+    for (int q = 0; q < 2 * N; ++q) {
+        const double ang = -kPi * (double)q / (double)N;
+        tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    std::vector<float> D(N);
+    for (int i = 0; i < N; ++i) {
+        const double xi = 2.0 * kPi * (double)(i - N / 2) / ((double)kC * N);
+        D[i] = (float)(S / host_i0(kK * std::sqrt(a * a - xi * xi)));
+    }
+    return g_noalloc_tables[N] = {std::move(tw), std::move(D)};
+}
+
+// The device address of `scratch` on the current device, looked up once per device.
+int noalloc_base(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "NFFT2_2D_GPU_c64_noalloc: device %d above the supported %d", device,
+                           kNoAllocMaxDevices);
+    if (!g_noalloc_ptr[device]) {
+        if (noalloc_layout().total > kNoAllocBytes)
+            return nfft2d_fail(NFFT2D_ERR_SIZE, "NFFT2_2D_GPU_c64_noalloc: scratch layout exceeds the array");
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return nfft2d_fail(NFFT2D_ERR_HIP, "NFFT2_2D_GPU_c64_noalloc: hipGetSymbolAddress(scratch): %s",
+                               hipGetErrorString(e));
+        g_noalloc_ptr[device] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_noalloc_ptr[device]);
+    return NFFT2D_OK;
+}
+}  // namespace
+
+extern "C" int NFFT2_2D_GPU_c64_noalloc(nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, const float *y,
+                                        int N1, int N2, int M, void *stream)
+{
+    if (M < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "NFFT2_2D_GPU_c64_noalloc: M < 0");
+    if (!result) return nfft2d_fail(NFFT2D_ERR_ARG, "NFFT2_2D_GPU_c64_noalloc: null result pointer");
+    if (M > 0 && (!data || !x || !y)) return nfft2d_fail(NFFT2D_ERR_ARG, "NFFT2_2D_GPU_c64_noalloc: null pointer");
+    if (N1 <= 0 || N2 <= 0 || (N1 & 1) || (N2 & 1) || N1 > kNoAllocMaxN || N2 > kNoAllocMaxN)
+        return nfft2d_fail(NFFT2D_ERR_SIZE,
+                           "NFFT2_2D_GPU_c64_noalloc: N1, N2 must be even and at most %d (got %d, %d); larger "
+                           "transforms go through NFFT2_2D_GPU_c64",
+                           kNoAllocMaxN, N1, N2);
+    if (M > kNoAllocMaxM)
+        return nfft2d_fail(NFFT2D_ERR_SIZE,
+                           "NFFT2_2D_GPU_c64_noalloc: M = %d above the maximum %d; larger point sets go through "
+                           "NFFT2_2D_GPU_c64",
+                           M, kNoAllocMaxM);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char *base = nullptr;
+    int rc = noalloc_base(device, &base);
+    if (rc) return rc;
+    const NoAllocLayout o = noalloc_layout();
+    // a plan whose buffers are views of `scratch`, sized to the maxima so that
+    // no path of setpts / spread grows (allocates) anything
+    nfft2d_plan_s p;
+    p.N1 = N1; p.N2 = N2; p.L1 = kC * N1; p.L2 = kC * N2; p.dtype = NFFT2D_C64; p.device = device;
+    p.esz = sizeof(float2);
+    p.nb1 = (p.L1 + kBin - 1) / kBin;
+    p.nb2 = (p.L2 + kBin - 1) / kBin;
+    p.nbins = p.nb1 * p.nb2;
+    p.geo.L1 = p.L1; p.geo.L2 = p.L2; p.geo.nb1 = p.nb1;
+    p.geo.inv1d = 1.0 / p.L1; p.geo.inv2d = 1.0 / p.L2;
+    p.geo.inv1f = (float)p.geo.inv1d; p.geo.inv2f = (float)p.geo.inv2d;
+    if (!g_noalloc_wa_ready) {
+        make_window_args(g_noalloc_wa);
+        g_noalloc_wa_ready = true;
+    }
+    p.wa = g_noalloc_wa;
+    p.tw1 = base + o.tw1; p.tw2 = base + o.tw2; p.D1 = base + o.D1; p.D2 = base + o.D2;
+    p.grid = base + o.grid; p.tmp = base + o.tmp; p.pts = base + o.pts;
+    p.hist = reinterpret_cast<int *>(base + o.hist); p.tot = reinterpret_cast<int *>(base + o.tot);
+    p.nsub = reinterpret_cast<int *>(base + o.nsub); p.subbeg = reinterpret_cast<int *>(base + o.subbeg);
+    p.gexcl = reinterpret_cast<int2 *>(base + o.gexcl); p.gsum = reinterpret_cast<int2 *>(base + o.gsum);
+    p.sub = reinterpret_cast<int4 *>(base + o.sub); p.scratch = base + o.tiles;
+    p.Mcap = kNoAllocMaxM; p.subcap = kNoAllocMaxSub; p.histcap = 256 * kNoAllocMaxBins; p.scratchcap = kNoAllocMaxSub;
+    p.rank = nullptr;                                  // the LDS histogram path needs none
+    hipStream_t s = (hipStream_t)stream;
+    for (int dim = 0; dim < 2; ++dim) {
+        const int N = dim == 0 ? N1 : N2;
+        const auto &tb = noalloc_tables(N);
+        NFFT2D_HIP_TRY(hipMemcpyAsync(dim == 0 ? p.tw1 : p.tw2, tb.first.data(), sizeof(float2) * 2 * N,
+                                      hipMemcpyHostToDevice, s));
+        NFFT2D_HIP_TRY(hipMemcpyAsync(dim == 0 ? p.D1 : p.D2, tb.second.data(), sizeof(float) * N,
+                                      hipMemcpyHostToDevice, s));
+    }
+    // the same calls as NFFT2_2D_GPU_c64: setpts, spread, FFT + crop + deconvolution
+    if ((rc = setpts_impl<float>(&p, x, y, M, s))) return rc;
+    if ((rc = spread_impl<float>(&p, data, false, s))) return rc;
+    return t1_passes<float, -1>(&p, result, s);
+}
+
+// ============================================================================
 // Batches: many complex64 type-2 NFFTs of different sizes, one launch per stage
 // (kernels in nfft2d_batch.hpp).
 #include "nfft2d_batch.hpp"
