@@ -227,11 +227,37 @@ int nfft2d_ndft_eval(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const float *x,
 int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x,
                       const float *y, int M, void *stream);
 
+/* ------------------------------------------------------------------ 1-D lines */
+
+/* Batched 1-D NFFTs, complex64, K = 6, c = 2: `batch` lines per call, each
+ * line with N modes and M points, stored line after line (line b: modes at
+ * b*N, points and samples at b*M).  N even, at most 2048, or a power of two up
+ * to 4096 (type-1: at most 2048); coordinates in grid units with period N.
+ *   nfft1d_type2  BP_FBP_FFBP/NFFT1_1D.m:1 (called once per pulse in
+ *                 BP_FBP_FFBP/BP_NFFT.m:18):
+ *                 result[b][l] = sum_k data[b][k + N/2] e^{-2 pi i k x[b][l] / N}
+ *   nfft1d_type1  NFFT/NFFT2/1D/Matlab/NFFT2_1D.m:1 (NFFT2_1D(data, x, N, 2, 6),
+ *                 Synthesis/Aper_Synth_1D/Aper_Synth_1D.m:74):
+ *                 result[b][k + N/2] = sum_l data[b][l] e^{-2 pi i k x[b][l] / N}
+ * The reference rounds c x half away from zero (MATLAB round); here half to
+ * even, as the 2-D path (the two differ only at exact ties and are both valid
+ * NFFTs).  The type-1 sums are exact in fixed point: its result does not
+ * depend on the order of the additions.  The first call with more lines than
+ * before allocates the plan's grid: make it outside a graph capture. */
+typedef struct nfft1d_plan_s nfft1d_plan;
+int nfft1d_plan_create(nfft1d_plan **plan, int N);
+int nfft1d_plan_destroy(nfft1d_plan *plan);
+int nfft1d_type2(nfft1d_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, int M, int batch,
+                 void *stream);
+int nfft1d_type1(nfft1d_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, int M, int batch,
+                 void *stream);
+
 /* ------------------------------------------------------------------ batches */
 
-/* Many complex64 type-2 transforms of different sizes (K = 6, c = 2; N1, N2
- * powers of two in [8, 1024]; any M), run as ONE kernel launch per stage over
- * all of them: nfft2d_batch_setpts is 5 launches and nfft2d_batch_type2 is 4,
+/* Many complex64 transforms of different sizes (K = 6, c = 2; N1, N2 even in
+ * [2, 1024]: powers of two from 8 run Stockham passes, other sizes direct
+ * ones; any M), run as ONE kernel launch per stage over all of them:
+ * nfft2d_batch_setpts is 5 launches, nfft2d_batch_type2 and type1 4 each,
  * whatever the number of transforms.  Neither allocates nor records a
  * hipMemset, so both can be captured into one hipGraph and replayed.  There is
  * no reference counterpart: the reference runs one NFFT2_2D_GPU call per
@@ -264,6 +290,19 @@ int nfft2d_batch_setpts_type2(nfft2d_batch *batch, void *stream);
 int nfft2d_batch_autotune(nfft2d_batch *batch, void *stream, int reps);
 int nfft2d_batch_set_maxpts(nfft2d_batch *batch, int t, int maxpts);
 int nfft2d_batch_get_maxpts(const nfft2d_batch *batch, int t);
+/* Type-1 of every transform (the reference's NFFT2_2D_GPU, NFFT/NFFT2_2D.cu:329,
+ * once per transform; e.g. once per particle of a PSO generation,
+ * Synthesis/PSO/PSO.cu:348 -> Synthesis.cu:395): result[t][k1][k2] =
+ * sum_l data[t][l] e^{-2 pi i (k1 x[t][l] / N1 + k2 y[t][l] / N2)}.
+ *   set_io_type1   data[t] (M[t] samples) and result[t] (N1 x N2): device
+ *                  arrays, kept by reference; allocates each transform's
+ *                  spread tiles; N1, N2 >= 22 (NFFT2D_ERR_SIZE otherwise);
+ *   type1          4 launches (spread, tile merge, the two FFT passes with
+ *                  crop and deconvolution), needs setpts;
+ *   setpts_type1   setpts then type1 on `stream` (9 launches). */
+int nfft2d_batch_set_io_type1(nfft2d_batch *batch, const nfft2d_c64 *const *data, nfft2d_c64 *const *result);
+int nfft2d_batch_type1(nfft2d_batch *batch, void *stream);
+int nfft2d_batch_setpts_type1(nfft2d_batch *batch, void *stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,10 @@ _SIGS = {
     "nfft2d_ndft_set_modes": (_i, [_vp, _vp, _vp]),
     "nfft2d_ndft_eval": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "nfft2d_ndft_type2": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "nfft1d_plan_create": (_i, [ctypes.POINTER(_vp), _i]),
+    "nfft1d_plan_destroy": (_i, [_vp]),
+    "nfft1d_type2": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "nfft1d_type1": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp]),
     "nfft2d_batch_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                                  ctypes.c_uint]),
     "nfft2d_batch_destroy": (_i, [_vp]),
@@ -81,6 +85,9 @@ _SIGS = {
     "nfft2d_batch_type2": (_i, [_vp, _vp]),
     "nfft2d_batch_setpts_type2": (_i, [_vp, _vp]),
     "nfft2d_batch_autotune": (_i, [_vp, _vp, _i]),
+    "nfft2d_batch_set_io_type1": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "nfft2d_batch_type1": (_i, [_vp, _vp]),
+    "nfft2d_batch_setpts_type1": (_i, [_vp, _vp]),
     "nfft2d_batch_set_maxpts": (_i, [_vp, _i, _i]),
     "nfft2d_batch_get_maxpts": (_i, [_vp, _i]),
 }

@@ -1,10 +1,13 @@
 """Batched type-2 NFFTs of different sizes, over the C ABI (nfft2d_batch_*).
 
 ``Batch(sizes, device)`` with ``sizes = [(N1, N2, M), ...]`` builds one plan
-per transform (complex64, K = 6, c = 2; N1, N2 powers of two in [8, 1024]).
-Every stage then runs as one kernel launch over all transforms:
-``setpts()`` is 5 launches and ``type2()`` 4, so both can be captured into
-one graph.  Arrays are kept by reference (``set_points`` / ``set_io``): keep
+per transform (complex64, K = 6, c = 2; N1, N2 even in [2, 1024]: powers of
+two from 8 run the Stockham passes, other sizes direct ones).  Every stage
+then runs as one kernel launch over all transforms: ``setpts()`` is 5
+launches, ``type2()`` and ``type1()`` 4 each, so they can be captured into one
+graph.  ``type1`` (N1, N2 >= 22) is the reference's NFFT2_2D_GPU for every
+transform, e.g. the far fields of one PSO generation
+(``Synthesis/PSO/PSO.cu:348``, ``Synthesis.cu:395``) in one graph.  Arrays are kept by reference (``set_points`` / ``set_io``): keep
 the tensors alive while the batch uses them.
 """
 from __future__ import annotations
@@ -38,7 +41,7 @@ def _check(t: torch.Tensor, dtype, numel: int, name: str, device) -> None:
 
 
 class Batch:
-    """Many complex64 type-2 NFFTs of different sizes, one launch per stage."""
+    """Many complex64 type-2 / type-1 NFFTs of different sizes, one launch per stage."""
 
     def __init__(self, sizes, device=None):
         self.sizes = [(int(a), int(b), int(m)) for a, b, m in sizes]
@@ -97,6 +100,29 @@ class Batch:
         concurrent with the grid passes."""
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_batch_setpts_type2(self._h, _stream(self.device)), "nfft2d_batch_setpts_type2")
+
+    def set_io_type1(self, data, results) -> "Batch":
+        """Type-1 arrays: data[t] (M[t] samples), results[t] (N1 x N2 modes); kept by reference."""
+        if len(data) != self.count or len(results) != self.count:
+            raise ValueError("one sample array and one result array per transform")
+        for t, (d, r) in enumerate(zip(data, results)):
+            _check(d, torch.complex64, self.sizes[t][2], f"data[{t}]", self.device)
+            _check(r, torch.complex64, self.sizes[t][0] * self.sizes[t][1], f"results[{t}]", self.device)
+        self._io1 = (list(data), list(results))
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_set_io_type1(self._h, self._ptrs(data), self._ptrs(results)),
+                  "nfft2d_batch_set_io_type1")
+        return self
+
+    def type1(self) -> None:
+        """results[t] = type-1 of data[t] at the sorted points of transform t (NFFT2_2D_GPU per transform)."""
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_type1(self._h, _stream(self.device)), "nfft2d_batch_type1")
+
+    def setpts_type1(self) -> None:
+        """setpts() then type1() (9 launches in all)."""
+        with torch.cuda.device(self.device):
+            check(self._lib.nfft2d_batch_setpts_type1(self._h, _stream(self.device)), "nfft2d_batch_setpts_type1")
 
     def autotune(self, reps: int = 3) -> list:
         """Choose the subproblem size per size class by timing; returns the per-transform choice."""

@@ -27,6 +27,9 @@ def hipcc() -> str:
 def compile_cmd(out: str) -> list[str]:
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # MFMA accumulators in VGPRs: the gathers read them back with VALU
+            # FMAs, which AGPRs would cost a v_accvgpr_read each
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-Wno-unused-command-line-argument",
             "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
 

@@ -18,6 +18,7 @@
 #include "../../include/nfft2d.h"
 #include "nfft2d_internal.hpp"
 #include "nfft2d_kernels.hpp"
+#include "nfft1d_kernels.hpp"
 
 using namespace nfft2d;
 
@@ -1129,6 +1130,153 @@ extern "C" int NFFT2_2D_GPU_c64_noalloc(nfft2d_c64 *result, const nfft2d_c64 *da
 }
 
 // ============================================================================
+// Batched 1-D NFFTs (complex64): BP_FBP_FFBP/NFFT1_1D.m (type-2) and
+// NFFT/NFFT2/1D/Matlab/NFFT2_1D.m (type-1), B lines per call (kernels in
+// nfft1d_kernels.hpp; the row passes are the 2-D transform's).
+struct nfft1d_plan_s {
+    int N = 0, L = 0, device = 0, rows = 0;   // rows: lines the grid and the unit row factors hold
+    void *tw = nullptr, *D = nullptr;         // 2N twiddles, N deconvolution factors S / phi(xi_k)
+    void *ones = nullptr, *grid = nullptr;    // rows x 1.0f (the row passes' per-row factor), rows x L grid
+    WinArgs wa;
+};
+
+static void free_plan1d(nfft1d_plan *p)
+{
+    if (!p) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(p->device);
+    hfree(p->tw); hfree(p->D); hfree(p->ones); hfree(p->grid);
+    (void)hipSetDevice(cur);
+    delete p;
+}
+
+extern "C" int nfft1d_plan_create(nfft1d_plan **out, int N)
+{
+    if (!out) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft1d_plan_create: null output pointer");
+    *out = nullptr;
+    if (N < 2 || (N & 1) || N > 4096 || (!is_pow2(N) && N > 2048))
+        return nfft2d_fail(NFFT2D_ERR_SIZE,
+                           "nfft1d_plan_create: N must be even, at most 2048, or a power of two up to 4096 (got %d)", N);
+    int device = 0;
+    NFFT2D_HIP_TRY(hipGetDevice(&device));
+    nfft1d_plan *p = new (std::nothrow) nfft1d_plan_s();
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft1d_plan_create: host allocation failed");
+    p->N = N;
+    p->L = kC * N;
+    p->device = device;
+    make_window_args(p->wa);
+    const auto &tb = noalloc_tables(N);        // the complex64 tables of upload_tables<float>
+    if (hipMalloc(&p->tw, sizeof(float2) * 2 * N) != hipSuccess || hipMalloc(&p->D, sizeof(float) * N) != hipSuccess) {
+        free_plan1d(p);
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft1d_plan_create: hipMalloc of the tables failed");
+    }
+    if (hipMemcpy(p->tw, tb.first.data(), sizeof(float2) * 2 * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->D, tb.second.data(), sizeof(float) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        free_plan1d(p);
+        return nfft2d_fail(NFFT2D_ERR_HIP, "nfft1d_plan_create: upload of the tables failed");
+    }
+    *out = p;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft1d_plan_destroy(nfft1d_plan *p)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft1d_plan_destroy: null plan");
+    free_plan1d(p);
+    return NFFT2D_OK;
+}
+
+// Grid and unit row factors for B lines (grown, never shrunk; the first call
+// with more lines than before allocates).
+static int plan1d_rows(nfft1d_plan *p, int B)
+{
+    if (B <= p->rows) return NFFT2D_OK;
+    hfree(p->ones); hfree(p->grid);
+    p->ones = p->grid = nullptr;
+    p->rows = 0;
+    if (hipMalloc(&p->ones, sizeof(float) * B) != hipSuccess ||
+        hipMalloc(&p->grid, sizeof(float2) * (size_t)B * p->L) != hipSuccess)
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft1d: hipMalloc for %d lines failed", B);
+    std::vector<float> one(B, 1.0f);
+    NFFT2D_HIP_TRY(hipMemcpy(p->ones, one.data(), sizeof(float) * B, hipMemcpyHostToDevice));
+    p->rows = B;
+    return NFFT2D_OK;
+}
+
+static int check1d(nfft1d_plan *p, const void *a, const void *b, const void *x, int M, int B, bool type2,
+                   const char *what)
+{
+    if (!p) return nfft2d_fail(NFFT2D_ERR_ARG, "%s: null plan", what);
+    if (M < 0 || B < 0 || M > kMaxPoints) return nfft2d_fail(NFFT2D_ERR_ARG, "%s: bad M = %d or batch = %d", what, M, B);
+    if ((long long)M * B > (1LL << 31) - 1)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "%s: batch x M above 2^31 - 1", what);
+    // a: the output (type-2: M per line, type-1: N), b: the input (type-2: N, type-1: M per line)
+    const bool need_a = B > 0 && (!type2 || M > 0), need_b = B > 0 && (type2 || M > 0);
+    if ((need_a && !a) || (need_b && !b) || (B > 0 && M > 0 && !x))
+        return nfft2d_fail(NFFT2D_ERR_ARG, "%s: null pointer", what);
+    return nfft2d_check_device(p->device);
+}
+
+extern "C" int nfft1d_type2(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, int M,
+                            int batch, void *stream)
+{
+    int rc = check1d(p, result, data, x, M, batch, true, "nfft1d_type2");
+    if (rc || batch == 0 || M == 0) return rc;
+    if ((rc = plan1d_rows(p, batch))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const float2 *f = reinterpret_cast<const float2 *>(data);
+    float2 *G = (float2 *)p->grid;
+    const float *ones = (const float *)p->ones, *D = (const float *)p->D;
+    const float2 *tw = (const float2 *)p->tw;
+    if (fast_path(p->N, NFFT2D_C64)) {
+        dispatch_n(p->N, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            constexpr int R = rows_per_block<N>();
+            hipLaunchKernelGGL((k_t2_rows<float, N, -1>), dim3(grid1d(batch, R)), dim3(R * (N / 8)), 0, s, f, G, ones, D,
+                               tw, batch);
+        });
+    } else {
+        hipLaunchKernelGGL((k_t2_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->N, s, f, G, ones, D,
+                           tw, batch, p->N);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gather1d, dim3(batch, grid1d(M, k1dPointsPerBlock)), dim3(k1dThreads), sizeof(float2) * p->L, s,
+                       G, p->L, 1.0f / p->L, x, M, p->wa, reinterpret_cast<float2 *>(result));
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft1d_type1(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64 *data, const float *x, int M,
+                            int batch, void *stream)
+{
+    int rc = check1d(p, result, data, x, M, batch, false, "nfft1d_type1");
+    if (rc || batch == 0) return rc;
+    if (p->N > 2048) return nfft2d_fail(NFFT2D_ERR_SIZE, "nfft1d_type1: N above 2048 (its LDS accumulators)");
+    if ((rc = plan1d_rows(p, batch))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    float2 *G = (float2 *)p->grid, *F = reinterpret_cast<float2 *>(result);
+    const float *ones = (const float *)p->ones, *D = (const float *)p->D;
+    const float2 *tw = (const float2 *)p->tw;
+    hipLaunchKernelGGL(k_spread1d, dim3(batch), dim3(k1dThreads), sizeof(unsigned long long) * 2 * p->L, s,
+                       reinterpret_cast<const float2 *>(data), x, M, p->L, 1.0f / p->L, p->wa, G);
+    NFFT2D_LAUNCH_CHECK();
+    if (fast_path(p->N, NFFT2D_C64)) {
+        dispatch_n(p->N, [&](auto n) {
+            constexpr int N = decltype(n)::value;
+            constexpr int R = rows_per_block<N>();
+            hipLaunchKernelGGL((k_t1_rows<float, N, -1>), dim3(grid1d(batch, R)), dim3(R * (N / 8)), 0, s, G, F, ones, D,
+                               tw, batch);
+        });
+    } else {
+        hipLaunchKernelGGL((k_t1_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->L, s, G, F, ones,
+                           D, tw, batch, p->N);
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+// ============================================================================
 // Batches: many complex64 type-2 NFFTs of different sizes, one launch per stage
 // (kernels in nfft2d_batch.hpp).
 #include "nfft2d_batch.hpp"
@@ -1140,12 +1288,12 @@ struct nfft2d_batch_s {
     BatchDesc *ddesc = nullptr;
     int2 *own = nullptr;                  // owner tables of the 5 per-transform stages, back to back
     int off_hist = 0, n_hist = 0, off_col = 0, n_col = 0, off_scat = 0, n_scat = 0;
-    int off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0;
+    int off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0, off_merge = 0, n_merge = 0;
     int *base = nullptr;                  // count + 1: subproblem offsets, base[count] = total
     int4 *bsub = nullptr;                 // the batch's subproblem list
     long long bsubcap = 0;
     int max_nbins = 0;
-    bool have_points = false, have_io = false;
+    bool have_points = false, have_io = false, have_io1 = false;
     bool sorted = false;                  // the subproblem list matches the points and sizes (batch_setpts ran)
     Fork fk;                              // setpts beside the grid passes (nfft2d_batch_setpts_type2)
 };
@@ -1180,6 +1328,9 @@ static void batch_fill_desc(nfft2d_batch *b, int t)
     d.tw1 = (const float2 *)p->tw1; d.tw2 = (const float2 *)p->tw2;
     d.hist = p->hist; d.rank = p->rank; d.tot = p->tot; d.subbeg = p->subbeg; d.nsub = p->nsub;
     d.gexcl = p->gexcl; d.gsum = p->gsum; d.sub = p->sub; d.pts = (PtRec<float> *)p->pts;
+    d.scratch = (float2 *)p->scratch;
+    d.r1 = merge_radius(p->L1);
+    d.r2 = merge_radius(p->L2);
 }
 
 static int batch_upload(nfft2d_batch *b)
@@ -1196,8 +1347,8 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     if (count <= 0 || !N1 || !N2 || !M) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: count <= 0 or null sizes");
     if (flags != 0) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: flags must be 0");
     for (int t = 0; t < count; ++t) {
-        if (!is_pow2(N1[t]) || !is_pow2(N2[t]) || N1[t] < 8 || N2[t] < 8 || N1[t] > kBatchMaxN || N2[t] > kBatchMaxN)
-            return nfft2d_fail(NFFT2D_ERR_SIZE, "batch_create: transform %d: N1=%d N2=%d must be powers of two in [8, %d]",
+        if (N1[t] < 2 || N2[t] < 2 || (N1[t] & 1) || (N2[t] & 1) || N1[t] > kBatchMaxN || N2[t] > kBatchMaxN)
+            return nfft2d_fail(NFFT2D_ERR_SIZE, "batch_create: transform %d: N1=%d N2=%d must be even and in [2, %d]",
                                t, N1[t], N2[t], kBatchMaxN);
         if (M[t] < 0 || M[t] > kMaxPoints)
             return nfft2d_fail(NFFT2D_ERR_ARG, "batch_create: transform %d: M = %d outside [0, %d]", t, M[t], kMaxPoints);
@@ -1237,12 +1388,16 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     add_stage(b->off_scat, b->n_scat, [&](int t) {
         return (int)grid1d(std::max(b->hdesc[t].M, b->hdesc[t].nbins), 256);
     });
+    // the Stockham passes take R rows / B columns per block, the direct ones one
     add_stage(b->off_rows, b->n_rows, [&](int t) {
-        return (int)grid1d(b->hdesc[t].N1, batch_rows_per_block(b->hdesc[t].N2));
+        const BatchDesc &d = b->hdesc[t];
+        return batch_fft_n(d.N2) ? (int)grid1d(d.N1, batch_rows_per_block(d.N2)) : d.N1;
     });
     add_stage(b->off_cols, b->n_cols, [&](int t) {
-        return (int)grid1d(b->hdesc[t].L2, batch_cols_per_block(b->hdesc[t].N1));
+        const BatchDesc &d = b->hdesc[t];
+        return batch_fft_n(d.N1) ? (int)grid1d(d.L2, batch_cols_per_block(d.N1)) : d.L2;
     });
+    add_stage(b->off_merge, b->n_merge, [&](int t) { return b->hdesc[t].nbins; });
     b->bsubcap = std::max<long long>(subcap, 1);
     if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * count) != hipSuccess ||
         hipMalloc(&b->own, sizeof(int2) * own.size()) != hipSuccess ||
@@ -1370,6 +1525,66 @@ extern "C" int nfft2d_batch_type2(nfft2d_batch *b, void *stream)
     hipStream_t s = (hipStream_t)stream;
     rc = batch_grid(b, s);
     return rc ? rc : batch_gather(b, s);
+}
+
+// Type-1 of every transform: its M[t] samples data[t] at its sorted points
+// to the N1 x N2 modes result[t] (NFFT2_2D_GPU per transform).  set_io_type1
+// allocates each plan's spread tiles (the tile merge needs at least 3 bins
+// and 44 cells per dimension: N1, N2 >= 22).
+extern "C" int nfft2d_batch_set_io_type1(nfft2d_batch *b, const nfft2d_c64 *const *data, nfft2d_c64 *const *result)
+{
+    if (!b || !data || !result) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_io_type1: null argument");
+    int rc = nfft2d_check_device(b->device);
+    if (rc) return rc;
+    for (int t = 0; t < b->count; ++t) {
+        if (!result[t] || (b->hdesc[t].M > 0 && !data[t]))
+            return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_io_type1: transform %d: null array", t);
+        if (!mfma_spread_ok(b->plans[t]))
+            return nfft2d_fail(NFFT2D_ERR_SIZE, "batch_set_io_type1: transform %d: N1, N2 must be at least 22", t);
+    }
+    for (int t = 0; t < b->count; ++t) {
+        nfft2d_plan *p = b->plans[t];
+        if (p->subcap > p->scratchcap) {
+            free_scratch(p);
+            if (hipMalloc(&p->scratch, sizeof(float2) * kTileCells * (size_t)p->subcap) != hipSuccess)
+                return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_set_io_type1: hipMalloc of the spread tiles failed");
+            p->scratchcap = p->subcap;
+        }
+        b->hdesc[t].scratch = (float2 *)p->scratch;
+        b->hdesc[t].d1 = reinterpret_cast<const float2 *>(data[t]);
+        b->hdesc[t].F1 = reinterpret_cast<float2 *>(result[t]);
+    }
+    b->have_io1 = true;
+    return batch_upload(b);
+}
+
+extern "C" int nfft2d_batch_type1(nfft2d_batch *b, void *stream)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_type1: null batch");
+    if (!b->have_io1) return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type1: nfft2d_batch_set_io_type1 was not called");
+    if (!b->sorted)
+        return nfft2d_fail(NFFT2D_ERR_STATE, "batch_type1: the points are not sorted (call nfft2d_batch_setpts after "
+                                             "nfft2d_batch_set_points / nfft2d_batch_set_maxpts)");
+    int rc = nfft2d_check_device(b->device);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(kb_spread, dim3((unsigned)b->bsubcap), dim3(256), 0, s, b->ddesc, b->bsub, b->base, b->count,
+                       b->plans[0]->wa);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_spread_merge, dim3(b->n_merge), dim3(256), 0, s, b->ddesc, b->own + b->off_merge);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_t1_cols, dim3(b->n_cols), dim3(256), 0, s, b->ddesc, b->own + b->off_cols);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_t1_rows, dim3(b->n_rows), dim3(128), 0, s, b->ddesc, b->own + b->off_rows);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+// setpts then type-1 (the spread needs the sorted points; one stream).
+extern "C" int nfft2d_batch_setpts_type1(nfft2d_batch *b, void *stream)
+{
+    int rc = nfft2d_batch_setpts(b, stream);
+    return rc ? rc : nfft2d_batch_type1(b, stream);
 }
 
 // setpts + type-2 of the batch with the bin sort on the batch's second stream,

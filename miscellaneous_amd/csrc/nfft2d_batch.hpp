@@ -1,15 +1,16 @@
 // nfft2d_batch.hpp -- many type-2 NFFTs of different sizes in one launch per stage.
 //
 // A batch holds one plan per transform (complex64, K = 6, c = 2, N1 and N2
-// powers of two up to 1024).  Every stage of setpts and of the type-2 runs as
+// even, up to 1024; powers of two from 8 run the Stockham passes, other sizes
+// the direct ones).  Every stage of setpts and of the type-2 runs as
 // ONE kernel over the blocks of all transforms: block b finds its transform
 // and its block index inside the transform in an owner table built when the
 // batch is created (the block counts of these stages depend on the sizes
 // only).  The gather is persistent: its blocks walk one list of the
 // subproblems of all transforms, each entry tagged with its transform.
-// Together: 5 launches for setpts and 3 for the type-2, whatever the number
-// of transforms, all stream-ordered and free of allocation, so the whole
-// sequence can be captured into one hipGraph.
+// Together: 5 launches for setpts, 4 for the type-2 and 4 for the type-1,
+// whatever the number of transforms, all stream-ordered and free of
+// allocation, so the whole sequence can be captured into one hipGraph.
 #pragma once
 
 #include "nfft2d_kernels.hpp"
@@ -33,10 +34,18 @@ struct BatchDesc {
     const float *x, *y;
     const float2* f;
     float2* out;
+    // type-1: samples in, modes out; spread tiles; tile-merge radii (merge_radius)
+    const float2* d1;
+    float2* F1;
+    float2* scratch;
+    int r1, r2;
 };
 
 // Largest N of the batched FFT passes.
 constexpr int kBatchMaxN = 1024;
+
+// Powers of two from 8 run the Stockham passes; other even N the direct ones.
+__host__ __device__ constexpr bool batch_fft_n(int N) { return N >= 8 && (N & (N - 1)) == 0; }
 
 template <typename F>
 __device__ __forceinline__ void batch_dispatch_n(int N, F&& f)
@@ -146,6 +155,7 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ 
         const int c = d.tot[i];
         const int2 e = d.gexcl[i], gb = d.gsum[i / kColBins];
         const int pos = gb.x + e.x, sp = base[o.x] + gb.y + e.y;
+        d.subbeg[i] = gb.y + e.y;                            // the bin's first subproblem in its transform
         const int ns = (c + d.maxpts - 1) / d.maxpts;
         for (int k = 0; k < ns; ++k)
             bsub[sp + k] = make_int4(i, pos + k * d.maxpts, min(d.maxpts, c - k * d.maxpts), o.x);
@@ -163,12 +173,24 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ 
     d.pts[p] = rec;
 }
 
-// ---------------------------------------------------------------- type-2
+// ---------------------------------------------------------------- grid passes
+// Row passes: 128 threads, block o.y of the transform's rows (R rows of N2 for
+// the Stockham passes, one row for the direct ones); column passes: 256
+// threads, likewise over the L2 columns.  The LDS buffer fits the largest N
+// of either form (the direct passes stage one line of up to 2 kBatchMaxN).
+constexpr int kBatchLineLds = 2 * kBatchMaxN;
+constexpr int batch_lds_rows() { return batch_rows_lds() > kBatchLineLds ? batch_rows_lds() : kBatchLineLds; }
+constexpr int batch_lds_cols() { return batch_cols_lds() > kBatchLineLds ? batch_cols_lds() : kBatchLineLds; }
+
 __global__ __launch_bounds__(128) void kb_t2_rows(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
 {
-    __shared__ float2 lds[batch_rows_lds()];
+    __shared__ float2 lds[batch_lds_rows()];
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
+    if (!batch_fft_n(d.N2)) {
+        t2_rows_direct_body<float, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1, d.N2);
+        return;
+    }
     batch_dispatch_n(d.N2, [&](auto n) {
         constexpr int N = decltype(n)::value;
         t2_rows_body<float, N, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1);
@@ -177,12 +199,74 @@ __global__ __launch_bounds__(128) void kb_t2_rows(const BatchDesc* __restrict__ 
 
 __global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
 {
-    __shared__ float2 lds[batch_cols_lds()];
+    __shared__ float2 lds[batch_lds_cols()];
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
+    if (!batch_fft_n(d.N1)) {
+        t2_cols_direct_body<float, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.N1, d.L2);
+        return;
+    }
     batch_dispatch_n(d.N1, [&](auto n) {
         constexpr int N = decltype(n)::value;
         t2_cols_body<float, N, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.L2);
+    });
+}
+
+// ---------------------------------------------------------------- type-1
+// Spread: one block per entry of the batch's subproblem list (base[count]
+// entries), the plan kernel's split-FP16 MFMA tiles into each transform's
+// own tile scratch, indexed by the subproblem's place in its transform.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void kb_spread(
+    const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub, const int* __restrict__ base, int count, WinArgs wa)
+{
+    const int s = blockIdx.x;
+    if (s >= base[count]) return;
+    const int4 e = bsub[s];
+    const BatchDesc& t = bd[e.w];
+    spread_halo_body<false>(s - base[e.w], make_int4(e.x, e.y, e.z, 0), t.pts, t.d1, t.subbeg, t.tot, t.maxpts, t.geo,
+                            wa, t.scratch);
+}
+
+// Merge: one block per bin of each transform (owner table), summing the
+// neighbouring tiles as k_spread_merge does.
+__global__ __launch_bounds__(256) void kb_spread_merge(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& t = bd[o.x];
+    const int nb2 = t.nbins / t.geo.nb1;
+    if (t.r1 == 1 && t.r2 == 1) spread_merge_body<float, 1, 1>(o.y, t.scratch, t.subbeg, t.geo, nb2, t.Gt);
+    else if (t.r1 == 1) spread_merge_body<float, 1, 2>(o.y, t.scratch, t.subbeg, t.geo, nb2, t.Gt);
+    else if (t.r2 == 1) spread_merge_body<float, 2, 1>(o.y, t.scratch, t.subbeg, t.geo, nb2, t.Gt);
+    else spread_merge_body<float, 2, 2>(o.y, t.scratch, t.subbeg, t.geo, nb2, t.Gt);
+}
+
+__global__ __launch_bounds__(256) void kb_t1_cols(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    __shared__ float2 lds[batch_lds_cols()];
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    if (!batch_fft_n(d.N1)) {
+        t1_cols_direct_body<float, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.N1, d.L2);
+        return;
+    }
+    batch_dispatch_n(d.N1, [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        t1_cols_body<float, N, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.L2);
+    });
+}
+
+__global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
+{
+    __shared__ float2 lds[batch_lds_rows()];
+    const int2 o = own[blockIdx.x];
+    const BatchDesc& d = bd[o.x];
+    if (!batch_fft_n(d.N2)) {
+        t1_rows_direct_body<float, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1, d.N2);
+        return;
+    }
+    batch_dispatch_n(d.N2, [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        t1_rows_body<float, N, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1);
     });
 }
 

@@ -176,20 +176,19 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t2_cols(
 // ================================================================= type-1 passes
 // Column pass: Tp[k1][n2] = sum_{n1} Gt[n2][n1] e^{SIGN 2 pi i n1 k1 / L1} for
 // k1 in [-N1/2, N1/2) only (the reference crops the rest, NFFT/NFFT2_2D.cu:74-86).
+template <typename T, int N> constexpr int t1_cols_lds() { return cols_per_block<N>() * 2 * lds_line<N>(); }
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t1_cols(
-    const cplx<T>* __restrict__ Gt, cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int L2)
+__device__ __forceinline__ void t1_cols_body(int blk, cplx<T>* lds, const cplx<T>* __restrict__ Gt,
+                                             cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int L2)
 {
     using C = cplx<T>;
     constexpr int B = cols_per_block<N>();
     constexpr int TPL = N / 8, LL = lds_line<N>(), V = 2;
     constexpr int R0 = FftPlan<N>::radix(0);
     constexpr int RL = FftPlan<N>::radix(FftPlan<N>::stages - 1);
-    __shared__ C lds[B * V * LL];
     const int t = threadIdx.x;
     const int bF = t / TPL, jF = t % TPL;  // first stage: whole grid rows (contiguous)
     const int bL = t % B, jL = t / B;      // last stage: B outputs side by side
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int n2F = min(blk * B + bF, L2 - 1);
     const C* in = Gt + (size_t)n2F * (2 * N);
     C v[V][8];
@@ -214,22 +213,29 @@ __global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t1_cols(
             }
     }
 }
+template <typename T, int N, int SIGN>
+__global__ __launch_bounds__(cols_per_block<N>() * (N / 8)) void k_t1_cols(
+    const cplx<T>* __restrict__ Gt, cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int L2)
+{
+    __shared__ cplx<T> lds[t1_cols_lds<T, N>()];
+    t1_cols_body<T, N, SIGN>(xcd_block(blockIdx.x, gridDim.x), lds, Gt, Tp, tw, L2);
+}
 
 // Row pass + crop + deconvolution: F[k1][k2] = D1[k1] D2[k2] sum_{n2} Tp[k1][n2] e^{SIGN 2 pi i n2 k2 / L2}
 // (NFFT/NFFT2_2D.cu:74-100 Decimation_and_Scaling fused into the last FFT stage).
+template <typename T, int N> constexpr int t1_rows_lds() { return rows_per_block<N>() * 2 * lds_line<N>(); }
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t1_rows(
-    const cplx<T>* __restrict__ Tp, cplx<T>* __restrict__ F, const T* __restrict__ D1,
-    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
+__device__ __forceinline__ void t1_rows_body(int blk, cplx<T>* lds, const cplx<T>* __restrict__ Tp,
+                                             cplx<T>* __restrict__ F, const T* __restrict__ D1,
+                                             const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
 {
     using C = cplx<T>;
     constexpr int ROWS = rows_per_block<N>();
     constexpr int TPL = N / 8, LL = lds_line<N>(), V = 2;
     constexpr int R0 = FftPlan<N>::radix(0);
     constexpr int RL = FftPlan<N>::radix(FftPlan<N>::stages - 1);
-    __shared__ C lds[ROWS * V * LL];
     const int g = threadIdx.x / TPL, j = threadIdx.x % TPL;
-    const int i1 = blockIdx.x * ROWS + g;
+    const int i1 = blk * ROWS + g;
     const int row = i1 < N1 ? i1 : N1 - 1;
     const C* in = Tp + (size_t)row * (2 * N);
     C v[V][8];
@@ -255,6 +261,14 @@ __global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t1_rows(
             }
     }
 }
+template <typename T, int N, int SIGN>
+__global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t1_rows(
+    const cplx<T>* __restrict__ Tp, cplx<T>* __restrict__ F, const T* __restrict__ D1,
+    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1)
+{
+    __shared__ cplx<T> lds[t1_rows_lds<T, N>()];
+    t1_rows_body<T, N, SIGN>(blockIdx.x, lds, Tp, F, D1, D2, tw, N1);
+}
 
 // ================================================================= direct-DFT passes
 // Same four passes for any even N (not a power of two, or N < 8): each output
@@ -262,14 +276,13 @@ __global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t1_rows(
 // (k n) mod L into the 2N-entry table.  O(N^2) per line; used for the
 // reference's non-power-of-two sizes (e.g. 2Nu = 88 in Synthesis/PSO).
 template <typename T, int SIGN>
-__global__ __launch_bounds__(256) void k_t2_rows_direct(
-    const cplx<T>* __restrict__ f, cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
-    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
+__device__ __forceinline__ void t2_rows_direct_body(int i1, cplx<T>* line, const cplx<T>* __restrict__ f,
+                                                    cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
+                                                    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1,
+                                                    int N2)
 {
     using C = cplx<T>;
-    extern __shared__ unsigned char smem_raw[];
-    C* line = reinterpret_cast<C*>(smem_raw);
-    const int i1 = blockIdx.x, L2 = 2 * N2;
+    const int L2 = 2 * N2;
     for (int i2 = threadIdx.x; i2 < N2; i2 += blockDim.x)
         line[i2] = cscale(f[(size_t)i1 * N2 + i2], D1[i1] * D2[i2]);
     __syncthreads();
@@ -285,13 +298,21 @@ __global__ __launch_bounds__(256) void k_t2_rows_direct(
 }
 
 template <typename T, int SIGN>
-__global__ __launch_bounds__(256) void k_t2_cols_direct(
-    const cplx<T>* __restrict__ Tm, cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int N1, int L2)
+__global__ __launch_bounds__(256) void k_t2_rows_direct(
+    const cplx<T>* __restrict__ f, cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
+    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
+{
+    extern __shared__ unsigned char smem_raw[];
+    t2_rows_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), f, Tm, D1, D2, tw, N1, N2);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void t2_cols_direct_body(int n2, cplx<T>* line, const cplx<T>* __restrict__ Tm,
+                                                    cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int N1,
+                                                    int L2)
 {
     using C = cplx<T>;
-    extern __shared__ unsigned char smem_raw[];
-    C* line = reinterpret_cast<C*>(smem_raw);
-    const int n2 = blockIdx.x, L1 = 2 * N1;
+    const int L1 = 2 * N1;
     for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) line[i1] = Tm[(size_t)i1 * L2 + n2];
     __syncthreads();
     for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x) {
@@ -306,13 +327,20 @@ __global__ __launch_bounds__(256) void k_t2_cols_direct(
 }
 
 template <typename T, int SIGN>
-__global__ __launch_bounds__(256) void k_t1_cols_direct(
-    const cplx<T>* __restrict__ Gt, cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int N1, int L2)
+__global__ __launch_bounds__(256) void k_t2_cols_direct(
+    const cplx<T>* __restrict__ Tm, cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int N1, int L2)
+{
+    extern __shared__ unsigned char smem_raw[];
+    t2_cols_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Tm, Gt, tw, N1, L2);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void t1_cols_direct_body(int n2, cplx<T>* line, const cplx<T>* __restrict__ Gt,
+                                                    cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int N1,
+                                                    int L2)
 {
     using C = cplx<T>;
-    extern __shared__ unsigned char smem_raw[];
-    C* line = reinterpret_cast<C*>(smem_raw);
-    const int n2 = blockIdx.x, L1 = 2 * N1;
+    const int L1 = 2 * N1;
     for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x) line[n1] = Gt[(size_t)n2 * L1 + n1];
     __syncthreads();
     for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) {
@@ -325,14 +353,21 @@ __global__ __launch_bounds__(256) void k_t1_cols_direct(
 }
 
 template <typename T, int SIGN>
-__global__ __launch_bounds__(256) void k_t1_rows_direct(
-    const cplx<T>* __restrict__ Tp, cplx<T>* __restrict__ F, const T* __restrict__ D1,
-    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
+__global__ __launch_bounds__(256) void k_t1_cols_direct(
+    const cplx<T>* __restrict__ Gt, cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int N1, int L2)
+{
+    extern __shared__ unsigned char smem_raw[];
+    t1_cols_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Gt, Tp, tw, N1, L2);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void t1_rows_direct_body(int i1, cplx<T>* line, const cplx<T>* __restrict__ Tp,
+                                                    cplx<T>* __restrict__ F, const T* __restrict__ D1,
+                                                    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1,
+                                                    int N2)
 {
     using C = cplx<T>;
-    extern __shared__ unsigned char smem_raw[];
-    C* line = reinterpret_cast<C*>(smem_raw);
-    const int i1 = blockIdx.x, L2 = 2 * N2;
+    const int L2 = 2 * N2;
     for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x) line[n2] = Tp[(size_t)i1 * L2 + n2];
     __syncthreads();
     for (int i2 = threadIdx.x; i2 < N2; i2 += blockDim.x) {
@@ -342,6 +377,15 @@ __global__ __launch_bounds__(256) void k_t1_rows_direct(
         for (int n2 = 0; n2 < L2; ++n2) acc = cadd(acc, cmul(line[n2], twid<SIGN>(tw, (int)((kk * n2) % L2))));
         F[(size_t)i1 * N2 + i2] = cscale(acc, D1[i1] * D2[i2]);
     }
+}
+
+template <typename T, int SIGN>
+__global__ __launch_bounds__(256) void k_t1_rows_direct(
+    const cplx<T>* __restrict__ Tp, cplx<T>* __restrict__ F, const T* __restrict__ D1,
+    const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
+{
+    extern __shared__ unsigned char smem_raw[];
+    t1_rows_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Tp, F, D1, D2, tw, N1, N2);
 }
 
 // ================================================================= setpts (bin sort)
@@ -1499,20 +1543,19 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo)
 // (A bin just over maxpts points then costs one block, not a second, nearly
 // empty one.)
 constexpr int kSpUnit = 2;
+// Subproblem s (entry d = (bin, first point, count)) of a plan whose bins'
+// first subproblems are subbeg and point totals tot; its tile goes to
+// scratch[s].
 template <bool SORTED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_spread_halo(const PtRec<float>* __restrict__ pts,
-                                                     const float2* __restrict__ data, const int4* __restrict__ sub,
-                                                     const int* __restrict__ nsub, const int* __restrict__ subbeg,
-                                                     const int* __restrict__ tot, int maxpts, BinGeom geo,
-                                                     WinArgs wa, float2* __restrict__ scratch)
+__device__ __forceinline__ void spread_halo_body(int s, int4 d, const PtRec<float>* __restrict__ pts,
+                                                 const float2* __restrict__ data, const int* __restrict__ subbeg,
+                                                 const int* __restrict__ tot, int maxpts, const BinGeom& geo,
+                                                 const WinArgs& wa, float2* __restrict__ scratch)
 {
     __shared__ __attribute__((aligned(16))) union {
         SpreadWaveLds w[4];
         float red[4 * 32 * 32 * 2];   // [w][cell(32x32)][re, im], after the products
     } sm;
-    const int s = blockIdx.x;
-    if (s >= *nsub) return;
-    const int4 d = sub[s];
     const int k = s - subbeg[d.x];                          // index of subproblem s in its bin
     if (k % kSpUnit != 0) return;
     const int cnt = min(kSpUnit * maxpts, tot[d.x] - k * maxpts);   // points of the unit from d.y on
@@ -1634,6 +1677,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
         out[e] = make_float2(sr, si);
     }
+}
+template <bool SORTED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_spread_halo(
+    const PtRec<float>* __restrict__ pts, const float2* __restrict__ data, const int4* __restrict__ sub,
+    const int* __restrict__ nsub, const int* __restrict__ subbeg, const int* __restrict__ tot, int maxpts, BinGeom geo,
+    WinArgs wa, float2* __restrict__ scratch)
+{
+    const int s = blockIdx.x;
+    if (s >= *nsub) return;
+    spread_halo_body<SORTED>(s, sub[s], pts, data, subbeg, tot, maxpts, geo, wa, scratch);
 }
 
 // The complex128 form of k_spread_halo: the same tile
@@ -1769,14 +1822,13 @@ __global__ __launch_bounds__(256) void k_spread_halo64(const PtRec<double>* __re
 __host__ __device__ constexpr int merge_radius(int L) { return (L % kBin != 0 && L % kBin < kK) ? 2 : 1; }
 
 template <typename T, int R1, int R2>
-__global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict__ scratch,
-                                                      const int* __restrict__ subbeg, BinGeom geo, int nb2,
-                                                      cplx<T>* __restrict__ Gt)
+__device__ __forceinline__ void spread_merge_body(int bin, const cplx<T>* __restrict__ scratch,
+                                                  const int* __restrict__ subbeg, const BinGeom& geo, int nb2,
+                                                  cplx<T>* __restrict__ Gt)
 {
     using C = cplx<T>;
     constexpr int W1 = 2 * R1 + 1, W2 = 2 * R2 + 1, NN = W1 * W2;
     const int nb1 = geo.nb1;
-    const int bin = blockIdx.x;
     const int b1 = bin % nb1, b2 = bin / nb1;
     const int n2 = b2 * kBin + threadIdx.x / kBin, n1 = b1 * kBin + threadIdx.x % kBin;
     if (n2 >= geo.L2 || n1 >= geo.L1) return;
@@ -1820,6 +1872,13 @@ __global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict_
         }
     }
     Gt[(size_t)n2 * geo.L1 + n1] = mk(sr, si);
+}
+template <typename T, int R1, int R2>
+__global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict__ scratch,
+                                                      const int* __restrict__ subbeg, BinGeom geo, int nb2,
+                                                      cplx<T>* __restrict__ Gt)
+{
+    spread_merge_body<T, R1, R2>(blockIdx.x, scratch, subbeg, geo, nb2, Gt);
 }
 
 }  // namespace nfft2d

@@ -12,6 +12,9 @@ algorithm:
 * ``nfft2_2d``  -- type-1 (non-uniform -> uniform), ``NFFT/NFFT2_2D.cu:329-495``
   and its MATLAB twin ``NFFT/NFFT2/Matlab/NFFT2_2D.m``.
 * ``ndft2_2d``  -- the direct NED sum, ``NFFT/NDFT2_2D.cu:28-29,67``.
+* ``nfft1_1d`` / ``nfft2_1d`` -- the 1-D type-2 (``BP_FBP_FFBP/NFFT1_1D.m``) and type-1
+  (``NFFT/NFFT2/1D/Matlab/NFFT2_1D.m``), with their direct sums ``ndft1d_type2`` /
+  ``ndft1d_type1`` (the closed forms of ``NFFT1_1D_BLAS.m`` / ``NFFT2_1D_BLAS.m``).
 * ``ndft_type2`` / ``ndft_type1`` -- direct FP64 sums of both directions (the
   closed forms every fast result is checked against).
 * ``window``, ``window_deriv``, ``bessel_i0``, ``spectral_window``, ``fft2`` --
@@ -34,6 +37,12 @@ Pins (what checks each function, in ``tests/test_oracle.py``):
   (``NFFT/NFFT2/Matlab/Test_NFFT2_2D.m:24-29``).
 * ``ndft2_2d`` / ``ndft_type2`` / ``ndft_type1``: brute-force numpy sums
   written independently in the golden script, and each other.
+* ``ndft_type1_modes``: the golden type-1 sums at seeded modes (square and
+  non-square, where swapped indices or periods fail).
+* ``ndft1d_type2`` / ``ndft1d_type1``: numpy brute-force sums of the BLAS
+  forms; ``nfft1_1d`` / ``nfft2_1d``: those direct sums (the algorithm's
+  error at c = 2, K = 6 is ~1e-12), the adjoint identity between them, and
+  the 2-D restatement on one mode row.
 """
 from __future__ import annotations
 
@@ -92,6 +101,14 @@ def _load():
     lib.orc_ndft_type1.argtypes = [_dp] * 4 + [ctypes.c_int] * 3
     lib.orc_ndft_type1_modes.restype = None
     lib.orc_ndft_type1_modes.argtypes = [_dp] * 4 + [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2 + [ctypes.c_int]
+    for name in ("orc_nfft1_1d", "orc_nfft2_1d"):
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = [_dp] * 3 + [ctypes.c_int] * 5
+    for name in ("orc_ndft1d_type2", "orc_ndft1d_type1"):
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = [_dp] * 3 + [ctypes.c_int] * 2
     lib.orc_num_threads.restype = ctypes.c_int
     lib.orc_set_num_threads.argtypes = [ctypes.c_int]
     _lib = lib
@@ -255,6 +272,53 @@ def ndft_type1_modes(d: np.ndarray, x1, x2, N1: int, N2: int, i1, i2) -> np.ndar
                                  _ptr(x1, ctypes.c_double), _ptr(x2, ctypes.c_double), N1, N2, len(x1),
                                  _ptr(i1, ctypes.c_int), _ptr(i2, ctypes.c_int), len(i1))
     return F
+
+
+def _line(a, dtype=np.complex128):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.view(np.float64)
+
+
+def nfft1_1d(data: np.ndarray, x, K: int = 6, c: int = 2, round_mode: int = 1) -> np.ndarray:
+    """Type-2 1-D NFFT, BP_FBP_FFBP/NFFT1_1D.m: y_l ~ sum_k data_k e^{-2 pi i k x_l / N}
+    (data: N modes, k at index k + N/2; x in grid units).  round_mode 1 = MATLAB round."""
+    d, dv = _line(data)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x), dtype=np.complex128)
+    _load().orc_nfft1_1d(_ptr(out.view(np.float64), ctypes.c_double), _ptr(dv, ctypes.c_double),
+                         _ptr(x, ctypes.c_double), len(d), len(x), K, c, round_mode)
+    return out
+
+
+def nfft2_1d(data: np.ndarray, x, N: int, K: int = 6, c: int = 2, round_mode: int = 1) -> np.ndarray:
+    """Type-1 1-D NFFT, NFFT/NFFT2/1D/Matlab/NFFT2_1D.m (called there as NFFT2_1D(data, x, N, c, K)):
+    F_k ~ sum_l data_l e^{-2 pi i k x_l / N}, k in [-N/2, N/2) at index k + N/2."""
+    d, dv = _line(data)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(N, dtype=np.complex128)
+    _load().orc_nfft2_1d(_ptr(out.view(np.float64), ctypes.c_double), _ptr(dv, ctypes.c_double),
+                         _ptr(x, ctypes.c_double), N, len(x), K, c, round_mode)
+    return out
+
+
+def ndft1d_type2(f: np.ndarray, x) -> np.ndarray:
+    """Direct FP64 1-D type-2 sum (NFFT/NFFT1/1D/Matlab/NFFT1_1D_BLAS.m in grid units)."""
+    fd, fv = _line(f)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x), dtype=np.complex128)
+    _load().orc_ndft1d_type2(_ptr(out.view(np.float64), ctypes.c_double), _ptr(fv, ctypes.c_double),
+                             _ptr(x, ctypes.c_double), len(fd), len(x))
+    return out
+
+
+def ndft1d_type1(d: np.ndarray, x, N: int) -> np.ndarray:
+    """Direct FP64 1-D type-1 sum (NFFT/NFFT2/1D/Matlab/NFFT2_1D_BLAS.m in grid units)."""
+    dd, dv = _line(d)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(N, dtype=np.complex128)
+    _load().orc_ndft1d_type1(_ptr(out.view(np.float64), ctypes.c_double), _ptr(dv, ctypes.c_double),
+                             _ptr(x, ctypes.c_double), N, len(x))
+    return out
 
 
 def rel_l2(a, b) -> float:

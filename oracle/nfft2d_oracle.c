@@ -299,3 +299,137 @@ void orc_set_num_threads(int n)
     (void)n;
 #endif
 }
+
+/* -------------------------------------------------------------------------- */
+/* 1-D transforms (FP64), one line of a batch per call.                          */
+
+/* Type-2, BP_FBP_FFBP/NFFT1_1D.m step by step (c = 2, K = 6 there):
+ *   phi_k = I0(K sqrt(alpha^2 - xi_k^2)) / I0(K alpha), xi_k = 2 pi k / (c N)   :12-14
+ *   mu_l = round(c x_l)                                                           :20
+ *   spectrum_phi[l][m] = phi_hat(c x_l - (mu_l + m)) / I0(K alpha), m = -K..K   :22-30
+ *   u = zeros(c N); u[(c-1) N/2 .. (c+1) N/2) = data ./ phi                      :34-35
+ *   U = fft(ifftshift(u))                                                         :41
+ *   result_l = sum_m spectrum_phi[l][m] U[(mu_l + m) mod c N]                     :47-48
+ * data: N complex (mode k at index k + N/2), x: M grid-unit coordinates
+ * (period N), result: M complex. */
+void orc_nfft1_1d(double *result, const double *data, const double *x, int N, int M, int K, int c,
+                  int round_mode)
+{
+    const double alpha = orc_alpha(c), S = orc_bessel_i0((double)K * alpha);
+    const long L = (long)c * N;
+    double *phi = (double *)malloc(sizeof(double) * N);
+    double *u = (double *)calloc(2 * L, sizeof(double));
+    double *U = (double *)malloc(sizeof(double) * 2 * L);
+    double *buf = (double *)malloc(sizeof(double) * 2 * L), *tmp = (double *)malloc(sizeof(double) * 2 * L);
+    orc_spectral_window(phi, N, K, c);
+    for (int i = 0; i < N; ++i) {
+        const long j = (long)(c - 1) * N / 2 + i;
+        u[2 * j] = data[2 * i] / (phi[i] / S);
+        u[2 * j + 1] = data[2 * i + 1] / (phi[i] / S);
+    }
+    /* ifftshift: U[n] = u[(n + L/2) mod L], then the forward DFT */
+    for (long n = 0; n < L; ++n) {
+        const long s = (n + L / 2) % L;
+        U[2 * n] = u[2 * s];
+        U[2 * n + 1] = u[2 * s + 1];
+    }
+    line_dft_d(U, L, 1, -1, buf, tmp);
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < M; ++l) {
+        const double cx = (double)c * x[l], mu = orc_round(cx, round_mode);
+        double sr = 0, si = 0;
+        for (int m = -K; m <= K; ++m) {
+            const double w = orc_window(cx - (mu + m), K, alpha) / S;
+            const long n = pos_mod((long)mu + m, L);
+            sr += w * U[2 * n];
+            si += w * U[2 * n + 1];
+        }
+        result[2 * l] = sr;
+        result[2 * l + 1] = si;
+    }
+    free(phi); free(u); free(U); free(buf); free(tmp);
+}
+
+/* Type-1, NFFT/NFFT2/1D/Matlab/NFFT2_1D.m step by step: M samples data_l at
+ * x_l (grid units, period N) to N outputs:
+ *   phi_k = I0(K sqrt(alpha^2 - xi_k^2)), xi_k = 2 pi k / (c N) (not normalised) :22-24
+ *   u[k + c N/2] += data_l phi_hat(c x_l - k) for every k in [-c N/2, c N/2) with
+ *       |round(c x_l) - k| <= K, or the same test one period c N up or down     :59-78
+ *   FFT = fftshift(fft(ifftshift(u)))                                             :85
+ *   result = FFT[(c-1) N/2 .. (c+1) N/2) ./ phi                                   :91 */
+void orc_nfft2_1d(double *result, const double *data, const double *x, int N, int M, int K, int c,
+                  int round_mode)
+{
+    const double alpha = orc_alpha(c);
+    const long L = (long)c * N;
+    double *phi = (double *)malloc(sizeof(double) * N);
+    double *u = (double *)calloc(2 * L, sizeof(double));
+    double *U = (double *)malloc(sizeof(double) * 2 * L);
+    double *buf = (double *)malloc(sizeof(double) * 2 * L), *tmp = (double *)malloc(sizeof(double) * 2 * L);
+    orc_spectral_window(phi, N, K, c);
+    for (int l = 0; l < M; ++l) {
+        const double cx = (double)c * x[l], r = orc_round(cx, round_mode);
+        for (long k = -L / 2; k < L / 2; ++k) {
+            double d = 0;
+            int hit = 1;
+            if (fabs(r - (double)k) <= K) d = cx - (double)k;
+            else if (fabs(r + (double)L - (double)k) <= K) d = cx - (double)k + (double)L;
+            else if (fabs(r - (double)L - (double)k) <= K) d = cx - (double)k - (double)L;
+            else hit = 0;
+            if (!hit) continue;
+            const double w = orc_window(d, K, alpha);
+            const long j = k + L / 2;
+            u[2 * j] += data[2 * l] * w;
+            u[2 * j + 1] += data[2 * l + 1] * w;
+        }
+    }
+    for (long n = 0; n < L; ++n) {               /* ifftshift */
+        const long s = (n + L / 2) % L;
+        U[2 * n] = u[2 * s];
+        U[2 * n + 1] = u[2 * s + 1];
+    }
+    line_dft_d(U, L, 1, -1, buf, tmp);
+    for (int i = 0; i < N; ++i) {                /* fftshift, crop, deconvolve */
+        const long j = (long)(c - 1) * N / 2 + i;      /* index in the shifted spectrum */
+        const long n = (j + L / 2) % L;                /* its unshifted position */
+        result[2 * i] = U[2 * n] / phi[i];
+        result[2 * i + 1] = U[2 * n + 1] / phi[i];
+    }
+    free(phi); free(u); free(U); free(buf); free(tmp);
+}
+
+/* Direct 1-D sums, the closed forms of NFFT/NFFT1/1D/Matlab/NFFT1_1D_BLAS.m and
+ * NFFT/NFFT2/1D/Matlab/NFFT2_1D_BLAS.m in grid units:
+ *   type-2: y_l = sum_k f_k e^{-2 pi i k x_l / N}   (f: N, k at index k + N/2)
+ *   type-1: F_k = sum_l d_l e^{-2 pi i k x_l / N}   (F: N)                     */
+void orc_ndft1d_type2(double *y, const double *f, const double *x, int N, int M)
+{
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < M; ++l) {
+        double sr = 0, si = 0;
+        for (int i = 0; i < N; ++i) {
+            double cr, ci;
+            cis_neg((double)(i - N / 2) * x[l] / (double)N, &cr, &ci);
+            sr += f[2 * i] * cr - f[2 * i + 1] * ci;
+            si += f[2 * i] * ci + f[2 * i + 1] * cr;
+        }
+        y[2 * l] = sr;
+        y[2 * l + 1] = si;
+    }
+}
+
+void orc_ndft1d_type1(double *F, const double *d, const double *x, int N, int M)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double sr = 0, si = 0;
+        for (int l = 0; l < M; ++l) {
+            double cr, ci;
+            cis_neg((double)(i - N / 2) * x[l] / (double)N, &cr, &ci);
+            sr += d[2 * l] * cr - d[2 * l + 1] * ci;
+            si += d[2 * l] * ci + d[2 * l + 1] * cr;
+        }
+        F[2 * i] = sr;
+        F[2 * i + 1] = si;
+    }
+}

@@ -102,9 +102,9 @@ def _ints(v):
 
 
 @pytest.mark.parametrize("N1,N2,M,code,needle", [
-    ([64, 48], [64, 64], [10, 10], -2, "powers of two"),
+    ([64, 63], [64, 64], [10, 10], -2, "even"),
     ([64, 2048], [64, 64], [10, 10], -2, "1024"),
-    ([64, 4], [64, 64], [10, 10], -2, "[8,"),
+    ([64, 0], [64, 64], [10, 10], -2, "[2,"),
     ([64, 64], [64, 64], [10, -1], -1, "outside [0, "),
 ])
 def test_batch_create_validation(lib, N1, N2, M, code, needle):
@@ -129,6 +129,20 @@ def test_batch_null_arguments_rejected(lib):
     assert lib.nfft2d_batch_set_maxpts(None, 0, 128) == -1
     assert lib.nfft2d_batch_get_maxpts(None, 0) == -1
     assert lib.nfft2d_batch_count(None) == -1
+    assert lib.nfft2d_batch_set_io_type1(None, None, None) == -1
+    assert lib.nfft2d_batch_type1(None, None) == -1
+    assert lib.nfft2d_batch_setpts_type1(None, None) == -1
+
+
+def test_1d_plan_arguments_rejected(lib):
+    h = ctypes.c_void_p()
+    assert lib.nfft1d_plan_create(None, 64) == -1
+    for bad in (7, 0, -2, 4098, 3000):
+        assert lib.nfft1d_plan_create(ctypes.byref(h), bad) == -2
+        assert not h.value
+    assert lib.nfft1d_plan_destroy(None) == -1
+    assert lib.nfft1d_type2(None, None, None, None, 10, 1, None) == -1
+    assert lib.nfft1d_type1(None, None, None, None, 10, 1, None) == -1
 
 
 def test_missing_library_fails_loudly(tmp_path):

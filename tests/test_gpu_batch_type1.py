@@ -1,0 +1,140 @@
+"""GPU parity of the batched type-1 (the reference's NFFT2_2D_GPU per transform) and of
+batches with sizes that are not powers of two.
+
+The batch runs the plans' kernels (split-FP16 MFMA spread, tile merge, FFT passes), but its
+bin sort ranks a bin's points in the order the waves reach the bin's counter, as a plan's
+does, so a type-1 sum may round differently between runs: each transform is held to rel-L2
+1e-6 against its own plan and 1e-5 against the FP64 direct sum.  Type-2 outputs are
+per-point and stay bitwise equal to the plans.
+"""
+import numpy as np
+import pytest
+import torch
+
+import miscellaneous_amd as m
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def cn01(rng, shape):
+    return ((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)).astype(np.complex64)
+
+
+def pts(rng, N1, N2, M):
+    h = M // 2
+    x1 = np.concatenate([rng.uniform(-N1 / 2, N1 / 2, h), rng.normal(0, N1 / 20, M - h)]).astype(np.float32)
+    x2 = np.concatenate([rng.uniform(-N2 / 2, N2 / 2, h), rng.normal(0, N2 / 20, M - h)]).astype(np.float32)
+    return x1, x2
+
+
+SIZES1 = [(88, 88, 1936), (64, 64, 5000), (26, 34, 700), (128, 96, 20000), (22, 22, 1), (256, 256, 0),
+          (100, 64, 3000), (512, 512, 60000)]
+
+
+def test_batch_type1_mixed_sizes():
+    rng = np.random.default_rng(41)
+    xs, ys, ds = [], [], []
+    for (N1, N2, M) in SIZES1:
+        x1, x2 = pts(rng, N1, N2, M)
+        xs.append(T(x1)); ys.append(T(x2)); ds.append(T(cn01(rng, M)))
+    outs = [torch.empty(N1 * N2, dtype=torch.complex64, device=DEV) for (N1, N2, _) in SIZES1]
+    b = m.Batch(SIZES1, device=DEV).set_points(xs, ys).set_io_type1(ds, outs)
+    with pytest.raises(m.NFFT2DError, match="not sorted"):
+        b.type1()
+    b.setpts_type1()
+    torch.cuda.synchronize()
+    for t, (N1, N2, M) in enumerate(SIZES1):
+        print(f"case batch type-1 transform {t}: {N1}x{N2} M={M}", flush=True)
+        got = outs[t].cpu().numpy()
+        ref_plan = m.Plan(N1, N2, device=DEV).setpts(xs[t], ys[t]).type1(ds[t]).cpu().numpy().ravel()
+        if M == 0:
+            assert np.all(got == 0)
+            continue
+        assert oracle.rel_l2(got, ref_plan) < 1e-6, t
+        i1 = rng.integers(0, N1, 256)
+        i2 = rng.integers(0, N2, 256)
+        ref = oracle.ndft_type1_modes(ds[t].cpu().numpy().astype(np.complex128), xs[t].cpu().numpy(),
+                                      ys[t].cpu().numpy(), N1, N2, i1, i2)
+        assert oracle.rel_l2(got.reshape(N1, N2)[i1, i2], ref) < 1e-5, t
+
+
+def test_batch_type2_sizes_not_powers_of_two():
+    """Type-2 batches with direct FFT passes (88, 100, 26, 6 ...) next to Stockham ones: every
+    transform bitwise equal to its own plan, and 256 outputs within 1e-5 of the direct sum."""
+    rng = np.random.default_rng(42)
+    sizes = [(88, 88, 1936), (100, 64, 3000), (26, 34, 700), (6, 10, 50), (64, 64, 4000), (2, 8, 9)]
+    fs, xs, ys = [], [], []
+    for (N1, N2, M) in sizes:
+        x1, x2 = pts(rng, N1, N2, M)
+        xs.append(T(x1)); ys.append(T(x2)); fs.append(T(cn01(rng, (N1, N2))))
+    outs = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in sizes]
+    b = m.Batch(sizes, device=DEV).set_points(xs, ys).set_io(fs, outs)
+    b.setpts_type2()
+    torch.cuda.synchronize()
+    for t, (N1, N2, M) in enumerate(sizes):
+        plan = m.Plan(N1, N2, device=DEV).setpts(xs[t], ys[t])
+        assert torch.equal(plan.type2(fs[t]), outs[t]), t
+        sel = rng.choice(M, min(M, 256), replace=False)
+        ref = oracle.ndft_type2(fs[t].cpu().numpy().astype(np.complex128), xs[t].cpu().numpy()[sel],
+                                ys[t].cpu().numpy()[sel])
+        assert oracle.rel_l2(outs[t].cpu().numpy()[sel], ref) < 1e-5, t
+
+
+def test_batch_type1_refusals():
+    b = m.Batch([(16, 32, 100)], device=DEV)
+    d = [torch.zeros(100, dtype=torch.complex64, device=DEV)]
+    r = [torch.zeros(16 * 32, dtype=torch.complex64, device=DEV)]
+    with pytest.raises(m.NFFT2DError, match="at least 22"):
+        b.set_io_type1(d, r)
+    b2 = m.Batch([(32, 32, 100)], device=DEV)
+    with pytest.raises(m.NFFT2DError, match="set_io_type1"):
+        b2.type1()
+
+
+def test_pso_generation_far_fields():
+    """One PSO generation of Synthesis/PSO: 250 particles, Nu = Nv = 44 (88 x 88 far fields),
+    M = 44 x 44 = 1936 elements each, batched setpts + type-1 + filter captured in one graph and
+    replayed with new positions and excitations; every far field against the drop-in NFFT2_2D_GPU
+    times the filter, and 8 against the FP64 direct sum."""
+    rng = np.random.default_rng(43)
+    P, Nu, Nv, M = 250, 44, 44, 1936
+    gen = m.FarFieldGeneration(P, Nu, Nv, M, device=DEV)
+    filt = rng.uniform(0.5, 1.5, (2 * Nv) * (2 * Nu)).astype(np.float32)
+    gen.filter.copy_(T(filt))
+
+    def new_inputs():
+        g = np.stack(np.meshgrid(np.arange(44), np.arange(44)), -1).reshape(-1, 2).astype(np.float64)
+        for p in range(P):
+            jit = rng.uniform(-0.3, 0.3, (M, 2))
+            xy = (g - 21.5) * 1.9 + jit
+            gen.X[p].copy_(T(xy[:, 0].astype(np.float32)))
+            gen.Y[p].copy_(T(xy[:, 1].astype(np.float32)))
+        gen.excitations.copy_(T(cn01(rng, (P, M))))
+
+    new_inputs()
+    print(f"case PSO generation: {P} particles, {2 * Nv}x{2 * Nu}, M={M}", flush=True)
+    gen.capture()
+    for rep in range(2):
+        if rep:
+            new_inputs()
+        out = gen.replay().clone()
+        torch.cuda.synchronize()
+        worst = 0.0
+        for p in range(P):
+            ref = torch.empty(2 * Nv, 2 * Nu, dtype=torch.complex64, device=DEV)
+            m.NFFT2_2D_GPU(ref, gen.excitations[p], gen.X[p], gen.Y[p], 2 * Nv, 2 * Nu, M)
+            ref *= gen.filter.view(2 * Nv, 2 * Nu)
+            worst = max(worst, oracle.rel_l2(out[p].cpu().numpy(), ref.cpu().numpy()))
+        print(f"PSO replay {rep}: worst rel-L2 to the drop-in over {P} particles {worst:.2e}", flush=True)
+        assert worst < 1e-6
+        for p in rng.choice(P, 8, replace=False):
+            F = oracle.ndft_type1(gen.excitations[p].cpu().numpy().astype(np.complex128), gen.X[p].cpu().numpy(),
+                                  gen.Y[p].cpu().numpy(), 2 * Nv, 2 * Nu) * filt.reshape(2 * Nv, 2 * Nu)
+            assert oracle.rel_l2(out[p].cpu().numpy(), F) < 1e-5

@@ -262,3 +262,53 @@ def test_planted_mistakes_are_caught(golden, mutation):
     assert oracle.rel_l2(y, g["b_y"]) > 1e-2
     F = oracle.nfft2_2d(g["b_d"].astype(np.complex128), g["b_x1"], g["b_x2"], N1, N2, mutation=mutation)
     assert oracle.rel_l2(F, g["b_F"]) > 1e-2
+
+
+# ------------------------------------------------------------------ 1-D transforms
+def _numpy_ndft1d(N, x, f=None, d=None):
+    """The BLAS forms NFFT1_1D_BLAS.m / NFFT2_1D_BLAS.m in grid units, as numpy matrix products."""
+    k = np.arange(-N // 2, N // 2)
+    E = np.exp(-2j * np.pi * np.outer(x, k) / N)       # [l][k]
+    return E @ f if f is not None else E.T @ d
+
+
+@pytest.mark.parametrize("N,M", [(16, 40), (64, 300), (22, 57)])
+def test_1d_direct_sums_vs_numpy(N, M):
+    rng = np.random.default_rng(N + M)
+    x = rng.uniform(-N, N, M)
+    f = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    d = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    assert oracle.rel_l2(oracle.ndft1d_type2(f, x), _numpy_ndft1d(N, x, f=f)) < 1e-13
+    assert oracle.rel_l2(oracle.ndft1d_type1(d, x, N), _numpy_ndft1d(N, x, d=d)) < 1e-13
+
+
+@pytest.mark.parametrize("N,M", [(16, 40), (64, 300), (22, 57), (128, 1000)])
+@pytest.mark.parametrize("round_mode", [0, 1])
+def test_1d_restatements_vs_direct_sums(N, M, round_mode):
+    """NFFT1_1D.m / NFFT2_1D.m restated against their direct sums (ties and points a period out
+    included: x is drawn over two periods and half the points sit on half-integer c x)."""
+    rng = np.random.default_rng(3 * N + M)
+    x = rng.uniform(-N, N, M)
+    x[: M // 4] = np.round(2 * x[: M // 4] + 0.5) / 2 - 0.25          # c x on a .5 tie
+    f = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    d = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    assert oracle.rel_l2(oracle.nfft1_1d(f, x, round_mode=round_mode), oracle.ndft1d_type2(f, x)) < TOL_FAST
+    assert oracle.rel_l2(oracle.nfft2_1d(d, x, N, round_mode=round_mode), oracle.ndft1d_type1(d, x, N)) < TOL_FAST
+
+
+def test_1d_adjoint_and_2d_consistency():
+    """<A f, g> = <f, conj(type1(conj g))> between the two 1-D restatements, and the 1-D type-2 equals
+    the 2-D restatement on a single mode row (k1 = 0 of a 2 x N array, x1 = 0)."""
+    rng = np.random.default_rng(77)
+    N, M = 32, 200
+    x = rng.uniform(-N / 2, N / 2, M)
+    f = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    g = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    lhs = np.vdot(g, oracle.nfft1_1d(f, x))
+    rhs = np.vdot(np.conj(oracle.nfft2_1d(np.conj(g), x, N)), f)
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+    f2 = np.zeros((2, N), complex)
+    f2[1] = f                                              # k1 = 0 is row 1 of 2
+    y2 = oracle.nfft1_2d(f2, np.zeros(M), x, round_mode=1)
+    assert oracle.rel_l2(oracle.nfft1_1d(f, x), y2) < TOL_FAST
+
