@@ -523,22 +523,219 @@ def bench_knab(args, dev):
             "rel_l2_vs_bandlimited_signal": rel}
 
 
+def bench_pso(args, dev):
+    """One PSO generation's far fields (Synthesis/PSO): 250 particles x NFFT2_2D_GPU on 88 x 88 with
+    M = 1936 elements + the far-field filter, as one batched graph, against 250 drop-in calls."""
+    import miscellaneous_amd as m
+
+    P, Nu, Nv, M = 250, 44, 44, 1936
+    rng = np.random.default_rng(SEED + 7)
+    gen = m.FarFieldGeneration(P, Nu, Nv, M, device=dev)
+    g = np.stack(np.meshgrid(np.arange(44), np.arange(44)), -1).reshape(-1, 2).astype(np.float64)
+    xy = (g[None] - 21.5) * 1.9 + rng.uniform(-0.3, 0.3, (P, M, 2))
+    gen.X.copy_(torch.from_numpy(xy[..., 0].astype(np.float32)))
+    gen.Y.copy_(torch.from_numpy(xy[..., 1].astype(np.float32)))
+    gen.excitations.copy_(cn01(torch.Generator().manual_seed(SEED + 7), (P, M), dev))
+    gen.filter.copy_(torch.from_numpy(rng.uniform(0.5, 1.5, 4 * Nu * Nv).astype(np.float32)))
+    graph = gen.capture()
+    s = time_graph(graph, max(args.steps // 4, 10), 3)
+    # the reference's way: one NFFT2_2D_GPU call (plus the filter) per particle
+    out = torch.empty(2 * Nv, 2 * Nu, dtype=torch.complex64, device=dev)
+
+    def loop():
+        for p in range(P):
+            m.NFFT2_2D_GPU(out, gen.excitations[p], gen.X[p], gen.Y[p], 2 * Nv, 2 * Nu, M)
+            out.mul_(gen.filter.view(2 * Nv, 2 * Nu))
+
+    loop()
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loop()
+    torch.cuda.synchronize()
+    s_loop = (time.perf_counter() - t0) / reps
+    return {"config": "far fields of one PSO generation (Synthesis/PSO): 250 particles, NFFT2_2D_GPU on 88x88 "
+                      "(Nu = Nv = 44) from M = 1936 elements each, times the far-field filter; batched setpts + "
+                      "type-1 + filter replayed from one graph",
+            "ms_per_generation": s * 1e3, "value": P * M / s / 1e6, "unit": "Mpts/s",
+            "dropin_loop_ms_per_generation": s_loop * 1e3, "speedup_vs_dropin_loop": s_loop / s}
+
+
+def bench_1d(args, dev):
+    """Batched 1-D NFFTs: the per-pulse type-2 of BP_FBP_FFBP/BP_NFFT.m:18 (64 pulses x 424 frequency
+    samples at 65,536 pixel ranges each) and a type-1 batch (1024 lines of 4096 samples to 256 modes)."""
+    import miscellaneous_amd as m
+
+    gen = torch.Generator().manual_seed(SEED + 8)
+    B, N, M = 64, 424, 65536
+    f = cn01(gen, (B, N), dev)
+    x = ((torch.rand(B, M, generator=gen) - 0.5) * 0.9 * N).to(dev)
+    plan = m.Plan1D(N, device=dev)
+    out = plan.type2(f, x)
+    sel = torch.from_numpy(np.random.default_rng(SEED + 8).choice(M, 1024, replace=False)).to(dev)
+    k = torch.arange(-N // 2, N // 2, device=dev, dtype=torch.float64)
+    ph = torch.outer(x[0][sel].double(), k) / N
+    ref = (torch.polar(torch.ones_like(ph), -2 * np.pi * (ph - torch.floor(ph))) @ f[0].to(torch.complex128))
+    rel2 = ((out[0][sel].to(torch.complex128) - ref).norm() / ref.norm()).item()
+    g2 = capture(lambda: plan.type2(f, x))
+    s2 = time_graph(g2, max(args.steps // 4, 10), 3)
+    B1, N1, M1 = 1024, 256, 4096
+    d = cn01(gen, (B1, M1), dev)
+    x1 = ((torch.rand(B1, M1, generator=gen) - 0.5) * N1).to(dev)
+    plan1 = m.Plan1D(N1, device=dev)
+    plan1.type1(d, x1)
+    g1 = capture(lambda: plan1.type1(d, x1))
+    s1 = time_graph(g1, max(args.steps // 4, 10), 3)
+    return {"type2": {"config": f"{B} lines x {N} modes -> {M} points each (BP_NFFT.m per-pulse NFFT1_1D), complex64",
+                      "ms": s2 * 1e3, "value": B * M / s2 / 1e6, "unit": "Mpts/s", "rel_l2_1024": rel2},
+            "type1": {"config": f"{B1} lines x {M1} samples -> {N1} modes (NFFT2_1D), complex64, exact fixed-point sums",
+                      "ms": s1 * 1e3, "value": B1 * M1 / s1 / 1e6, "unit": "Mpts/s"}}
+
+
+def bench_ndft_dropin(args, dev):
+    """NDFT2_2D_GPU on meshgrid outputs (the reference's caller passes the integer meshgrid): the FP64
+    matrix-core contraction, at the PSO size (88 x 88, M = 1936) and at 256 x 256 with M = 65,536."""
+    import miscellaneous_amd as m
+
+    res = {}
+    rng = np.random.default_rng(SEED + 9)
+    for tag, (nu, M) in {"pso_88": (88, 1936), "grid_256": (256, 65536)}.items():
+        X = torch.from_numpy(rng.uniform(-nu / 2, nu / 2, M)).to(dev)
+        Y = torch.from_numpy(rng.uniform(-nu / 2, nu / 2, M)).to(dev)
+        d = torch.from_numpy((rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)).to(dev)
+        n = torch.arange(nu * nu, device=dev)
+        u = (n // nu - nu // 2).double()
+        v = (n % nu - nu // 2).double()
+        out = torch.empty(nu * nu, dtype=torch.complex128, device=dev)
+
+        def call():
+            m.NDFT2_2D_GPU(0, X, Y, u, v, d, out, nu, nu, M, nu * nu)
+
+        call()
+        torch.cuda.synchronize()
+        reps = 20 if M < 10000 else 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        torch.cuda.synchronize()
+        s = (time.perf_counter() - t0) / reps
+        flops = 8.0 * M * nu * nu
+        res[tag] = {"config": f"{nu}x{nu} meshgrid outputs, M = {M} points, FP64", "ms": s * 1e3,
+                    "fp64_tflops": flops / s / 1e12, "mfma_util": flops / s / 78.6e12}
+    res["note"] = "useful FP64 flops 8 M N over host-timed calls; peak 78.6 TFLOP/s FP64 matrix (spec)"
+    return res
+
+
+def bench_autograd(args, dev):
+    """The torch.autograd.Function path of the gradient (forward, loss, backward to f and x), eager
+    steps at 256 x 256, M = 262,144, next to the fused type2_lsq + lsq_adjoint step of bench_grad."""
+    import miscellaneous_amd as m
+
+    N, M = 256, 262144
+    gen = torch.Generator().manual_seed(SEED + 10)
+    x1 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev).requires_grad_(True)
+    x2 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev).requires_grad_(True)
+    f = cn01(gen, (N, N), dev).requires_grad_(True)
+    y_obs = cn01(gen, (M,), dev)
+    plan = m.Plan(N, N, torch.complex64, device=dev)
+
+    def step():
+        f.grad = x1.grad = x2.grad = None
+        loss = ((m.nfft2d_type2(f, x1, x2, plan) - y_obs).abs() ** 2).sum()
+        loss.backward()
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    reps = max(args.steps // 4, 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t0) / reps
+    return {"config": "256x256 modes, M=262144 uniform; loss ||nfft2d_type2(f, x1, x2) - y||^2 through "
+                      "torch.autograd (grad f and x); eager steps, the tile sort skipped for unchanged points",
+            "steps_per_s": 1.0 / s, "ms_per_step": s * 1e3}
+
+
 def cpu_baseline(t2, seconds: float):
+    """The CPU restatement of the reference algorithm on the host cores, median of 5 timed runs after a
+    warm-up: FP32 type-2 on all cores (the reported value), and beside it the 1-thread run, the type-1
+    (NFFT/NFFT2_2D.cu's direction, OpenMP atomic spread as NFFT/NFFT3/C/NFFT3_2D_C.cpp:78-81) and FP64."""
     import oracle
 
     f = t2["f"].cpu().numpy()
     x1, x2 = t2["x1n"], t2["x2n"]
-    oracle.nfft1_2d(f, x1[:1000], x2[:1000], precision="float")    # warm-up / load
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        oracle.nfft1_2d(f, x1, x2, precision="float")
-        reps += 1
-        el = time.perf_counter() - t0
-        if (el >= seconds and reps >= 2) or reps >= 1000:
-            break
-    return dict(value=len(x1) * reps / el / 1e6, unit="Mpts/s", cores=oracle.num_threads(), kind="port",
-                sample=f"{reps} full type-2 transforms (512x512, M=1048576 polar, FP32) of the CPU restatement "
-                       f"of BP_FBP_FFBP/NFFT1_2D.m, {el:.1f} s")
+    M, N = len(x1), f.shape[0]
+    rng = np.random.default_rng(SEED)
+    d = ((rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)).astype(np.complex64)
+    cores = oracle.num_threads()
+
+    def median_time(fn, runs=5):
+        fn()
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    legs = {}
+    t = median_time(lambda: oracle.nfft1_2d(f, x1, x2, precision="float"))
+    value = M / t / 1e6
+    legs["type2_fp32"] = {"Mpts_s": value, "s": t, "threads": cores}
+    legs["type1_fp32"] = {"Mpts_s": M / (s := median_time(lambda: oracle.nfft2_2d(d, x1, x2, N, N, precision="float"))) / 1e6,
+                          "s": s, "threads": cores}
+    legs["type2_fp64"] = {"Mpts_s": M / (s := median_time(lambda: oracle.nfft1_2d(f.astype(np.complex128), x1, x2))) / 1e6,
+                          "s": s, "threads": cores}
+    legs["type1_fp64"] = {"Mpts_s": M / (s := median_time(lambda: oracle.nfft2_2d(d.astype(np.complex128), x1, x2, N, N))) / 1e6,
+                          "s": s, "threads": cores}
+    oracle.set_num_threads(1)
+    try:
+        sub = slice(0, M // 8)                       # one eighth of the points: same grid work, 1/8 of the gathers
+        s = median_time(lambda: oracle.nfft1_2d(f, x1[sub], x2[sub], precision="float"), runs=3)
+        legs["type2_fp32_1thread_M_over_8"] = {"Mpts_s": (M // 8) / s / 1e6, "s": s, "threads": 1}
+    finally:
+        oracle.set_num_threads(cores)
+    return dict(value=value, unit="Mpts/s", cores=cores, kind="port",
+                sample=f"median of 5 full type-2 transforms (512x512, M=1048576 polar, FP32) of the CPU restatement "
+                       f"of BP_FBP_FFBP/NFFT1_2D.m on {cores} threads, after one warm-up; legs: type-1 "
+                       f"(NFFT/NFFT2_2D.cu), FP64, and 1 thread on M/8 points", legs=legs)
+
+
+def profile_workload(args):
+    """A few eager calls of one workload, for rocprofv3 counter passes (no timing of its own)."""
+    import miscellaneous_amd as m
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator().manual_seed(SEED)
+    if args.profile == "copy":
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+    elif args.profile == "ndft_bf16x3":
+        N, M = 256, 65536
+        f = cn01(gen, (N, N), dev)
+        x1 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+        x2 = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+        p = m.NdftPlan(N, N, "bf16x3", device=dev).set_modes(f)
+        out = torch.empty(M, dtype=torch.complex64, device=dev)
+        for _ in range(args.steps):
+            p.eval(x1, x2, out=out)
+    else:
+        N = 512
+        x1n, x2n = polar_points(N)
+        plan = m.Plan(N, N, torch.complex64, device=dev).setpts(torch.from_numpy(x1n).to(dev),
+                                                                 torch.from_numpy(x2n).to(dev))
+        f = cn01(gen, (N, N), dev)
+        d = cn01(gen, (len(x1n),), dev)
+        for _ in range(args.steps):
+            plan.type2(f) if args.profile == "type2" else plan.type1(d)
+    torch.cuda.synchronize()
+    print(f"profile workload {args.profile}: {args.steps} calls done", flush=True)
 
 
 def main():
@@ -555,7 +752,14 @@ def main():
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "copy"],
+                    help="run only --steps eager calls of one workload (polar 512^2 type-2 / type-1, the "
+                         "BF16x3 NDFT at 256^2, M = 65536, or a 256 MiB device copy) and exit: the command "
+                         "the rocprofv3 counter runs profile")
     args = ap.parse_args()
+    if args.profile:
+        profile_workload(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
@@ -580,6 +784,10 @@ def main():
         extra["type1_polar512"] = bench_type1(args, dev)
         extra["fp64_polar512"] = bench_fp64(args, dev)
         extra["knab_polar_interp"] = bench_knab(args, dev)
+        extra["pso_generation"] = bench_pso(args, dev)
+        extra["nfft_1d"] = bench_1d(args, dev)
+        extra["ndft2_2d_dropin"] = bench_ndft_dropin(args, dev)
+        extra["autograd"] = bench_autograd(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]
