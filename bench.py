@@ -593,6 +593,49 @@ def bench_1d(args, dev):
                       "ms": s1 * 1e3, "value": B1 * M1 / s1 / 1e6, "unit": "Mpts/s"}}
 
 
+def type3_workload(dev, n=1 << 20):
+    """Type-3 inputs: n points uniform in [-50, 50]^2 with CN(0,1) weights, n outputs uniform in [-10, 10]^2
+    (X1 S = Y1 T = 500: a 1636 x 1636 grid at eps 1e-6)."""
+    rng = np.random.default_rng(SEED + 11)
+    x = torch.from_numpy(rng.uniform(-50, 50, n)).to(dev)
+    y = torch.from_numpy(rng.uniform(-50, 50, n)).to(dev)
+    s = torch.from_numpy(rng.uniform(-10, 10, n)).to(dev)
+    t = torch.from_numpy(rng.uniform(-10, 10, n)).to(dev)
+    f = torch.from_numpy((rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)).to(dev)
+    return x, y, s, t, f
+
+
+def bench_type3(args, dev):
+    """NUFFT_2D_GPU (the type-3 of NFFT/NFFT3/C/NFFT3_2D_C.cpp:199, FP64) at 1M inputs -> 1M outputs, eps 1e-6;
+    host-timed whole calls (each includes the extent read-back that sizes the grid), rel-L2 on 128 outputs
+    against an FP64 direct sum over all inputs."""
+    import miscellaneous_amd as m
+
+    x, y, s, t, f = type3_workload(dev)
+    n = x.numel()
+    eps = 1e-6
+    par = m.nufft3_params(x, y, s, t, eps)
+    F = m.NUFFT_2D(x, y, s, t, f, eps)
+    torch.cuda.synchronize()
+    reps = max(args.steps // 20, 5)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.NUFFT_2D(x, y, s, t, f, eps)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    sel = torch.from_numpy(np.random.default_rng(SEED + 12).choice(n, 128, replace=False)).to(dev)
+    ref = torch.empty(128, dtype=torch.complex128, device=dev)
+    for c in range(0, 128, 8):
+        ph = torch.outer(s[sel[c:c + 8]], x) + torch.outer(t[sel[c:c + 8]], y)
+        ref[c:c + 8] = torch.polar(torch.ones_like(ph), -ph) @ f
+    rel = ((F[sel] - ref).norm() / ref.norm()).item()
+    return {"config": f"type-3 NUFFT (NUFFT_2D_CPU's algorithm, FP64), {n} inputs uniform in [-50, 50]^2 -> {n} "
+                      f"outputs uniform in [-10, 10]^2, eps {eps}: grid {par['Mrx']} x {par['Mry']}, "
+                      f"{2 * par['msp'] + 1}^2 Gaussian taps per point",
+            "ms_per_call": sec * 1e3, "value": 2 * n / sec / 1e6, "unit": "Mpts/s (inputs + outputs)",
+            "rel_l2_128": rel, "eps": eps}
+
+
 def bench_ndft_dropin(args, dev):
     """NDFT2_2D_GPU on meshgrid outputs (the reference's caller passes the integer meshgrid): the FP64
     matrix-core contraction, at the PSO size (88 x 88, M = 1936) and at 256 x 256 with M = 65,536."""
@@ -716,6 +759,10 @@ def profile_workload(args):
         dst = torch.empty_like(src)
         for _ in range(args.steps):
             dst.copy_(src)
+    elif args.profile == "type3":
+        x, y, s, t, f = type3_workload(dev)
+        for _ in range(args.steps):
+            m.NUFFT_2D(x, y, s, t, f, 1e-6)
     elif args.profile == "ndft_bf16x3":
         N, M = 256, 65536
         f = cn01(gen, (N, N), dev)
@@ -752,9 +799,10 @@ def main():
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "copy"],
+    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "copy", "type3"],
                     help="run only --steps eager calls of one workload (polar 512^2 type-2 / type-1, the "
-                         "BF16x3 NDFT at 256^2, M = 65536, or a 256 MiB device copy) and exit: the command "
+                         "BF16x3 NDFT at 256^2, M = 65536, the FP64 type-3 at 1M -> 1M, or a 256 MiB device "
+                         "copy) and exit: the command "
                          "the rocprofv3 counter runs profile")
     args = ap.parse_args()
     if args.profile:
@@ -788,6 +836,7 @@ def main():
         extra["nfft_1d"] = bench_1d(args, dev)
         extra["ndft2_2d_dropin"] = bench_ndft_dropin(args, dev)
         extra["autograd"] = bench_autograd(args, dev)
+        extra["type3"] = bench_type3(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]

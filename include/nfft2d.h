@@ -304,6 +304,34 @@ int nfft2d_batch_set_io_type1(nfft2d_batch *batch, const nfft2d_c64 *const *data
 int nfft2d_batch_type1(nfft2d_batch *batch, void *stream);
 int nfft2d_batch_setpts_type1(nfft2d_batch *batch, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Type-3 2-D NUFFT (non-uniform to non-uniform), FP64:
+ *   F[j] = sum_k f[k] e^{-i (s[j] x[k] + t[j] y[k])},  j < len_out, k < len_in,
+ * to the accuracy eps, by Gaussian gridding with the parameters of
+ * NUFFT_2D_CPU (NFFT/NFFT3/C/NFFT3_2D_C.cpp:199, declared at
+ * NFFT/NFFT3/C/NFFT3_2D_C.h:6-15): R = 2.2, 2 msp + 1 taps per dimension from
+ * eps, an Mry x Mrx grid from the extents of x, y, s, t.  Same argument order
+ * and meaning; f, F as fftw_complex (interleaved re, im).
+ *   NUFFT_2D_GPU         device arrays; runs on the null stream, synchronises
+ *                        once (the extents go to the host to size the grid);
+ *   NUFFT_2D_GPU_host    host arrays, the contract of NUFFT_2D_CPU itself
+ *                        (copies in, runs, copies out; returns when F is set);
+ *   NUFFT_2D_GPU_params  the 14 derived values, in the order
+ *                        xb, yb, sb, tb, kk_x, kk_y, Dx, Dy, b, t1, fact^2, msp,
+ *                        Mrx, Mry (:214-267), computed from device arrays.
+ * Errors (NFFT2D_ERR_ARG): eps outside (0, 1), a non-finite coordinate, all x
+ * or all y equal (the reference divides by the half-extent there).
+ * NFFT2D_ERR_SIZE: msp > 42 (eps below ~1e-14) or Mrx, Mry above 4096
+ * (8192 for powers of two).  len_in = 0 gives F = 0; len_out = 0 does
+ * nothing.  Not bitwise reproducible across runs (inputs are binned in
+ * arrival order); calls on one device serialise on a shared workspace. */
+int NUFFT_2D_GPU(const double *x, const double *y, const double *s, const double *t, const nfft2d_c128 *f,
+                 nfft2d_c128 *F, double eps, int len_in, int len_out);
+int NUFFT_2D_GPU_host(const double *x, const double *y, const double *s, const double *t, const nfft2d_c128 *f,
+                      nfft2d_c128 *F, double eps, int len_in, int len_out);
+int NUFFT_2D_GPU_params(const double *x, const double *y, const double *s, const double *t, double eps, int len_in,
+                        int len_out, double *params);
+
 #ifdef __cplusplus
 }
 #endif

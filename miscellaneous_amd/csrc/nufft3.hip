@@ -1,0 +1,889 @@
+// nufft3.hip -- the type-3 2-D NUFFT (non-uniform to non-uniform, FP64) on
+// gfx950:
+//     F_j = sum_k f_k e^{-i (s_j x_k + t_j y_k)},
+// the GPU counterpart of NUFFT_2D_CPU (NFFT/NFFT3/C/NFFT3_2D_C.cpp:199-312):
+// Gaussian gridding with the reference's parameters (R = 2.2, the tap
+// half-width msp from eps, an Mry x Mrx grid sized by the data extents).
+//
+// One call:
+//   1. k3_extent / k3_extent_final: min and max of x, y, s, t (and whether all
+//      are finite).  The host reads them back -- the call's one
+//      synchronisation -- and derives every parameter exactly as :214-267.
+//   2. k3_prep_in / k3_prep_out: per input its pre-modulated sample and grid
+//      cell, per output its cell; both are counted into 16 x 16-cell bins and
+//      sorted by bin (k3_scan, k3_scatter).
+//   3. k3_spread: one wave per 16 x 16 tile of the grid sums
+//      c_k ey_k(j) ex_k(i) over the inputs within reach of the tile, four
+//      inputs per v_mfma_f64_16x16x4f64 (the Gaussian of :76 is separable);
+//      the cell factor fact e^{b (Dx^2 dsi^2 + Dy^2 dsj^2)} multiplies the
+//      finished tile.  Each cell is written once: no atomics, no zero fill.
+//   4. The forward 2-D DFT of the grid (:280-292): a pass over the rows, a
+//      transpose, a pass over the rows of the transpose.  Each line is one
+//      LDS-resident radix-4 Stockham FFT; lengths that are not powers of two
+//      go through Bluestein's chirp-z convolution at the next power of two
+//      >= 2n - 1.
+//   5. k3_interp: one block per output bin stages the bin's grid patch in LDS
+//      with the circular and FFT shifts and the checkerboard sign of
+//      circAndFFTShiftsFunction (:135-151) folded into the index; one wave
+//      per output sums the (2 msp + 1)^2 separable Gaussian taps and applies
+//      the post-compensation of spectralConvolutionFunctionCompensation
+//      (:156-194).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "../../include/nfft2d.h"
+#include "nfft2d_internal.hpp"
+
+namespace nufft3 {
+
+constexpr int kTile = 16;              // grid tile and bin edge, cells
+constexpr int kMaxMsp = 42;            // LDS patch (16 + 2 msp)^2 x 16 B <= 160 000 B
+constexpr int kMaxFft = 8192;          // longest LDS-resident FFT (128 KB)
+constexpr int kExtentBlocks = 1024;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct Par {
+    double xb, yb, sb, tb, kkx, kky, Dx, Dy, b, t1, fact;
+    int msp, Mrx, Mry, nbx, nby;
+};
+
+struct alignas(16) InRec {     // one input, in bin order
+    double xt, yt, re, im;     // scaled offsets (x - xb) / Dx, (y - yb) / Dy; pre-modulated sample
+    int nno, mmo, pad0, pad1;  // its cell
+};
+struct alignas(16) OutRec {    // one output, in bin order
+    double difs, dift, sd, td; // fractional cell offsets; s - sb, t - tb (rescaled)
+    int nsn, ntn, idx, pad;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ------------------------------------------------------------ 1. extents
+// 9 values per block: max x, min x, max y, min y, max s, min s, max t, min t,
+// and 1.0 if any coordinate is not finite.  The start values are -FLT_MAX /
+// FLT_MAX, as the reference's thrust::reduce calls (:214-231).
+__device__ __forceinline__ double wmax(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wmin(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k3_extent(const double *__restrict__ x, const double *__restrict__ y,
+                                                 const double *__restrict__ s, const double *__restrict__ t,
+                                                 int len_in, int len_out, double *__restrict__ part)
+{
+    __shared__ double red[4][9];
+    double v[9];
+    for (int q = 0; q < 8; q += 2) {
+        v[q] = -(double)FLT_MAX;
+        v[q + 1] = (double)FLT_MAX;
+    }
+    v[8] = 0.0;
+    const int stride = gridDim.x * 256;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < len_in; i += stride) {
+        const double a = x[i], c = y[i];
+        if (!isfinite(a) || !isfinite(c)) v[8] = 1.0;
+        if (a > v[0]) v[0] = a;
+        if (a < v[1]) v[1] = a;
+        if (c > v[2]) v[2] = c;
+        if (c < v[3]) v[3] = c;
+    }
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < len_out; i += stride) {
+        const double a = s[i], c = t[i];
+        if (!isfinite(a) || !isfinite(c)) v[8] = 1.0;
+        if (a > v[4]) v[4] = a;
+        if (a < v[5]) v[5] = a;
+        if (c > v[6]) v[6] = c;
+        if (c < v[7]) v[7] = c;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const double r = (q & 1) && q < 8 ? wmin(v[q]) : wmax(v[q]);
+        if (lane == 0) red[w][q] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int q = threadIdx.x;
+        double r = red[0][q];
+        for (int ww = 1; ww < 4; ++ww) r = (q & 1) && q < 8 ? fmin(r, red[ww][q]) : fmax(r, red[ww][q]);
+        part[blockIdx.x * 9 + q] = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void k3_extent_final(const double *__restrict__ part, int nblk,
+                                                      double *__restrict__ out)
+{
+    for (int q = 0; q < 9; ++q) {
+        const bool mn = (q & 1) && q < 8;
+        double r = mn ? (double)FLT_MAX : -(double)FLT_MAX;
+        if (q == 8) r = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 64) r = mn ? fmin(r, part[b * 9 + q]) : fmax(r, part[b * 9 + q]);
+        r = mn ? wmin(r) : wmax(r);
+        if (threadIdx.x == 0) out[q] = r;
+    }
+}
+
+// ------------------------------------------------------------ 2. points and bins
+__device__ __forceinline__ int bin_of(int i, int j, const Par &p)
+{
+    return (j / kTile) * p.nbx + i / kTile;
+}
+
+__global__ __launch_bounds__(256) void k3_prep_in(const double *__restrict__ x0, const double *__restrict__ y0,
+                                                  const double2 *__restrict__ f, int n, Par p,
+                                                  InRec *__restrict__ rec, int2 *__restrict__ kr,
+                                                  int *__restrict__ cnt)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    // spatialConvolutionFunction (:38-63), on the rescaled x / kk_x, y / kk_y (:247-248)
+    const double x = x0[k] / p.kkx, y = y0[k] / p.kky;
+    const double arg = p.sb * x + p.tb * y;
+    double sn, cs;
+    sincos(arg, &sn, &cs);
+    const double2 v = f[k];
+    InRec r;
+    r.re = v.x * cs + v.y * sn;
+    r.im = v.y * cs - v.x * sn;
+    r.xt = (x - p.xb) / p.Dx;
+    r.yt = (y - p.yb) / p.Dy;
+    r.nno = (int)floor((double)(p.Mrx / 2) + r.xt);
+    r.mmo = (int)floor((double)(p.Mry / 2) + r.yt);
+    r.pad0 = r.pad1 = 0;
+    // the extents put every cell in [Mrx / 4 - 1, 3 Mrx / 4]; the bound keeps
+    // a rounding at the edge from leaving the grid
+    const int nb = p.nbx * p.nby;
+    const bool ok = r.nno >= 0 && r.nno < p.Mrx && r.mmo >= 0 && r.mmo < p.Mry;
+    const int bin = ok ? bin_of(r.nno, r.mmo, p) : nb;   // bin nb is never read
+    rec[k] = r;
+    kr[k] = make_int2(bin, atomicAdd(&cnt[bin], 1));
+}
+
+__global__ __launch_bounds__(256) void k3_prep_out(const double *__restrict__ s0, const double *__restrict__ t0,
+                                                   int n, Par p, OutRec *__restrict__ rec,
+                                                   int2 *__restrict__ kr, int *__restrict__ cnt,
+                                                   double2 *__restrict__ F)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    // spectralConvolutionFunctionCompensation (:164-167), s * kk_x, t * kk_y (:249-250), Ds = Dt = 1
+    const double s = s0[k] * p.kkx, t = t0[k] * p.kky;
+    OutRec r;
+    r.sd = s - p.sb;
+    r.td = t - p.tb;
+    r.nsn = (int)floor((double)(p.Mrx / 2) + r.sd);
+    r.difs = ((double)(p.Mrx / 2) + r.sd) - r.nsn;
+    r.ntn = (int)floor((double)(p.Mry / 2) + r.td);
+    r.dift = ((double)(p.Mry / 2) + r.td) - r.ntn;
+    r.idx = k;
+    r.pad = 0;
+    const int nb = p.nbx * p.nby;
+    const bool ok = r.nsn >= 0 && r.nsn < p.Mrx && r.ntn >= 0 && r.ntn < p.Mry;
+    const int bin = ok ? bin_of(r.nsn, r.ntn, p) : nb;
+    if (!ok) F[k] = make_double2(NAN, NAN);
+    rec[k] = r;
+    kr[k] = make_int2(bin, atomicAdd(&cnt[bin], 1));
+}
+
+// Exclusive scan of one count array per block (block 0: inputs, block 1:
+// outputs), nb + 1 entries each; start[nb + 1] = total.
+__global__ __launch_bounds__(1024) void k3_scan(const int *__restrict__ cnt_in, const int *__restrict__ cnt_out,
+                                                int nb, int *__restrict__ st_in, int *__restrict__ st_out)
+{
+    __shared__ int sums[1024];
+    const int *cnt = blockIdx.x ? cnt_out : cnt_in;
+    int *st = blockIdx.x ? st_out : st_in;
+    const int n = nb + 1;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
+    int acc = 0;
+    for (int i = lo; i < hi; ++i) acc += cnt[i];
+    sums[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = threadIdx.x >= o ? sums[threadIdx.x - o] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = sums[threadIdx.x] - acc;
+    for (int i = lo; i < hi; ++i) {
+        st[i] = run;
+        run += cnt[i];
+    }
+    if (threadIdx.x == 1023) st[n] = sums[1023];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k3_scatter(const R *__restrict__ rec, const int2 *__restrict__ kr, int n,
+                                                  const int *__restrict__ st, R *__restrict__ out)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int2 e = kr[k];
+    out[st[e.x] + e.y] = rec[k];
+}
+
+// ------------------------------------------------------------ 3. Gaussian gridding
+// Block = 4 waves = a 2 x 2 group of 16 x 16 tiles; tile (tx, ty) covers cells
+// i in [16 tx, 16 tx + 16), j in [16 ty, 16 ty + 16) of the row-major Mry x Mrx
+// grid.  An input k reaches cell (i, j) when |i - nno_k| <= msp and
+// |j - mmo_k| <= msp (:65-72).  The wave scans the bins within
+// ceil(msp / 16) of its tile 64 inputs at a time, keeps those that reach the
+// tile (a per-wave list in LDS) and takes them four per MFMA step: lane l
+// supplies A[r][k] = c_k ey_k(j0 + r) and B[k][r] = ex_k(i0 + r) at
+// r = l % 16, k = l / 16, and accumulator element q of lane l is cell
+// (j0 + 4 q + l / 16, i0 + l % 16).
+__global__ __launch_bounds__(256) void k3_spread(const InRec *__restrict__ pts, const int *__restrict__ st, Par p,
+                                                 double2 *__restrict__ G)
+{
+    __shared__ int list[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tx = 2 * blockIdx.x + (w & 1), ty = 2 * blockIdx.y + (w >> 1);
+    if (tx >= p.nbx || ty >= p.nby) return;   // no block-wide barrier below
+    const int i0 = tx * kTile, j0 = ty * kTile;
+    const int r = lane & 15, ks = lane >> 4;
+    const int jrow = j0 + r, icol = i0 + r;
+    const double dsj = (double)(jrow - p.Mry / 2), dsi = (double)(icol - p.Mrx / 2);
+    const int msp = p.msp, reach = (msp + kTile - 1) / kTile;
+    d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    const int by0 = max(0, ty - reach), by1 = min(p.nby - 1, ty + reach);
+    const int bx0 = max(0, tx - reach), bx1 = min(p.nbx - 1, tx + reach);
+    for (int by = by0; by <= by1; ++by) {
+        const int s0 = st[by * p.nbx + bx0], s1 = st[by * p.nbx + bx1 + 1];   // the row's bins are contiguous
+        for (int base = s0; base < s1; base += 64) {
+            const int c = base + lane;
+            bool in = false;
+            if (c < s1) {
+                const int2 cell = *reinterpret_cast<const int2 *>(&pts[c].nno);
+                in = cell.x + msp >= i0 && cell.x - msp < i0 + kTile && cell.y + msp >= j0 &&
+                     cell.y - msp < j0 + kTile;
+            }
+            const unsigned long long mask = __ballot(in);
+            if (mask == 0ull) continue;
+            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            if (in) list[w][pos] = c;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int n = __popcll(mask);
+            for (int q = 0; q < n; q += 4) {
+                double ar = 0.0, ai = 0.0, bb = 0.0;
+                if (q + ks < n) {
+                    const InRec *e = pts + list[w][q + ks];
+                    const double2 xy = *reinterpret_cast<const double2 *>(&e->xt);
+                    const double2 c = *reinterpret_cast<const double2 *>(&e->re);
+                    const int2 cell = *reinterpret_cast<const int2 *>(&e->nno);
+                    if (abs(jrow - cell.y) <= msp && jrow < p.Mry) {
+                        const double d = xy.y - dsj;
+                        const double ey = exp(-p.t1 * (d * d));
+                        ar = c.x * ey;
+                        ai = c.y * ey;
+                    }
+                    if (abs(icol - cell.x) <= msp && icol < p.Mrx) {
+                        const double d = xy.x - dsi;
+                        bb = exp(-p.t1 * (d * d));
+                    }
+                }
+                accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bb, accr, 0, 0, 0);
+                acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bb, acci, 0, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();   // list reads done before the next chunk's writes
+        }
+    }
+    const int ii = i0 + r;
+    const double ddi = (double)(ii - p.Mrx / 2);
+    const double ex = p.b * (p.Dx * p.Dx * ddi * ddi);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int jj = j0 + 4 * q + ks;
+        if (jj < p.Mry && ii < p.Mrx) {
+            const double ddj = (double)(jj - p.Mry / 2);
+            const double sc = p.fact * exp(p.b * (p.Dy * p.Dy * ddj * ddj) + ex);
+            G[(size_t)jj * p.Mrx + ii] = make_double2(accr[q] * sc, acci[q] * sc);
+        }
+    }
+}
+
+// ------------------------------------------------------------ 4. FFTs of the grid lines
+// In-place radix-4 (and one radix-2) Stockham FFT of m = 2^a points in LDS,
+// forward (e^{-2 pi i jk / m}), after Govindaraju et al., "High performance
+// discrete Fourier transforms on graphics processors" (SC 2008): stage with
+// span Ns reads v_r = X[j + r m/R], twiddles it by e^{-2 pi i r (j mod Ns) /
+// (Ns R)}, and writes the R-point DFT of v to X[(j - j mod Ns) R + j mod Ns
+// + r Ns].  All reads of a stage precede its writes (barrier), so one buffer
+// serves.  tw[t] = e^{-2 pi i t / m}.  Every thread holds at most 8 values.
+__device__ void lds_fft(double2 *X, int m, const double2 *__restrict__ tw)
+{
+    const int T = blockDim.x, tid = threadIdx.x;
+    for (int Ns = 1; Ns < m;) {
+        const int R = (m / Ns) >= 4 ? 4 : 2;
+        const int nb = m / R, per = (nb + T - 1) / T;   // butterflies per thread: <= 2 (radix 4), <= 4 (radix 2)
+        const int tstep = m / (Ns * R);
+        double2 v[8];
+        if (R == 4) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = tid + u * T;
+                if (u < per && j < nb) {
+                    const int k = j & (Ns - 1);
+                    double2 a0 = X[j], a1 = X[j + nb], a2 = X[j + 2 * nb], a3 = X[j + 3 * nb];
+                    if (Ns > 1) {
+                        a1 = cmul(a1, tw[k * tstep]);
+                        a2 = cmul(a2, tw[2 * k * tstep]);
+                        a3 = cmul(a3, tw[3 * k * tstep]);
+                    }
+                    const double2 b0 = make_double2(a0.x + a2.x, a0.y + a2.y);
+                    const double2 b1 = make_double2(a0.x - a2.x, a0.y - a2.y);
+                    const double2 b2 = make_double2(a1.x + a3.x, a1.y + a3.y);
+                    const double2 b3 = make_double2(a1.y - a3.y, a3.x - a1.x);   // (a1 - a3) (-i)
+                    v[4 * u + 0] = make_double2(b0.x + b2.x, b0.y + b2.y);
+                    v[4 * u + 1] = make_double2(b1.x + b3.x, b1.y + b3.y);
+                    v[4 * u + 2] = make_double2(b0.x - b2.x, b0.y - b2.y);
+                    v[4 * u + 3] = make_double2(b1.x - b3.x, b1.y - b3.y);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = tid + u * T;
+                if (u < per && j < nb) {
+                    const int k = j & (Ns - 1), o = (j - k) * 4 + k;
+                    X[o] = v[4 * u];
+                    X[o + Ns] = v[4 * u + 1];
+                    X[o + 2 * Ns] = v[4 * u + 2];
+                    X[o + 3 * Ns] = v[4 * u + 3];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = tid + u * T;
+                if (u < per && j < nb) {
+                    const int k = j & (Ns - 1);
+                    double2 a0 = X[j], a1 = X[j + nb];
+                    if (Ns > 1) a1 = cmul(a1, tw[k * tstep]);
+                    v[2 * u] = make_double2(a0.x + a1.x, a0.y + a1.y);
+                    v[2 * u + 1] = make_double2(a0.x - a1.x, a0.y - a1.y);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = tid + u * T;
+                if (u < per && j < nb) {
+                    const int k = j & (Ns - 1), o = (j - k) * 2 + k;
+                    X[o] = v[2 * u];
+                    X[o + Ns] = v[2 * u + 1];
+                }
+            }
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
+// Forward DFT of `lines` contiguous lines of n points (line stride n), one
+// line per block, m = n (n a power of two) or Bluestein at m >= 2n - 1:
+//   X_k = w_k sum_j (x_j w_j) conj(w_{k-j}),  w_j = e^{-i pi j^2 / n},
+// the convolution taken as IFFT(FFT(x w) * bh), bh = FFT of the wrapped
+// conj(w) sequence, and IFFT(Y) = conj(FFT(conj(Y))) / m.
+__global__ __launch_bounds__(1024) void k3_fft_lines(double2 *__restrict__ data, int n, int m,
+                                                     const double2 *__restrict__ tw,
+                                                     const double2 *__restrict__ chirp,
+                                                     const double2 *__restrict__ bh)
+{
+    extern __shared__ double2 X[];
+    double2 *row = data + (size_t)blockIdx.x * n;
+    const int T = blockDim.x;
+    if (!chirp) {
+        for (int k = threadIdx.x; k < m; k += T) X[k] = row[k];
+        __syncthreads();
+        lds_fft(X, m, tw);
+        for (int k = threadIdx.x; k < m; k += T) row[k] = X[k];
+        return;
+    }
+    for (int k = threadIdx.x; k < m; k += T) X[k] = k < n ? cmul(row[k], chirp[k]) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft(X, m, tw);
+    for (int k = threadIdx.x; k < m; k += T) {
+        const double2 z = cmul(X[k], bh[k]);
+        X[k] = make_double2(z.x, -z.y);
+    }
+    __syncthreads();
+    lds_fft(X, m, tw);
+    const double inv = 1.0 / m;
+    for (int k = threadIdx.x; k < n; k += T) {
+        const double2 z = X[k];
+        row[k] = cmul(make_double2(z.x * inv, -z.y * inv), chirp[k]);
+    }
+}
+
+// tw[t] = e^{-2 pi i t / m}, t < m.
+__global__ __launch_bounds__(256) void k3_twiddles(int m, double2 *__restrict__ tw)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= m) return;
+    double sn, cs;
+    sincospi(2.0 * t / m, &sn, &cs);
+    tw[t] = make_double2(cs, -sn);
+}
+
+// Bluestein tables of a length-n line: chirp[j] = e^{-i pi j^2 / n} (j < n;
+// j^2 reduced mod 2n exactly) and b = the length-m sequence holding
+// conj(chirp[j]) at j and at m - j, zero between (transformed in place
+// afterwards).
+__global__ __launch_bounds__(256) void k3_chirp(int n, int m, double2 *__restrict__ chirp, double2 *__restrict__ b)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    double2 bj = make_double2(0.0, 0.0);
+    const int jj = j < n ? j : (j > m - n ? m - j : -1);
+    if (jj >= 0) {
+        const long long q = ((long long)jj * jj) % (2LL * n);
+        double sn, cs;
+        sincospi((double)q / n, &sn, &cs);
+        if (j < n) chirp[j] = make_double2(cs, -sn);
+        bj = make_double2(cs, sn);
+    }
+    b[j] = bj;
+}
+
+// out[c][r] = in[r][c], in: rows x cols.
+__global__ __launch_bounds__(256) void k3_transpose(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                    int rows, int cols)
+{
+    __shared__ double2 tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int rr = r0 + k, cc = c0 + tx;
+        if (rr < rows && cc < cols) tile[k][tx] = in[(size_t)rr * cols + cc];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int cc = c0 + k, rr = r0 + tx;
+        if (rr < rows && cc < cols) out[(size_t)cc * rows + rr] = tile[tx][k];
+    }
+}
+
+// ------------------------------------------------------------ 5. interpolation at the outputs
+// Block per output bin (by, bx): the patch of the shifted spectrum
+// gs[mm][nn] = (-1)^(i + j) FT[j][i], i = (nn + Mrx/2) mod Mrx,
+// j = (mm + Mry/2) mod Mry (FT the 2-D DFT, held transposed as FTt[i][j]),
+// for mm in [16 by - msp, 16 by + 16 + msp), nn likewise, goes to LDS as
+// P[nn'][mm'] (W = 16 + 2 msp per side; wrapped indices outside the grid
+// are loaded but never weighted).  One wave per output: lane l takes
+// mm' = ntn - msp - mm0 + l (and + 64 when 2 msp + 1 > 64) and sums
+// ex(nn) P[nn][mm] over the 2 msp + 1 columns, ex read lane by lane; then
+// the ey weights, a wave reduction, and the post-compensation.
+template <int NL>
+__global__ __launch_bounds__(256) void k3_interp(const double2 *__restrict__ FTt, const OutRec *__restrict__ outs,
+                                                 const int *__restrict__ st, Par p, double2 *__restrict__ F)
+{
+    extern __shared__ double2 P[];
+    const int bin = blockIdx.x;
+    const int s0 = st[bin], s1 = st[bin + 1];
+    if (s0 == s1) return;
+    const int bx = bin % p.nbx, by = bin / p.nbx;
+    const int msp = p.msp, W = kTile + 2 * msp, taps = 2 * msp + 1;
+    const int mm0 = by * kTile - msp, nn0 = bx * kTile - msp;
+    for (int e = threadIdx.x; e < W * W; e += 256) {
+        const int a = e % W, c = e / W;   // a: mm offset (contiguous in FTt), c: nn offset
+        int mm = mm0 + a, nn = nn0 + c;
+        mm += mm < 0 ? p.Mry : (mm >= p.Mry ? -p.Mry : 0);
+        nn += nn < 0 ? p.Mrx : (nn >= p.Mrx ? -p.Mrx : 0);
+        int i = nn + p.Mrx / 2, j = mm + p.Mry / 2;
+        i -= i >= p.Mrx ? p.Mrx : 0;
+        j -= j >= p.Mry ? p.Mry : 0;
+        double2 v = FTt[(size_t)i * p.Mry + j];
+        if ((i + j) & 1) v = make_double2(-v.x, -v.y);
+        P[c * W + a] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = s0 + w; o < s1; o += 4) {
+        const OutRec rec = outs[o];
+        const int ra = rec.ntn - msp - mm0, ca = rec.nsn - msp - nn0;   // window origin in the patch
+        double ey[NL], ex[NL], accr[NL], acci[NL];
+#pragma unroll
+        for (int h = 0; h < NL; ++h) {
+            const int l = lane + 64 * h;
+            const double npi = (double)(l - msp);
+            const double dy = rec.dift - npi, dx = rec.difs - npi;
+            ey[h] = l < taps ? exp(-p.t1 * (dy * dy)) : 0.0;
+            ex[h] = l < taps ? exp(-p.t1 * (dx * dx)) : 0.0;
+            accr[h] = acci[h] = 0.0;
+        }
+        for (int c = 0; c < taps; ++c) {
+            const int src = c & 63, h = c >> 6;
+            const double exv = NL == 1 || h == 0 ? ex[0] : ex[NL - 1];
+            const int lo = __builtin_amdgcn_readlane(__double2loint(exv), src);
+            const int hi = __builtin_amdgcn_readlane(__double2hiint(exv), src);
+            const double wc = __hiloint2double(hi, lo);
+            const double2 *col = P + (ca + c) * W + ra;
+#pragma unroll
+            for (int hh = 0; hh < NL; ++hh) {
+                const int l = lane + 64 * hh;
+                if (l < taps) {
+                    const double2 v = col[l];
+                    accr[hh] = fma(wc, v.x, accr[hh]);
+                    acci[hh] = fma(wc, v.y, acci[hh]);
+                }
+            }
+        }
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int h = 0; h < NL; ++h) {
+            sr = fma(ey[h], accr[h], sr);
+            si = fma(ey[h], acci[h], si);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, 64);
+            si += __shfl_xor(si, off, 64);
+        }
+        if (lane == 0) {
+            const double a1 = p.Dy * rec.td, a2 = p.Dx * rec.sd;
+            const double temp = exp(p.b * (a1 * a1 + a2 * a2));
+            const double arg = rec.sd * p.xb + rec.td * p.yb;
+            double sn, cs;
+            sincos(arg, &sn, &cs);
+            F[rec.idx] = make_double2((sr * cs + si * sn) * temp, (si * cs - sr * sn) * temp);
+        }
+    }
+}
+
+}  // namespace nufft3
+
+using namespace nufft3;
+
+// ============================================================== host side
+namespace {
+
+constexpr double kPi3 = 3.141592653589793238462643383279502884;
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int next_pow2(int n)
+{
+    int m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+int fft_len(int n) { return is_pow2(n) ? n : next_pow2(2 * n - 1); }
+
+// NUFFT_2D_CPU's parameters from the extents (:214-267), in its order of
+// operations.  ext: max/min of x, y, s, t.
+#pragma clang fp contract(off)
+void derive(const double *ext, double eps, double *out)
+{
+    double Mx = ext[0], mx = ext[1], My = ext[2], my = ext[3];
+    double Ms = ext[4], ms = ext[5], Mt = ext[6], mt = ext[7];
+    double xb = (Mx + mx) / 2, X1 = fabs(Mx - xb);
+    double yb = (My + my) / 2, Y1 = fabs(My - yb);
+    double sb = (Ms + ms) / 2, S = fabs(Ms - sb);
+    double tb = (Mt + mt) / 2, T = fabs(Mt - tb);
+    const double R = 2.2;
+    double msp = 2 + 2 * (R * R) * (-log(eps / 76)) / (kPi3 * (R * R - 2));
+    const double mrx = 2 * ceil((X1 * S / kPi3) * R * R + R * msp);
+    const double mry = 2 * ceil((Y1 * T / kPi3) * R * R + R * msp);
+    double kkx = X1 / (kPi3 / 2), kky = Y1 / (kPi3 / 2);
+    Mx /= kkx; mx /= kkx; xb = (Mx + mx) / 2;
+    Ms *= kkx; ms *= kkx; sb = (Ms + ms) / 2;
+    My /= kky; my /= kky; yb = (My + my) / 2;
+    Mt *= kky; mt *= kky; tb = (Mt + mt) / 2;
+    const double b = (msp - 2) / (4 * kPi3), t1 = 1 / (4 * b);
+    msp = floor(msp - 2);
+    double fact = 1. / (4. * kPi3 * b);
+    out[0] = xb; out[1] = yb; out[2] = sb; out[3] = tb; out[4] = kkx; out[5] = kky;
+    out[6] = 2 * kPi3 / mrx; out[7] = 2 * kPi3 / mry; out[8] = b; out[9] = t1; out[10] = fact * fact;
+    out[11] = msp; out[12] = mrx; out[13] = mry;
+}
+#pragma clang fp contract(on)
+
+struct Workspace {
+    unsigned char *ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_mu;
+std::map<int, Workspace> g_ws;
+
+int ensure(Workspace &ws, size_t bytes)
+{
+    if (ws.size >= bytes) return NFFT2D_OK;
+    if (ws.ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ws.ptr);
+        ws.ptr = nullptr;
+        ws.size = 0;
+    }
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(&ws.ptr, want) != hipSuccess) {
+        (void)hipGetLastError();
+        ws.ptr = nullptr;
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "type-3 workspace of %zu bytes", want);
+    }
+    ws.size = want;
+    return NFFT2D_OK;
+}
+
+size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct Carve {
+    unsigned char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t n)
+    {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += al(n * sizeof(T));
+        return p;
+    }
+};
+
+int check_args(const void *x, const void *y, const void *s, const void *t, const void *f, const void *F,
+               double eps, int len_in, int len_out)
+{
+    if (len_in < 0 || len_out < 0) return nfft2d_fail(NFFT2D_ERR_ARG, "len_in = %d, len_out = %d", len_in, len_out);
+    if (len_in > kMaxPoints || len_out > kMaxPoints)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "at most %d inputs and outputs", kMaxPoints);
+    if (!(eps > 0.0 && eps < 1.0)) return nfft2d_fail(NFFT2D_ERR_ARG, "eps = %g: 0 < eps < 1 required", eps);
+    if (len_out > 0 && (!s || !t || !F)) return nfft2d_fail(NFFT2D_ERR_ARG, "null output-side pointer");
+    if (len_in > 0 && (!x || !y || !f)) return nfft2d_fail(NFFT2D_ERR_ARG, "null input-side pointer");
+    return NFFT2D_OK;
+}
+
+// Extents on the device, parameters on the host; out[14] as orc_nufft3_params.
+int params(Workspace &ws, const double *x, const double *y, const double *s, const double *t, double eps,
+           int len_in, int len_out, double *out)
+{
+    const int L = std::max(len_in, len_out);
+    const int nblk = std::max(1, std::min(kExtentBlocks, (L + 255) / 256));
+    int rc = ensure(ws, al(sizeof(double) * 9 * (kExtentBlocks + 1)));
+    if (rc) return rc;
+    double *part = reinterpret_cast<double *>(ws.ptr), *fin = part + 9 * kExtentBlocks;
+    hipLaunchKernelGGL(k3_extent, dim3(nblk), dim3(256), 0, nullptr, x, y, s, t, len_in, len_out, part);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k3_extent_final, dim3(1), dim3(64), 0, nullptr, part, nblk, fin);
+    NFFT2D_LAUNCH_CHECK();
+    double ext[9];
+    NFFT2D_HIP_TRY(hipMemcpy(ext, fin, sizeof(ext), hipMemcpyDeviceToHost));
+    if (ext[8] != 0.0) return nfft2d_fail(NFFT2D_ERR_ARG, "a coordinate is not finite");
+    if (!(ext[0] > ext[1]) || !(ext[2] > ext[3]))
+        return nfft2d_fail(NFFT2D_ERR_ARG,
+                           "the inputs span no extent in x or y (all equal): the type-3 scaling divides by it");
+    derive(ext, eps, out);
+    return NFFT2D_OK;
+}
+
+int run(const double *x, const double *y, const double *s, const double *t, const nfft2d_c128 *f,
+        nfft2d_c128 *F, double eps, int len_in, int len_out)
+{
+    int rc = check_args(x, y, s, t, f, F, eps, len_in, len_out);
+    if (rc) return rc;
+    if (len_out == 0) return NFFT2D_OK;
+    if (len_in == 0) {   // the empty sum
+        NFFT2D_HIP_TRY(hipMemsetAsync(F, 0, sizeof(nfft2d_c128) * (size_t)len_out, nullptr));
+        return NFFT2D_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    int dev = 0;
+    NFFT2D_HIP_TRY(hipGetDevice(&dev));
+    Workspace &ws = g_ws[dev];
+    double pv[14];
+    if ((rc = params(ws, x, y, s, t, eps, len_in, len_out, pv))) return rc;
+    if (!(pv[12] <= kMaxFft && pv[13] <= kMaxFft))
+        return nfft2d_fail(NFFT2D_ERR_SIZE,
+                           "grid %.0f x %.0f (Mrx x Mry): at most 4096 (8192 for powers of two) per side; reduce the "
+                           "products of the extents X1 S and Y1 T",
+                           pv[12], pv[13]);
+    Par p;
+    p.xb = pv[0]; p.yb = pv[1]; p.sb = pv[2]; p.tb = pv[3]; p.kkx = pv[4]; p.kky = pv[5];
+    p.Dx = pv[6]; p.Dy = pv[7]; p.b = pv[8]; p.t1 = pv[9]; p.fact = pv[10];
+    p.msp = (int)pv[11]; p.Mrx = (int)pv[12]; p.Mry = (int)pv[13];
+    if (p.msp < 0 || p.msp > kMaxMsp)
+        return nfft2d_fail(NFFT2D_ERR_SIZE, "eps = %g gives %d taps per side; at most %d (eps >= ~1e-14)", eps, p.msp,
+                           kMaxMsp);
+    const int mx = fft_len(p.Mrx), my = fft_len(p.Mry);
+    if (mx > kMaxFft || my > kMaxFft)
+        return nfft2d_fail(NFFT2D_ERR_SIZE,
+                           "grid %d x %d (Mrx x Mry) needs FFTs of %d and %d points; at most %d (Mrx, Mry <= 4096, "
+                           "or 8192 for powers of two): reduce the products of the extents X1 S and Y1 T",
+                           p.Mrx, p.Mry, mx, my, kMaxFft);
+    p.nbx = (p.Mrx + kTile - 1) / kTile;
+    p.nby = (p.Mry + kTile - 1) / kTile;
+    const int nb = p.nbx * p.nby;
+    const size_t cells = (size_t)p.Mrx * p.Mry;
+
+    // workspace: counts, starts, records, the grid and its transpose, FFT tables
+    size_t need = 0;
+    {
+        const size_t sz[] = {al(sizeof(int) * (nb + 2)) * 4,
+                             al(sizeof(InRec) * len_in) * 2, al(sizeof(int2) * len_in),
+                             al(sizeof(OutRec) * len_out) * 2, al(sizeof(int2) * len_out),
+                             al(sizeof(double2) * cells) * 2,
+                             al(sizeof(double2) * mx) * 2 + al(sizeof(double2) * p.Mrx),
+                             al(sizeof(double2) * my) * 2 + al(sizeof(double2) * p.Mry)};
+        for (size_t v : sz) need += v;
+    }
+    if ((rc = ensure(ws, need))) return rc;
+    Carve cv{ws.ptr};
+    int *cnt_in = cv.take<int>(nb + 2), *cnt_out = cv.take<int>(nb + 2);
+    int *st_in = cv.take<int>(nb + 2), *st_out = cv.take<int>(nb + 2);
+    InRec *rin = cv.take<InRec>(len_in), *sin_ = cv.take<InRec>(len_in);
+    int2 *kin = cv.take<int2>(len_in);
+    OutRec *rout = cv.take<OutRec>(len_out), *sout = cv.take<OutRec>(len_out);
+    int2 *kout = cv.take<int2>(len_out);
+    double2 *G = cv.take<double2>(cells), *GT = cv.take<double2>(cells);
+    double2 *twx = cv.take<double2>(mx), *bhx = cv.take<double2>(mx), *chx = cv.take<double2>(p.Mrx);
+    double2 *twy = cv.take<double2>(my), *bhy = cv.take<double2>(my), *chy = cv.take<double2>(p.Mry);
+
+    hipStream_t st = nullptr;
+    NFFT2D_HIP_TRY(hipMemsetAsync(cnt_in, 0, al(sizeof(int) * (nb + 2)) * 2, st));   // cnt_in and cnt_out
+    const unsigned gin = (unsigned)((len_in + 255) / 256), gout = (unsigned)((len_out + 255) / 256);
+    hipLaunchKernelGGL(k3_prep_in, dim3(gin), dim3(256), 0, st, x, y, reinterpret_cast<const double2 *>(f), len_in,
+                       p, rin, kin, cnt_in);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k3_prep_out, dim3(gout), dim3(256), 0, st, s, t, len_out, p, rout, kout, cnt_out,
+                       reinterpret_cast<double2 *>(F));
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k3_scan, dim3(2), dim3(1024), 0, st, cnt_in, cnt_out, nb, st_in, st_out);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k3_scatter<InRec>, dim3(gin), dim3(256), 0, st, rin, kin, len_in, st_in, sin_);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k3_scatter<OutRec>, dim3(gout), dim3(256), 0, st, rout, kout, len_out, st_out, sout);
+    NFFT2D_LAUNCH_CHECK();
+
+    // FFT tables (twiddles; chirps and their transforms for Bluestein lines)
+    auto tables = [&](int n, int m, double2 *tw, double2 *ch, double2 *bh) -> int {
+        hipLaunchKernelGGL(k3_twiddles, dim3((m + 255) / 256), dim3(256), 0, st, m, tw);
+        NFFT2D_LAUNCH_CHECK();
+        if (is_pow2(n)) return NFFT2D_OK;
+        hipLaunchKernelGGL(k3_chirp, dim3((m + 255) / 256), dim3(256), 0, st, n, m, ch, bh);
+        NFFT2D_LAUNCH_CHECK();
+        const int T = std::min(1024, std::max(64, m / 4));
+        hipLaunchKernelGGL(k3_fft_lines, dim3(1), dim3(T), sizeof(double2) * m, st, bh, m, m, tw,
+                           (const double2 *)nullptr, (const double2 *)nullptr);
+        NFFT2D_LAUNCH_CHECK();
+        return NFFT2D_OK;
+    };
+    NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_fft_lines, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sizeof(double2) * kMaxFft));
+    if ((rc = tables(p.Mrx, mx, twx, chx, bhx))) return rc;
+    if ((rc = tables(p.Mry, my, twy, chy, bhy))) return rc;
+
+    // spread
+    hipLaunchKernelGGL(k3_spread, dim3((p.nbx + 1) / 2, (p.nby + 1) / 2), dim3(256), 0, st, sin_, st_in, p, G);
+    NFFT2D_LAUNCH_CHECK();
+
+    // 2-D DFT: rows (length Mrx), transpose, rows of the transpose (length Mry)
+    auto lines = [&](double2 *data, int n, int m, int count, const double2 *tw, const double2 *ch,
+                     const double2 *bh) -> int {
+        const int T = std::min(1024, std::max(64, m / 4));
+        const bool blue = !is_pow2(n);
+        hipLaunchKernelGGL(k3_fft_lines, dim3(count), dim3(T), sizeof(double2) * m, st, data, n, m, tw,
+                           blue ? ch : (const double2 *)nullptr, blue ? bh : (const double2 *)nullptr);
+        NFFT2D_LAUNCH_CHECK();
+        return NFFT2D_OK;
+    };
+    if ((rc = lines(G, p.Mrx, mx, p.Mry, twx, chx, bhx))) return rc;
+    hipLaunchKernelGGL(k3_transpose, dim3((p.Mrx + 31) / 32, (p.Mry + 31) / 32), dim3(256), 0, st, G, GT, p.Mry,
+                       p.Mrx);
+    NFFT2D_LAUNCH_CHECK();
+    if ((rc = lines(GT, p.Mry, my, p.Mrx, twy, chy, bhy))) return rc;
+
+    // interpolation at the outputs
+    const int W = kTile + 2 * p.msp;
+    const size_t lds = sizeof(double2) * (size_t)W * W;
+    if (2 * p.msp + 1 <= 64) {
+        NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_interp<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds));
+        hipLaunchKernelGGL(k3_interp<1>, dim3(nb), dim3(256), lds, st, GT, sout, st_out, p,
+                           reinterpret_cast<double2 *>(F));
+    } else {
+        NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_interp<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds));
+        hipLaunchKernelGGL(k3_interp<2>, dim3(nb), dim3(256), lds, st, GT, sout, st_out, p,
+                           reinterpret_cast<double2 *>(F));
+    }
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+}  // namespace
+
+extern "C" int NUFFT_2D_GPU(const double *x, const double *y, const double *s, const double *t,
+                            const nfft2d_c128 *f, nfft2d_c128 *F, double eps, int len_in, int len_out)
+{
+    return run(x, y, s, t, f, F, eps, len_in, len_out);
+}
+
+extern "C" int NUFFT_2D_GPU_params(const double *x, const double *y, const double *s, const double *t, double eps,
+                                   int len_in, int len_out, double *out)
+{
+    int rc = check_args(x, y, s, t, (const void *)1, (const void *)1, eps, len_in, len_out);
+    if (rc) return rc;
+    if (!out) return nfft2d_fail(NFFT2D_ERR_ARG, "null output array");
+    if (len_in == 0 || len_out == 0) return nfft2d_fail(NFFT2D_ERR_ARG, "no inputs or no outputs");
+    std::lock_guard<std::mutex> lk(g_mu);
+    int dev = 0;
+    NFFT2D_HIP_TRY(hipGetDevice(&dev));
+    return params(g_ws[dev], x, y, s, t, eps, len_in, len_out, out);
+}
+
+extern "C" int NUFFT_2D_GPU_host(const double *x, const double *y, const double *s, const double *t,
+                                 const nfft2d_c128 *f, nfft2d_c128 *F, double eps, int len_in, int len_out)
+{
+    int rc = check_args(x, y, s, t, f, F, eps, len_in, len_out);
+    if (rc) return rc;
+    if (len_out == 0) return NFFT2D_OK;
+    const size_t ni = (size_t)len_in, no = (size_t)len_out;
+    void *buf = nullptr;
+    const size_t bytes = al(16 * ni) + al(16 * ni) + al(16 * no) * 2;
+    if (hipMalloc(&buf, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "NUFFT_2D_GPU_host: %zu bytes", bytes);
+    }
+    unsigned char *b = static_cast<unsigned char *>(buf);
+    double *dx = reinterpret_cast<double *>(b), *dy = dx + ni;
+    nfft2d_c128 *df = reinterpret_cast<nfft2d_c128 *>(b + al(16 * ni));
+    double *ds = reinterpret_cast<double *>(b + al(16 * ni) * 2), *dt = ds + no;
+    nfft2d_c128 *dF = reinterpret_cast<nfft2d_c128 *>(b + al(16 * ni) * 2 + al(16 * no));
+    hipError_t e = hipSuccess;
+    if (ni) {
+        if (e == hipSuccess) e = hipMemcpy(dx, x, 8 * ni, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dy, y, 8 * ni, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(df, f, 16 * ni, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(ds, s, 8 * no, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, t, 8 * no, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(buf);
+        return nfft2d_fail(NFFT2D_ERR_HIP, "NUFFT_2D_GPU_host: copy in: %s", hipGetErrorString(e));
+    }
+    rc = run(ni ? dx : nullptr, ni ? dy : nullptr, ds, dt, ni ? df : nullptr, dF, eps, len_in, len_out);
+    if (rc == NFFT2D_OK) {
+        e = hipMemcpy(F, dF, 16 * no, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = nfft2d_fail(NFFT2D_ERR_HIP, "NUFFT_2D_GPU_host: copy out: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(buf);
+    return rc;
+}

@@ -19,6 +19,10 @@ algorithm:
   closed forms every fast result is checked against).
 * ``window``, ``window_deriv``, ``bessel_i0``, ``spectral_window``, ``fft2`` --
   the pieces, exposed so that each can be pinned on its own.
+* ``nufft3_2d`` -- the type-3 (non-uniform -> non-uniform) Gaussian-gridding
+  NUFFT, ``NFFT/NFFT3/C/NFFT3_2D_C.cpp:199-312`` (``NUFFT_2D_CPU``), with
+  ``nufft3_params`` (its parameters, :214-267) and the direct FP64 sum
+  ``ndft3_2d``.
 * ``oracle.knab`` (its own module) -- the band-limited polar interpolation
   ``BP_FBP_FFBP/knabNFFT2D.m`` and its helpers, on ``nfft1_2d``.
 
@@ -43,6 +47,12 @@ Pins (what checks each function, in ``tests/test_oracle.py``):
   forms; ``nfft1_1d`` / ``nfft2_1d``: those direct sums (the algorithm's
   error at c = 2, K = 6 is ~1e-12), the adjoint identity between them, and
   the 2-D restatement on one mode row.
+* ``ndft3_2d``: a numpy brute-force sum.  ``nufft3_2d``: that direct sum at
+  eps = 1e-6, 1e-9, 1e-12 (the Gaussian gridding's error is ~eps; each of three
+  planted mistakes -- no checkerboard sign, no post-compensation, flipped
+  pre-modulation -- fails it), anisotropic and off-centre extents, and
+  single-input closed forms.  ``nufft3_params``: the grid is even and at least
+  the bound of :237-238, and the scaled extents are +-pi/2.
 """
 from __future__ import annotations
 
@@ -109,6 +119,13 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = None
         fn.argtypes = [_dp] * 3 + [ctypes.c_int] * 2
+    lib.orc_nufft3_2d.restype = None
+    lib.orc_nufft3_2d.argtypes = [_dp] * 6 + [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.c_int]
+    lib.orc_nufft3_params.restype = None
+    lib.orc_nufft3_params.argtypes = [_dp] * 4 + [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
+    lib.orc_ndft3_2d.restype = None
+    lib.orc_ndft3_2d.argtypes = [_dp] * 6 + [ctypes.c_int, ctypes.c_int]
     lib.orc_num_threads.restype = ctypes.c_int
     lib.orc_set_num_threads.argtypes = [ctypes.c_int]
     _lib = lib
@@ -319,6 +336,45 @@ def ndft1d_type1(d: np.ndarray, x, N: int) -> np.ndarray:
     _load().orc_ndft1d_type1(_ptr(out.view(np.float64), ctypes.c_double), _ptr(dv, ctypes.c_double),
                              _ptr(x, ctypes.c_double), N, len(x))
     return out
+
+
+def _reals(*arrs):
+    return [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+
+
+def nufft3_2d(x, y, s, t, f, eps: float, mutation: int = 0):
+    """Type-3 2-D NUFFT, NFFT/NFFT3/C/NFFT3_2D_C.cpp:199-312 (NUFFT_2D_CPU) restated step by step:
+    F_j ~ sum_k f_k e^{-i (s_j x_k + t_j y_k)}.  Returns (F, (Mrx, Mry)).  mutation != 0 plants a
+    known mistake (tests only): 1 no checkerboard sign, 2 no post-compensation, 3 flipped
+    pre-modulation phase."""
+    x, y, s, t = _reals(x, y, s, t)
+    fd, fv = _line(f)
+    F = np.zeros(len(s), dtype=np.complex128)
+    g = (ctypes.c_int * 2)()
+    _load().orc_nufft3_2d(_ptr(F.view(np.float64), ctypes.c_double), _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+                          _ptr(s, ctypes.c_double), _ptr(t, ctypes.c_double), _ptr(fv, ctypes.c_double), float(eps),
+                          len(x), len(s), g, int(mutation))
+    return F, (g[0], g[1])
+
+
+def nufft3_params(x, y, s, t, eps: float) -> dict:
+    """The parameters NUFFT_2D_CPU derives from the data extents (NFFT3_2D_C.cpp:214-267)."""
+    x, y, s, t = _reals(x, y, s, t)
+    out = np.zeros(14)
+    _load().orc_nufft3_params(_ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), _ptr(s, ctypes.c_double),
+                              _ptr(t, ctypes.c_double), float(eps), len(x), len(s), _ptr(out, ctypes.c_double))
+    keys = ["xb", "yb", "sb", "tb", "kkx", "kky", "Dx", "Dy", "b", "t1", "fact", "msp", "Mrx", "Mry"]
+    return {k: (int(v) if k in ("msp", "Mrx", "Mry") else float(v)) for k, v in zip(keys, out)}
+
+
+def ndft3_2d(x, y, s, t, f) -> np.ndarray:
+    """Direct FP64 type-3 sum F_j = sum_k f_k e^{-i (s_j x_k + t_j y_k)}."""
+    x, y, s, t = _reals(x, y, s, t)
+    fd, fv = _line(f)
+    F = np.zeros(len(s), dtype=np.complex128)
+    _load().orc_ndft3_2d(_ptr(F.view(np.float64), ctypes.c_double), _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+                         _ptr(s, ctypes.c_double), _ptr(t, ctypes.c_double), _ptr(fv, ctypes.c_double), len(x), len(s))
+    return F
 
 
 def rel_l2(a, b) -> float:

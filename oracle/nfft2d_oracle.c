@@ -43,6 +43,7 @@
  *             2 = skip the ifftshift before the FFT,
  *             3 = window tap offset shifted by one grid cell.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -431,5 +432,159 @@ void orc_ndft1d_type1(double *F, const double *d, const double *x, int N, int M)
         }
         F[2 * i] = sr;
         F[2 * i + 1] = si;
+    }
+}
+
+/* -------------------------------------------------------------------------- */
+/* Type-3 (non-uniform to non-uniform), FP64.                                    */
+
+/* Parameters of NFFT/NFFT3/C/NFFT3_2D_C.cpp:214-267 (NUFFT_2D_CPU) from the data
+ * extents: R = 2.2, msp (taps half-width before and after :267), the grid
+ * Mrx x Mry, its steps Dx, Dy, the Gaussian's b and t1, the centres and the
+ * rescaling kk_x, kk_y of x (divided) and s (multiplied). */
+typedef struct {
+    double xb, yb, sb, tb, kkx, kky, Dx, Dy, b, t1, fact;
+    int msp, Mrx, Mry;
+} orc_nufft3_par;
+
+static void minmax(const double *a, int n, double *mx, double *mn)
+{
+    double hi = -(double)FLT_MAX, lo = (double)FLT_MAX;   /* the start values of :214-215 */
+    for (int i = 0; i < n; ++i) {
+        if (a[i] > hi) hi = a[i];
+        if (a[i] < lo) lo = a[i];
+    }
+    *mx = hi;
+    *mn = lo;
+}
+
+void orc_nufft3_params(const double *x, const double *y, const double *s, const double *t, double eps, int len_in,
+                       int len_out, double *out /* 11 doubles + 3 ints as doubles */)
+{
+    double Mx, mx, My, my, Ms, ms, Mt, mt;
+    minmax(x, len_in, &Mx, &mx);
+    minmax(y, len_in, &My, &my);
+    minmax(s, len_out, &Ms, &ms);
+    minmax(t, len_out, &Mt, &mt);
+    double xb = (Mx + mx) / 2, X1 = fabs(Mx - xb);
+    double yb = (My + my) / 2, Y1 = fabs(My - yb);
+    double sb = (Ms + ms) / 2, S = fabs(Ms - sb);
+    double tb = (Mt + mt) / 2, T = fabs(Mt - tb);
+    const double R = 2.2;
+    double msp = 2 + 2 * (R * R) * (-log(eps / 76)) / (ORC_PI * (R * R - 2));
+    int Mrx = 2 * (int)ceil((X1 * S / ORC_PI) * R * R + R * msp);
+    int Mry = 2 * (int)ceil((Y1 * T / ORC_PI) * R * R + R * msp);
+    double Dx = 2 * ORC_PI / Mrx, Dy = 2 * ORC_PI / Mry;
+    double kkx = X1 / (ORC_PI / 2), kky = Y1 / (ORC_PI / 2);
+    Mx /= kkx; mx /= kkx; xb = (Mx + mx) / 2;
+    Ms *= kkx; ms *= kkx; sb = (Ms + ms) / 2;
+    My /= kky; my /= kky; yb = (My + my) / 2;
+    Mt *= kky; mt *= kky; tb = (Mt + mt) / 2;
+    double b = (msp - 2) / (4 * ORC_PI), t1 = 1 / (4 * b);
+    msp = floor(msp - 2);
+    double fact = 1. / (4. * ORC_PI * b);
+    out[0] = xb; out[1] = yb; out[2] = sb; out[3] = tb; out[4] = kkx; out[5] = kky;
+    out[6] = Dx; out[7] = Dy; out[8] = b; out[9] = t1; out[10] = fact * fact;
+    out[11] = msp; out[12] = Mrx; out[13] = Mry;
+}
+
+/* NUFFT_2D_CPU step by step: spatialConvolutionFunction (:38-85, Gaussian
+ * gridding of the pre-modulated samples), the forward 2-D DFT of the
+ * Mry x Mrx grid (:287-292), circAndFFTShiftsFunction (:135-151, half-size
+ * shift with the (-1)^(i+j) checkerboard), and
+ * spectralConvolutionFunctionCompensation (:156-194, Gaussian interpolation
+ * at the outputs and the post-compensation).  f, F interleaved complex.
+ * Returns the grid size in grid[0..1] (Mrx, Mry).  mutation (tests only):
+ * 1 drops the checkerboard sign, 2 the post-compensation factor, 3 flips the
+ * sign of the pre-modulation phase. */
+void orc_nufft3_2d(double *F, const double *x0, const double *y0, const double *s0, const double *t0, const double *f,
+                   double eps, int len_in, int len_out, int *grid, int mutation)
+{
+    double par[14];
+    orc_nufft3_params(x0, y0, s0, t0, eps, len_in, len_out, par);
+    const double xb = par[0], yb = par[1], sb = par[2], tb = par[3], kkx = par[4], kky = par[5];
+    const double Dx = par[6], Dy = par[7], b = par[8], t1 = par[9], fact = par[10];
+    const int msp = (int)par[11], Mrx = (int)par[12], Mry = (int)par[13];
+    grid[0] = Mrx;
+    grid[1] = Mry;
+    double *g = (double *)calloc((size_t)2 * Mrx * Mry, sizeof(double));
+    for (int k = 0; k < len_in; ++k) {
+        const double x = x0[k] / kkx, y = y0[k] / kky;
+        const double arg = (mutation == 3 ? -1 : 1) * (sb * x + tb * y);
+        const double re = f[2 * k] * cos(arg) + f[2 * k + 1] * sin(arg);
+        const double im = f[2 * k + 1] * cos(arg) - f[2 * k] * sin(arg);
+        const double xt = (x - xb) / Dx, yt = (y - yb) / Dy;
+        const int nno = (int)floor(Mrx / 2 + xt), mmo = (int)floor(Mry / 2 + yt);
+        for (int mm = -msp; mm <= msp; ++mm) {
+            const int j = mm + mmo;
+            const double dsj = j - Mry / 2;
+            for (int nn = -msp; nn <= msp; ++nn) {
+                const int i = nn + nno;
+                const double dsi = i - Mrx / 2;
+                const double e = fact * exp(b * (Dy * Dy * dsj * dsj + Dx * Dx * dsi * dsi) -
+                                            t1 * ((xt - dsi) * (xt - dsi) + (yt - dsj) * (yt - dsj)));
+                const size_t idx = (size_t)j * Mrx + i;
+                g[2 * idx] += re * e;
+                g[2 * idx + 1] += im * e;
+            }
+        }
+    }
+    /* forward 2-D DFT: rows (length Mrx), then columns (length Mry) */
+    {
+        const int nmax = Mrx > Mry ? Mrx : Mry;
+        double *buf = (double *)malloc(sizeof(double) * 2 * nmax), *tmp = (double *)malloc(sizeof(double) * 2 * nmax);
+        for (int j = 0; j < Mry; ++j) line_dft_d(g + (size_t)2 * j * Mrx, Mrx, 1, -1, buf, tmp);
+        for (int i = 0; i < Mrx; ++i) line_dft_d(g + (size_t)2 * i, Mry, Mrx, -1, buf, tmp);
+        free(buf); free(tmp);
+    }
+    /* shift with the checkerboard sign */
+    double *gs = (double *)malloc(sizeof(double) * 2 * (size_t)Mrx * Mry);
+    for (int i = 0; i < Mrx; ++i)
+        for (int j = 0; j < Mry; ++j) {
+            const int ii = (i + Mrx / 2) % Mrx, jj = (j + Mry / 2) % Mry;
+            const double sg = mutation == 1 ? 1. : 1. - 2 * ((i + j) & 1);
+            gs[2 * ((size_t)jj * Mrx + ii)] = sg * g[2 * ((size_t)j * Mrx + i)];
+            gs[2 * ((size_t)jj * Mrx + ii) + 1] = sg * g[2 * ((size_t)j * Mrx + i) + 1];
+        }
+    for (int kk = 0; kk < len_out; ++kk) {
+        const double s = s0[kk] * kkx, t = t0[kk] * kky;
+        const int nsn = (int)floor(Mrx / 2 + (s - sb));
+        const double difs = (Mrx / 2 + (s - sb)) - nsn;
+        const int ntn = (int)floor(Mry / 2 + (t - tb));
+        const double dift = (Mry / 2 + (t - tb)) - ntn;
+        const double temp = mutation == 2 ? 1. : exp(b * (Dy * (t - tb) * Dy * (t - tb) + Dx * (s - sb) * Dx * (s - sb)));
+        double ar = 0, ai = 0;
+        for (int nn = nsn - msp; nn <= nsn + msp; ++nn)
+            for (int mm = ntn - msp; mm <= ntn + msp; ++mm) {
+                const double npi = mm - ntn, npj = nn - nsn;
+                const double e = exp(-t1 * ((dift - npi) * (dift - npi) + (difs - npj) * (difs - npj)));
+                const size_t idx = (size_t)mm * Mrx + nn;
+                ar += gs[2 * idx] * e;
+                ai += gs[2 * idx + 1] * e;
+            }
+        const double arg = (s - sb) * xb + (t - tb) * yb;
+        F[2 * kk] = (ar * cos(arg) + ai * sin(arg)) * temp;
+        F[2 * kk + 1] = (ai * cos(arg) - ar * sin(arg)) * temp;
+    }
+    free(g);
+    free(gs);
+}
+
+/* Direct type-3 sum F_j = sum_k f_k e^{-i (s_j x_k + t_j y_k)} (the quantity
+ * NUFFT_2D_CPU approximates; phases are not reduced: cos/sin of FP64 take any
+ * argument). */
+void orc_ndft3_2d(double *F, const double *x, const double *y, const double *s, const double *t, const double *f,
+                  int len_in, int len_out)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < len_out; ++j) {
+        double ar = 0, ai = 0;
+        for (int k = 0; k < len_in; ++k) {
+            const double a = s[j] * x[k] + t[j] * y[k], c = cos(a), sn = sin(a);
+            ar += f[2 * k] * c + f[2 * k + 1] * sn;
+            ai += f[2 * k + 1] * c - f[2 * k] * sn;
+        }
+        F[2 * j] = ar;
+        F[2 * j + 1] = ai;
     }
 }

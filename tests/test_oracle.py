@@ -312,3 +312,68 @@ def test_1d_adjoint_and_2d_consistency():
     y2 = oracle.nfft1_2d(f2, np.zeros(M), x, round_mode=1)
     assert oracle.rel_l2(oracle.nfft1_1d(f, x), y2) < TOL_FAST
 
+
+
+# ---------------------------------------------------------------- type-3 (NFFT3_2D_C.cpp)
+def _type3_case(seed, n_in, n_out, xr, yr, sr, tr):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(*xr, n_in)
+    y = rng.uniform(*yr, n_in)
+    s = rng.uniform(*sr, n_out)
+    t = rng.uniform(*tr, n_out)
+    f = rng.standard_normal(n_in) + 1j * rng.standard_normal(n_in)
+    return x, y, s, t, f
+
+
+def test_ndft3_vs_numpy():
+    x, y, s, t, f = _type3_case(5, 60, 45, (-3, 2), (0, 4), (-7, 9), (-2, 2))
+    ref = np.exp(-1j * (np.outer(s, x) + np.outer(t, y))) @ f
+    assert oracle.rel_l2(oracle.ndft3_2d(x, y, s, t, f), ref) < 1e-13
+
+
+@pytest.mark.parametrize("eps", [1e-6, 1e-9, 1e-12])
+@pytest.mark.parametrize("case", [(11, 400, 300, (-4, 4), (-4, 4), (-5, 5), (-5, 5)),
+                                  (12, 300, 350, (1, 9), (-2, -1), (-3, 12), (20, 26)),   # off-centre, anisotropic
+                                  (13, 500, 200, (-1, 1), (-6, 6), (-30, 30), (-2, 3))])
+def test_nufft3_restatement_vs_direct_sum(eps, case):
+    """NUFFT_2D_CPU restated step by step approximates the exact type-3 sum to ~eps (relative L2;
+    the bound is 10 eps, the measured errors are 0.01-1 eps)."""
+    x, y, s, t, f = _type3_case(*case)
+    F, (Mrx, Mry) = oracle.nufft3_2d(x, y, s, t, f, eps)
+    assert oracle.rel_l2(F, oracle.ndft3_2d(x, y, s, t, f)) < 10 * eps
+    assert Mrx % 2 == 0 and Mry % 2 == 0
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3])
+def test_nufft3_planted_mistakes_are_caught(mutation):
+    x, y, s, t, f = _type3_case(11, 400, 300, (-4, 4), (-4, 4), (-5, 5), (-5, 5))
+    F, _ = oracle.nufft3_2d(x, y, s, t, f, 1e-9, mutation=mutation)
+    assert oracle.rel_l2(F, oracle.ndft3_2d(x, y, s, t, f)) > 1e-2
+
+
+def test_nufft3_single_input_closed_form():
+    """One input f at (x0, y0) among two anchors of zero weight: F_j = f e^{-i (s_j x0 + t_j y0)}."""
+    rng = np.random.default_rng(21)
+    s, t = rng.uniform(-6, 6, 100), rng.uniform(-3, 8, 100)
+    x = np.array([-2.0, 1.25, 3.0])
+    y = np.array([-1.0, 0.5, 2.0])
+    f = np.array([0.0, 2 - 1j, 0.0])
+    F, _ = oracle.nufft3_2d(x, y, s, t, f, 1e-10)
+    exact = (2 - 1j) * np.exp(-1j * (s * 1.25 + t * 0.5))
+    assert oracle.rel_l2(F, exact) < 1e-9
+
+
+def test_nufft3_params_invariants():
+    x, y, s, t, f = _type3_case(12, 300, 350, (1, 9), (-2, -1), (-3, 12), (20, 26))
+    for eps in (1e-3, 1e-6, 1e-12):
+        p = oracle.nufft3_params(x, y, s, t, eps)
+        X1, Y1 = (x.max() - x.min()) / 2, (y.max() - y.min()) / 2
+        S, T = (s.max() - s.min()) / 2, (t.max() - t.min()) / 2
+        msp_raw = p["msp"] + 2                                # msp = floor(msp_raw - 2)
+        assert p["Mrx"] % 2 == 0 and p["Mry"] % 2 == 0
+        assert p["Mrx"] >= 2 * (X1 * S / np.pi * 2.2 ** 2 + 2.2 * msp_raw) - 1e-9
+        assert p["Mry"] >= 2 * (Y1 * T / np.pi * 2.2 ** 2 + 2.2 * msp_raw) - 1e-9
+        # x / kk_x spans pi (half-extent pi / 2), s * kk_x spans 2 S kk_x
+        assert abs((x.max() - x.min()) / p["kkx"] - np.pi) < 1e-12
+        assert abs(p["Dx"] * p["Mrx"] - 2 * np.pi) < 1e-12
+        assert abs(p["t1"] * 4 * p["b"] - 1) < 1e-14
