@@ -11,8 +11,8 @@
 //      synchronisation -- and derives every parameter exactly as :214-267.
 //   2. k3_prep_in / k3_prep_out: per input its pre-modulated sample and grid
 //      cell, per output its cell; both are counted into 16 x 16-cell bins and
-//      sorted by bin (k3_scan, k3_scatter).
-//   3. k3_spread: one wave per 16 x 16 tile of the grid sums
+//      sorted by bin (k3_scan_*, k3_scatter).
+//   3. k3_spread: one block per 16 x 16 tile of the grid sums
 //      c_k ey_k(j) ex_k(i) over the inputs within reach of the tile, four
 //      inputs per v_mfma_f64_16x16x4f64 (the Gaussian of :76 is separable);
 //      the cell factor fact e^{b (Dx^2 dsi^2 + Dy^2 dsj^2)} multiplies the
@@ -194,41 +194,65 @@ __global__ __launch_bounds__(256) void k3_prep_out(const double *__restrict__ s0
     r.dift = ((double)(p.Mry / 2) + r.td) - r.ntn;
     r.idx = k;
     r.pad = 0;
-    const int nb = p.nbx * p.nby;
+    const int nb = p.nbx * p.nby * kTile;   // keys: bin and the row inside it
     const bool ok = r.nsn >= 0 && r.nsn < p.Mrx && r.ntn >= 0 && r.ntn < p.Mry;
-    const int bin = ok ? bin_of(r.nsn, r.ntn, p) : nb;
+    const int bin = ok ? bin_of(r.nsn, r.ntn, p) * kTile + r.ntn % kTile : nb;
     if (!ok) F[k] = make_double2(NAN, NAN);
     rec[k] = r;
     kr[k] = make_int2(bin, atomicAdd(&cnt[bin], 1));
 }
 
-// Exclusive scan of one count array per block (block 0: inputs, block 1:
-// outputs), nb + 1 entries each; start[nb + 1] = total.
-__global__ __launch_bounds__(1024) void k3_scan(const int *__restrict__ cnt_in, const int *__restrict__ cnt_out,
-                                                int nb, int *__restrict__ st_in, int *__restrict__ st_out)
+// Exclusive scan of the key counts in three steps: k3_scan_local scans each
+// 1024-key segment in LDS and writes the segment totals, k3_scan_totals scans
+// those (one block, a run of totals per thread), k3_scan_add adds them back.
+// start[n] is the total.
+__global__ __launch_bounds__(1024) void k3_scan_local(const int *__restrict__ cnt, int n, int *__restrict__ start,
+                                                      int *__restrict__ tot)
 {
     __shared__ int sums[1024];
-    const int *cnt = blockIdx.x ? cnt_out : cnt_in;
-    int *st = blockIdx.x ? st_out : st_in;
-    const int n = nb + 1;
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    const int v = i < n ? cnt[i] : 0;
+    sums[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int u = threadIdx.x >= o ? sums[threadIdx.x - o] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += u;
+        __syncthreads();
+    }
+    if (i < n) start[i] = sums[threadIdx.x] - v;
+    if (threadIdx.x == 1023) tot[blockIdx.x] = sums[1023];
+}
+
+__global__ __launch_bounds__(1024) void k3_scan_totals(int *__restrict__ tot, int n)
+{
+    __shared__ int sums[1024];
     const int per = (n + 1023) / 1024;
     const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
     int acc = 0;
-    for (int i = lo; i < hi; ++i) acc += cnt[i];
+    for (int i = lo; i < hi; ++i) acc += tot[i];
     sums[threadIdx.x] = acc;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        const int v = threadIdx.x >= o ? sums[threadIdx.x - o] : 0;
+        const int u = threadIdx.x >= o ? sums[threadIdx.x - o] : 0;
         __syncthreads();
-        sums[threadIdx.x] += v;
+        sums[threadIdx.x] += u;
         __syncthreads();
     }
     int run = sums[threadIdx.x] - acc;
     for (int i = lo; i < hi; ++i) {
-        st[i] = run;
-        run += cnt[i];
+        const int v = tot[i];
+        tot[i] = run;
+        run += v;
     }
-    if (threadIdx.x == 1023) st[n] = sums[1023];
+    if (threadIdx.x == 1023) tot[n] = sums[1023];
+}
+
+__global__ __launch_bounds__(256) void k3_scan_add(int *__restrict__ start, int n, const int *__restrict__ tot)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) start[i] += tot[i >> 10];
+    if (i == n) start[n] = tot[(n + 1023) >> 10];
 }
 
 template <typename R>
@@ -242,83 +266,128 @@ __global__ __launch_bounds__(256) void k3_scatter(const R *__restrict__ rec, con
 }
 
 // ------------------------------------------------------------ 3. Gaussian gridding
-// Block = 4 waves = a 2 x 2 group of 16 x 16 tiles; tile (tx, ty) covers cells
-// i in [16 tx, 16 tx + 16), j in [16 ty, 16 ty + 16) of the row-major Mry x Mrx
-// grid.  An input k reaches cell (i, j) when |i - nno_k| <= msp and
-// |j - mmo_k| <= msp (:65-72).  The wave scans the bins within
-// ceil(msp / 16) of its tile 64 inputs at a time, keeps those that reach the
-// tile (a per-wave list in LDS) and takes them four per MFMA step: lane l
-// supplies A[r][k] = c_k ey_k(j0 + r) and B[k][r] = ex_k(i0 + r) at
-// r = l % 16, k = l / 16, and accumulator element q of lane l is cell
-// (j0 + 4 q + l / 16, i0 + l % 16).
-__global__ __launch_bounds__(256) void k3_spread(const InRec *__restrict__ pts, const int *__restrict__ st, Par p,
-                                                 double2 *__restrict__ G)
+// Block per 16 x 16 tile (tx, ty) of the row-major Mry x Mrx grid: cells
+// i in [16 tx, 16 tx + 16), j in [16 ty, 16 ty + 16).  An input k reaches cell
+// (i, j) when |i - nno_k| <= msp and |j - mmo_k| <= msp (:65-72).  The 8
+// waves take turns over the 64-input chunks of the bins within ceil(msp / 16)
+// of the tile and sum their partial tiles in LDS at the end, in wave order.
+// In a chunk, the inputs that reach the tile go to a per-wave slot list in
+// LDS with the factors of their
+// Gaussians (fast Gaussian gridding, Greengard & Lee, SIAM Rev. 46 (2004)):
+// with u = yt - (j0 - Mry/2),
+//     e^{-t1 (yt - dsj)^2} = e^{-t1 u^2} (e^{2 t1 u})^r e^{-t1 r^2}   at row j0 + r,
+// so an input costs four exponentials per tile instead of one per tap.  The
+// slots are then taken four per MFMA step: lane l supplies
+// A[r][k] = c_k ey_k(j0 + r) and B[k][r] = ex_k(i0 + r) at r = l % 16,
+// k = l / 16 (the power by squaring on the lane's fixed r), and accumulator
+// element q of lane l is cell (j0 + 4 q + l / 16, i0 + l % 16).
+__device__ __forceinline__ double pow_r(double b, int r)   // b^r, 0 <= r < 16
 {
-    __shared__ int list[4][64];
+    double p = (r & 1) ? b : 1.0;
+    b *= b;
+    p *= (r & 2) ? b : 1.0;
+    b *= b;
+    p *= (r & 4) ? b : 1.0;
+    b *= b;
+    p *= (r & 8) ? b : 1.0;
+    return p;
+}
+
+constexpr int kSpreadWaves = 8;
+
+__global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__restrict__ pts,
+                                                                const int *__restrict__ st, Par p,
+                                                                double2 *__restrict__ G)
+{
+    // per wave: the slots of one chunk; after the sums, the partial tiles
+    __shared__ union {
+        struct {
+            double ar[kSpreadWaves][64], ai[kSpreadWaves][64], ry[kSpreadWaves][64], ex[kSpreadWaves][64],
+                rx[kSpreadWaves][64];
+            int2 d[kSpreadWaves][64];   // (nno - i0, mmo - j0)
+        } s;
+        double2 part[kSpreadWaves][kTile * kTile];
+    } sh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tx = 2 * blockIdx.x + (w & 1), ty = 2 * blockIdx.y + (w >> 1);
-    if (tx >= p.nbx || ty >= p.nby) return;   // no block-wide barrier below
+    const int tx = blockIdx.x % p.nbx, ty = blockIdx.x / p.nbx;
     const int i0 = tx * kTile, j0 = ty * kTile;
     const int r = lane & 15, ks = lane >> 4;
-    const int jrow = j0 + r, icol = i0 + r;
-    const double dsj = (double)(jrow - p.Mry / 2), dsi = (double)(icol - p.Mrx / 2);
+    const bool row_in = j0 + r < p.Mry, col_in = i0 + r < p.Mrx;
+    const double gr = exp(-p.t1 * (double)(r * r));
+    const double oy = (double)(j0 - p.Mry / 2), ox = (double)(i0 - p.Mrx / 2);
     const int msp = p.msp, reach = (msp + kTile - 1) / kTile;
     d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
     const int by0 = max(0, ty - reach), by1 = min(p.nby - 1, ty + reach);
     const int bx0 = max(0, tx - reach), bx1 = min(p.nbx - 1, tx + reach);
+    // the waves take the 64-input chunks of the candidate rows in turn
+    int chunk = 0;
     for (int by = by0; by <= by1; ++by) {
         const int s0 = st[by * p.nbx + bx0], s1 = st[by * p.nbx + bx1 + 1];   // the row's bins are contiguous
-        for (int base = s0; base < s1; base += 64) {
+        for (int base = s0; base < s1; base += 64, ++chunk) {
+            if (chunk % kSpreadWaves != w) continue;
             const int c = base + lane;
             bool in = false;
+            InRec e;
             if (c < s1) {
-                const int2 cell = *reinterpret_cast<const int2 *>(&pts[c].nno);
-                in = cell.x + msp >= i0 && cell.x - msp < i0 + kTile && cell.y + msp >= j0 &&
-                     cell.y - msp < j0 + kTile;
+                e = pts[c];
+                in = e.nno + msp >= i0 && e.nno - msp < i0 + kTile && e.mmo + msp >= j0 &&
+                     e.mmo - msp < j0 + kTile;
             }
             const unsigned long long mask = __ballot(in);
             if (mask == 0ull) continue;
-            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-            if (in) list[w][pos] = c;
+            if (in) {
+                const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                const double uy = e.yt - oy, ux = e.xt - ox;
+                const double e0y = exp(-p.t1 * (uy * uy));
+                sh.s.ar[w][pos] = e.re * e0y;
+                sh.s.ai[w][pos] = e.im * e0y;
+                sh.s.ry[w][pos] = exp(2.0 * p.t1 * uy);
+                sh.s.ex[w][pos] = exp(-p.t1 * (ux * ux));
+                sh.s.rx[w][pos] = exp(2.0 * p.t1 * ux);
+                sh.s.d[w][pos] = make_int2(e.nno - i0, e.mmo - j0);
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int n = __popcll(mask);
             for (int q = 0; q < n; q += 4) {
+                const int sl = q + ks;
                 double ar = 0.0, ai = 0.0, bb = 0.0;
-                if (q + ks < n) {
-                    const InRec *e = pts + list[w][q + ks];
-                    const double2 xy = *reinterpret_cast<const double2 *>(&e->xt);
-                    const double2 c = *reinterpret_cast<const double2 *>(&e->re);
-                    const int2 cell = *reinterpret_cast<const int2 *>(&e->nno);
-                    if (abs(jrow - cell.y) <= msp && jrow < p.Mry) {
-                        const double d = xy.y - dsj;
-                        const double ey = exp(-p.t1 * (d * d));
-                        ar = c.x * ey;
-                        ai = c.y * ey;
+                if (sl < n) {
+                    const int2 d = sh.s.d[w][sl];
+                    if (row_in && abs(r - d.y) <= msp) {
+                        const double wy = pow_r(sh.s.ry[w][sl], r) * gr;
+                        ar = sh.s.ar[w][sl] * wy;
+                        ai = sh.s.ai[w][sl] * wy;
                     }
-                    if (abs(icol - cell.x) <= msp && icol < p.Mrx) {
-                        const double d = xy.x - dsi;
-                        bb = exp(-p.t1 * (d * d));
-                    }
+                    if (col_in && abs(r - d.x) <= msp) bb = sh.s.ex[w][sl] * pow_r(sh.s.rx[w][sl], r) * gr;
                 }
                 accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bb, accr, 0, 0, 0);
                 acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bb, acci, 0, 0, 0);
             }
-            __builtin_amdgcn_wave_barrier();   // list reads done before the next chunk's writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();   // slot reads done before the next chunk's writes
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
-    const int ii = i0 + r;
-    const double ddi = (double)(ii - p.Mrx / 2);
-    const double ex = p.b * (p.Dx * p.Dx * ddi * ddi);
+    __syncthreads();   // every wave's slot reads are done: the partial tiles may overwrite them
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int jj = j0 + 4 * q + ks;
+    for (int q = 0; q < 4; ++q) sh.part[w][(4 * q + ks) * kTile + r] = make_double2(accr[q], acci[q]);
+    __syncthreads();
+    if (threadIdx.x < kTile * kTile) {
+        const int cell = threadIdx.x, ii = i0 + cell % kTile, jj = j0 + cell / kTile;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kSpreadWaves; ++ww) {   // fixed order
+            const double2 v = sh.part[ww][cell];
+            sr += v.x;
+            si += v.y;
+        }
         if (jj < p.Mry && ii < p.Mrx) {
-            const double ddj = (double)(jj - p.Mry / 2);
-            const double sc = p.fact * exp(p.b * (p.Dy * p.Dy * ddj * ddj) + ex);
-            G[(size_t)jj * p.Mrx + ii] = make_double2(accr[q] * sc, acci[q] * sc);
+            const double ddi = (double)(ii - p.Mrx / 2), ddj = (double)(jj - p.Mry / 2);
+            const double sc = p.fact * exp(p.b * (p.Dy * p.Dy * ddj * ddj) + p.b * (p.Dx * p.Dx * ddi * ddi));
+            G[(size_t)jj * p.Mrx + ii] = make_double2(sr * sc, si * sc);
         }
     }
 }
@@ -486,27 +555,36 @@ __global__ __launch_bounds__(256) void k3_transpose(const double2 *__restrict__ 
 }
 
 // ------------------------------------------------------------ 5. interpolation at the outputs
-// Block per output bin (by, bx): the patch of the shifted spectrum
-// gs[mm][nn] = (-1)^(i + j) FT[j][i], i = (nn + Mrx/2) mod Mrx,
-// j = (mm + Mry/2) mod Mry (FT the 2-D DFT, held transposed as FTt[i][j]),
-// for mm in [16 by - msp, 16 by + 16 + msp), nn likewise, goes to LDS as
-// P[nn'][mm'] (W = 16 + 2 msp per side; wrapped indices outside the grid
-// are loaded but never weighted).  One wave per output: lane l takes
-// mm' = ntn - msp - mm0 + l (and + 64 when 2 msp + 1 > 64) and sums
-// ex(nn) P[nn][mm] over the 2 msp + 1 columns, ex read lane by lane; then
-// the ey weights, a wave reduction, and the post-compensation.
-template <int NL>
-__global__ __launch_bounds__(256) void k3_interp(const double2 *__restrict__ FTt, const OutRec *__restrict__ outs,
-                                                 const int *__restrict__ st, Par p, double2 *__restrict__ F)
+// Block per output bin (by, bx), 8 waves.  The patch of the shifted spectrum
+//     gs[mm][nn] = (-1)^(i + j) FT[j][i],  i = (nn + Mrx/2) mod Mrx,
+//     j = (mm + Mry/2) mod Mry
+// (FT the 2-D DFT, held transposed as FTt[i][j]) for mm in
+// [16 by - msp, 16 by + 16 + msp) and nn likewise goes to LDS as P[c][a]
+// (c = nn offset, a = mm offset, W = 16 + 2 msp per side; wrapped indices
+// outside the grid are loaded but never weighted).  The outputs of the bin
+// come sorted by their row ntn.  A wave takes 16 of them at a time and
+// contracts, for each output j, H_j[a] = sum_c P[c][a] ex_j[c] on
+// v_mfma_f64_16x16x4f64 (A[a][k] = P[c0 + k][a0 + a] from LDS,
+// B[k][j] = ex_j[c0 + k], accumulator element q of lane l = H_{l % 16}[a0 + 4 q
+// + l / 16]) over the rows and columns that the 16 windows cover, then
+// F_j = sum_a ey_j[a] H_j[a] and the post-compensation.  Along c the taps
+// follow e^{-t1 (d - c - 4)^2} = e^{-t1 (d - c)^2} rho_c, rho_{c+4} = rho_c
+// e^{-32 t1} (two multiplications per step).
+constexpr int kInterpWaves = 8;
+
+__global__ __launch_bounds__(64 * kInterpWaves) void k3_interp(const double2 *__restrict__ FTt,
+                                                                const OutRec *__restrict__ outs,
+                                                                const int *__restrict__ st, Par p,
+                                                                double2 *__restrict__ F)
 {
     extern __shared__ double2 P[];
     const int bin = blockIdx.x;
-    const int s0 = st[bin], s1 = st[bin + 1];
+    const int s0 = st[bin * kTile], s1 = st[bin * kTile + kTile];   // the bin's 16 row keys
     if (s0 == s1) return;
     const int bx = bin % p.nbx, by = bin / p.nbx;
     const int msp = p.msp, W = kTile + 2 * msp, taps = 2 * msp + 1;
     const int mm0 = by * kTile - msp, nn0 = bx * kTile - msp;
-    for (int e = threadIdx.x; e < W * W; e += 256) {
+    for (int e = threadIdx.x; e < W * W; e += 64 * kInterpWaves) {
         const int a = e % W, c = e / W;   // a: mm offset (contiguous in FTt), c: nn offset
         int mm = mm0 + a, nn = nn0 + c;
         mm += mm < 0 ? p.Mry : (mm >= p.Mry ? -p.Mry : 0);
@@ -520,48 +598,57 @@ __global__ __launch_bounds__(256) void k3_interp(const double2 *__restrict__ FTt
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int o = s0 + w; o < s1; o += 4) {
-        const OutRec rec = outs[o];
-        const int ra = rec.ntn - msp - mm0, ca = rec.nsn - msp - nn0;   // window origin in the patch
-        double ey[NL], ex[NL], accr[NL], acci[NL];
+    const int jo = lane & 15, kq = lane >> 4;
+    const double kappa = exp(-32.0 * p.t1);
+    for (int g = s0 + kTile * w; g < s1; g += kTile * kInterpWaves) {
+        const int o = g + jo;
+        const bool valid = o < s1;
+        const OutRec rec = outs[valid ? o : s1 - 1];
+        const int ra = rec.ntn - msp - mm0, ca = rec.nsn - msp - nn0;   // window origin in the patch, in [0, 16)
+        int rlo = valid ? ra : 1 << 20, rhi = valid ? ra : -(1 << 20);
+        int clo = valid ? ca : 1 << 20, chi = valid ? ca : -(1 << 20);
 #pragma unroll
-        for (int h = 0; h < NL; ++h) {
-            const int l = lane + 64 * h;
-            const double npi = (double)(l - msp);
-            const double dy = rec.dift - npi, dx = rec.difs - npi;
-            ey[h] = l < taps ? exp(-p.t1 * (dy * dy)) : 0.0;
-            ex[h] = l < taps ? exp(-p.t1 * (dx * dx)) : 0.0;
-            accr[h] = acci[h] = 0.0;
+        for (int off = 1; off < 16; off <<= 1) {
+            rlo = min(rlo, __shfl_xor(rlo, off, 64));
+            rhi = max(rhi, __shfl_xor(rhi, off, 64));
+            clo = min(clo, __shfl_xor(clo, off, 64));
+            chi = max(chi, __shfl_xor(chi, off, 64));
         }
-        for (int c = 0; c < taps; ++c) {
-            const int src = c & 63, h = c >> 6;
-            const double exv = NL == 1 || h == 0 ? ex[0] : ex[NL - 1];
-            const int lo = __builtin_amdgcn_readlane(__double2loint(exv), src);
-            const int hi = __builtin_amdgcn_readlane(__double2hiint(exv), src);
-            const double wc = __hiloint2double(hi, lo);
-            const double2 *col = P + (ca + c) * W + ra;
+        rhi += taps - 1;
+        chi += taps - 1;
+        const double dxc = rec.difs + ca + msp;   // ex_j[c] = e^{-t1 (dxc - c)^2} on c in [ca, ca + 2 msp]
+        const double dya = rec.dift + ra + msp;   // ey_j[a] = e^{-t1 (dya - a)^2} on a in [ra, ra + 2 msp]
+        double sr = 0.0, si = 0.0;
+        for (int a0 = rlo; a0 <= rhi; a0 += 16) {
+            d4 hr = {0.0, 0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0, 0.0};
+            const int arow = a0 + jo;
+            double dc = dxc - (double)(clo + kq);
+            double ev = exp(-p.t1 * (dc * dc)), rho = exp(8.0 * p.t1 * dc - 16.0 * p.t1);
+            for (int c0 = clo; c0 <= chi; c0 += 4) {
+                const int c = c0 + kq;
+                const double bv = valid && c >= ca && c < ca + taps ? ev : 0.0;
+                const double2 av = arow < W && c < W ? P[c * W + arow] : make_double2(0.0, 0.0);
+                hr = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv, hr, 0, 0, 0);
+                hi = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv, hi, 0, 0, 0);
+                ev *= rho;
+                rho *= kappa;
+            }
 #pragma unroll
-            for (int hh = 0; hh < NL; ++hh) {
-                const int l = lane + 64 * hh;
-                if (l < taps) {
-                    const double2 v = col[l];
-                    accr[hh] = fma(wc, v.x, accr[hh]);
-                    acci[hh] = fma(wc, v.y, acci[hh]);
+            for (int q = 0; q < 4; ++q) {
+                const int a = a0 + 4 * q + kq;
+                if (a >= ra && a < ra + taps) {
+                    const double da = dya - (double)a;
+                    const double ey = exp(-p.t1 * (da * da));
+                    sr = fma(ey, hr[q], sr);
+                    si = fma(ey, hi[q], si);
                 }
             }
         }
-        double sr = 0.0, si = 0.0;
-#pragma unroll
-        for (int h = 0; h < NL; ++h) {
-            sr = fma(ey[h], accr[h], sr);
-            si = fma(ey[h], acci[h], si);
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            sr += __shfl_xor(sr, off, 64);
-            si += __shfl_xor(si, off, 64);
-        }
-        if (lane == 0) {
+        sr += __shfl_xor(sr, 16, 64);
+        si += __shfl_xor(si, 16, 64);
+        sr += __shfl_xor(sr, 32, 64);
+        si += __shfl_xor(si, 32, 64);
+        if (kq == 0 && valid) {
             const double a1 = p.Dy * rec.td, a2 = p.Dx * rec.sd;
             const double temp = exp(p.b * (a1 * a1 + a2 * a2));
             const double arg = rec.sd * p.xb + rec.td * p.yb;
@@ -731,12 +818,14 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     p.nbx = (p.Mrx + kTile - 1) / kTile;
     p.nby = (p.Mry + kTile - 1) / kTile;
     const int nb = p.nbx * p.nby;
+    const int nko = nb * kTile;   // output keys: bin and the row inside it
     const size_t cells = (size_t)p.Mrx * p.Mry;
 
     // workspace: counts, starts, records, the grid and its transpose, FFT tables
     size_t need = 0;
     {
-        const size_t sz[] = {al(sizeof(int) * (nb + 2)) * 4,
+        const size_t sz[] = {al(sizeof(int) * (nb + 2)) * 2 + al(sizeof(int) * (nko + 2)) * 2,
+                             al(sizeof(int) * ((nb + 1) / 1024 + 2)) + al(sizeof(int) * ((nko + 1) / 1024 + 2)),
                              al(sizeof(InRec) * len_in) * 2, al(sizeof(int2) * len_in),
                              al(sizeof(OutRec) * len_out) * 2, al(sizeof(int2) * len_out),
                              al(sizeof(double2) * cells) * 2,
@@ -746,8 +835,9 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     }
     if ((rc = ensure(ws, need))) return rc;
     Carve cv{ws.ptr};
-    int *cnt_in = cv.take<int>(nb + 2), *cnt_out = cv.take<int>(nb + 2);
-    int *st_in = cv.take<int>(nb + 2), *st_out = cv.take<int>(nb + 2);
+    int *cnt_in = cv.take<int>(nb + 2), *cnt_out = cv.take<int>(nko + 2);
+    int *st_in = cv.take<int>(nb + 2), *st_out = cv.take<int>(nko + 2);
+    int *tot_in = cv.take<int>((nb + 1) / 1024 + 2), *tot_out = cv.take<int>((nko + 1) / 1024 + 2);
     InRec *rin = cv.take<InRec>(len_in), *sin_ = cv.take<InRec>(len_in);
     int2 *kin = cv.take<int2>(len_in);
     OutRec *rout = cv.take<OutRec>(len_out), *sout = cv.take<OutRec>(len_out);
@@ -757,7 +847,7 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     double2 *twy = cv.take<double2>(my), *bhy = cv.take<double2>(my), *chy = cv.take<double2>(p.Mry);
 
     hipStream_t st = nullptr;
-    NFFT2D_HIP_TRY(hipMemsetAsync(cnt_in, 0, al(sizeof(int) * (nb + 2)) * 2, st));   // cnt_in and cnt_out
+    NFFT2D_HIP_TRY(hipMemsetAsync(cnt_in, 0, al(sizeof(int) * (nb + 2)) + al(sizeof(int) * (nko + 2)), st));
     const unsigned gin = (unsigned)((len_in + 255) / 256), gout = (unsigned)((len_out + 255) / 256);
     hipLaunchKernelGGL(k3_prep_in, dim3(gin), dim3(256), 0, st, x, y, reinterpret_cast<const double2 *>(f), len_in,
                        p, rin, kin, cnt_in);
@@ -765,8 +855,17 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     hipLaunchKernelGGL(k3_prep_out, dim3(gout), dim3(256), 0, st, s, t, len_out, p, rout, kout, cnt_out,
                        reinterpret_cast<double2 *>(F));
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k3_scan, dim3(2), dim3(1024), 0, st, cnt_in, cnt_out, nb, st_in, st_out);
-    NFFT2D_LAUNCH_CHECK();
+    // keys: nb bins (+ 1 out-of-grid) for the inputs, nko bin rows (+ 1) for the outputs
+    for (int side = 0; side < 2; ++side) {
+        const int n = (side ? nko : nb) + 1, segs = (n + 1023) / 1024;
+        int *cnt = side ? cnt_out : cnt_in, *start = side ? st_out : st_in, *tot = side ? tot_out : tot_in;
+        hipLaunchKernelGGL(k3_scan_local, dim3(segs), dim3(1024), 0, st, cnt, n, start, tot);
+        NFFT2D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k3_scan_totals, dim3(1), dim3(1024), 0, st, tot, segs);
+        NFFT2D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k3_scan_add, dim3(n / 256 + 1), dim3(256), 0, st, start, n, tot);
+        NFFT2D_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k3_scatter<InRec>, dim3(gin), dim3(256), 0, st, rin, kin, len_in, st_in, sin_);
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k3_scatter<OutRec>, dim3(gout), dim3(256), 0, st, rout, kout, len_out, st_out, sout);
@@ -791,7 +890,7 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     if ((rc = tables(p.Mry, my, twy, chy, bhy))) return rc;
 
     // spread
-    hipLaunchKernelGGL(k3_spread, dim3((p.nbx + 1) / 2, (p.nby + 1) / 2), dim3(256), 0, st, sin_, st_in, p, G);
+    hipLaunchKernelGGL(k3_spread, dim3(nb), dim3(64 * kSpreadWaves), 0, st, sin_, st_in, p, G);
     NFFT2D_LAUNCH_CHECK();
 
     // 2-D DFT: rows (length Mrx), transpose, rows of the transpose (length Mry)
@@ -813,17 +912,9 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     // interpolation at the outputs
     const int W = kTile + 2 * p.msp;
     const size_t lds = sizeof(double2) * (size_t)W * W;
-    if (2 * p.msp + 1 <= 64) {
-        NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_interp<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds));
-        hipLaunchKernelGGL(k3_interp<1>, dim3(nb), dim3(256), lds, st, GT, sout, st_out, p,
-                           reinterpret_cast<double2 *>(F));
-    } else {
-        NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_interp<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds));
-        hipLaunchKernelGGL(k3_interp<2>, dim3(nb), dim3(256), lds, st, GT, sout, st_out, p,
-                           reinterpret_cast<double2 *>(F));
-    }
+    NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_interp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k3_interp, dim3(nb), dim3(64 * kInterpWaves), lds, st, GT, sout, st_out, p,
+                       reinterpret_cast<double2 *>(F));
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }
