@@ -268,43 +268,32 @@ __global__ __launch_bounds__(256) void k3_scatter(const R *__restrict__ rec, con
 // ------------------------------------------------------------ 3. Gaussian gridding
 // Block per 16 x 16 tile (tx, ty) of the row-major Mry x Mrx grid: cells
 // i in [16 tx, 16 tx + 16), j in [16 ty, 16 ty + 16).  An input k reaches cell
-// (i, j) when |i - nno_k| <= msp and |j - mmo_k| <= msp (:65-72).  The 8
-// waves take turns over the 64-input chunks of the bins within ceil(msp / 16)
-// of the tile and sum their partial tiles in LDS at the end, in wave order.
-// In a chunk, the inputs that reach the tile go to a per-wave slot list in
-// LDS with the factors of their
-// Gaussians (fast Gaussian gridding, Greengard & Lee, SIAM Rev. 46 (2004)):
-// with u = yt - (j0 - Mry/2),
-//     e^{-t1 (yt - dsj)^2} = e^{-t1 u^2} (e^{2 t1 u})^r e^{-t1 r^2}   at row j0 + r,
-// so an input costs four exponentials per tile instead of one per tap.  The
-// slots are then taken four per MFMA step: lane l supplies
+// (i, j) when |i - nno_k| <= msp and |j - mmo_k| <= msp (:65-72).  The 4 waves
+// take turns over the 64-input chunks of the bins within ceil(msp / 16) of
+// the tile and sum their partial tiles in LDS at the end, in wave order.
+// In a chunk, each input that reaches the tile writes its 16 row and 16
+// column Gaussian factors for the tile to a slot in LDS (zero outside its
+// reach), from two exponentials per dimension and a product recurrence (fast
+// Gaussian gridding, Greengard & Lee, SIAM Rev. 46 (2004)): with
+// u = yt - (j0 - Mry/2),
+//     e^{-t1 (u - r - 1)^2} = e^{-t1 (u - r)^2} rho_r,
+//     rho_r = e^{2 t1 (u - r) - t1},  rho_{r+1} = rho_r e^{-2 t1}.
+// The slots are then taken four per MFMA step: lane l supplies
 // A[r][k] = c_k ey_k(j0 + r) and B[k][r] = ex_k(i0 + r) at r = l % 16,
-// k = l / 16 (the power by squaring on the lane's fixed r), and accumulator
-// element q of lane l is cell (j0 + 4 q + l / 16, i0 + l % 16).
-__device__ __forceinline__ double pow_r(double b, int r)   // b^r, 0 <= r < 16
-{
-    double p = (r & 1) ? b : 1.0;
-    b *= b;
-    p *= (r & 2) ? b : 1.0;
-    b *= b;
-    p *= (r & 4) ? b : 1.0;
-    b *= b;
-    p *= (r & 8) ? b : 1.0;
-    return p;
-}
-
-constexpr int kSpreadWaves = 8;
+// k = l / 16, and accumulator element q of lane l is cell
+// (j0 + 4 q + l / 16, i0 + l % 16).
+constexpr int kSpreadWaves = 4;
+constexpr int kChunk = 64;        // candidate inputs per wave and round
+constexpr int kRow = kTile + 1;   // padded slot row (doubles): lane-strided writes spread over the banks
 
 __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__restrict__ pts,
                                                                 const int *__restrict__ st, Par p,
                                                                 double2 *__restrict__ G)
 {
-    // per wave: the slots of one chunk; after the sums, the partial tiles
     __shared__ union {
         struct {
-            double ar[kSpreadWaves][64], ai[kSpreadWaves][64], ry[kSpreadWaves][64], ex[kSpreadWaves][64],
-                rx[kSpreadWaves][64];
-            int2 d[kSpreadWaves][64];   // (nno - i0, mmo - j0)
+            double ey[kSpreadWaves][kChunk][kRow], ex[kSpreadWaves][kChunk][kRow];
+            double2 c[kSpreadWaves][kChunk];
         } s;
         double2 part[kSpreadWaves][kTile * kTile];
     } sh;
@@ -312,40 +301,46 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__re
     const int tx = blockIdx.x % p.nbx, ty = blockIdx.x / p.nbx;
     const int i0 = tx * kTile, j0 = ty * kTile;
     const int r = lane & 15, ks = lane >> 4;
-    const bool row_in = j0 + r < p.Mry, col_in = i0 + r < p.Mrx;
-    const double gr = exp(-p.t1 * (double)(r * r));
     const double oy = (double)(j0 - p.Mry / 2), ox = (double)(i0 - p.Mrx / 2);
+    const double kap = exp(-2.0 * p.t1);
     const int msp = p.msp, reach = (msp + kTile - 1) / kTile;
+    const int rows = min(kTile, p.Mry - j0), cols = min(kTile, p.Mrx - i0);   // cells inside the grid
     d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
     const int by0 = max(0, ty - reach), by1 = min(p.nby - 1, ty + reach);
     const int bx0 = max(0, tx - reach), bx1 = min(p.nbx - 1, tx + reach);
-    // the waves take the 64-input chunks of the candidate rows in turn
-    int chunk = 0;
+    const InRec zero = {0.0, 0.0, 0.0, 0.0, -(1 << 29), -(1 << 29), 0, 0};   // reaches nothing
     for (int by = by0; by <= by1; ++by) {
         const int s0 = st[by * p.nbx + bx0], s1 = st[by * p.nbx + bx1 + 1];   // the row's bins are contiguous
-        for (int base = s0; base < s1; base += 64, ++chunk) {
-            if (chunk % kSpreadWaves != w) continue;
-            const int c = base + lane;
-            bool in = false;
-            InRec e;
-            if (c < s1) {
-                e = pts[c];
-                in = e.nno + msp >= i0 && e.nno - msp < i0 + kTile && e.mmo + msp >= j0 &&
-                     e.mmo - msp < j0 + kTile;
-            }
+        int base = s0 + kChunk * w;
+        InRec nxt = base + lane < s1 ? pts[base + lane] : zero;
+        for (; base < s1; base += kChunk * kSpreadWaves) {
+            const InRec e = nxt;
+            const int after = base + kChunk * kSpreadWaves + lane;
+            nxt = after < s1 ? pts[after] : zero;
+            const bool in = e.nno + msp >= i0 && e.nno - msp < i0 + kTile && e.mmo + msp >= j0 &&
+                            e.mmo - msp < j0 + kTile;
             const unsigned long long mask = __ballot(in);
             if (mask == 0ull) continue;
             if (in) {
                 const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                           __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                sh.s.c[w][pos] = make_double2(e.re, e.im);
+                // rows: reach [mmo - msp, mmo + msp] relative to j0, cells inside the grid
+                const int ylo = max(0, e.mmo - msp - j0), yhi = min(rows - 1, e.mmo + msp - j0);
+                const int xlo = max(0, e.nno - msp - i0), xhi = min(cols - 1, e.nno + msp - i0);
                 const double uy = e.yt - oy, ux = e.xt - ox;
-                const double e0y = exp(-p.t1 * (uy * uy));
-                sh.s.ar[w][pos] = e.re * e0y;
-                sh.s.ai[w][pos] = e.im * e0y;
-                sh.s.ry[w][pos] = exp(2.0 * p.t1 * uy);
-                sh.s.ex[w][pos] = exp(-p.t1 * (ux * ux));
-                sh.s.rx[w][pos] = exp(2.0 * p.t1 * ux);
-                sh.s.d[w][pos] = make_int2(e.nno - i0, e.mmo - j0);
+                double gy = exp(-p.t1 * (uy * uy)), ry = exp(2.0 * p.t1 * uy - p.t1);
+                double gx = exp(-p.t1 * (ux * ux)), rx = exp(2.0 * p.t1 * ux - p.t1);
+                double *py = sh.s.ey[w][pos], *px = sh.s.ex[w][pos];
+#pragma unroll
+                for (int q = 0; q < kTile; ++q) {
+                    py[q] = q >= ylo && q <= yhi ? gy : 0.0;
+                    px[q] = q >= xlo && q <= xhi ? gx : 0.0;
+                    gy *= ry;
+                    ry *= kap;
+                    gx *= rx;
+                    rx *= kap;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -355,13 +350,11 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__re
                 const int sl = q + ks;
                 double ar = 0.0, ai = 0.0, bb = 0.0;
                 if (sl < n) {
-                    const int2 d = sh.s.d[w][sl];
-                    if (row_in && abs(r - d.y) <= msp) {
-                        const double wy = pow_r(sh.s.ry[w][sl], r) * gr;
-                        ar = sh.s.ar[w][sl] * wy;
-                        ai = sh.s.ai[w][sl] * wy;
-                    }
-                    if (col_in && abs(r - d.x) <= msp) bb = sh.s.ex[w][sl] * pow_r(sh.s.rx[w][sl], r) * gr;
+                    const double2 c = sh.s.c[w][sl];
+                    const double wy = sh.s.ey[w][sl][r];
+                    ar = c.x * wy;
+                    ai = c.y * wy;
+                    bb = sh.s.ex[w][sl][r];
                 }
                 accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bb, accr, 0, 0, 0);
                 acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bb, acci, 0, 0, 0);
@@ -399,8 +392,14 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__re
 // span Ns reads v_r = X[j + r m/R], twiddles it by e^{-2 pi i r (j mod Ns) /
 // (Ns R)}, and writes the R-point DFT of v to X[(j - j mod Ns) R + j mod Ns
 // + r Ns].  All reads of a stage precede its writes (barrier), so one buffer
-// serves.  tw[t] = e^{-2 pi i t / m}.  Every thread holds at most 8 values.
-__device__ void lds_fft(double2 *X, int m, const double2 *__restrict__ tw)
+// serves.  Twiddles e^{-2 pi i t / m} = T1[t / 64] T2[t % 64] from two small
+// LDS tables.  Every thread holds at most 8 values.
+struct TwLds {
+    const double2 *t1, *t2;
+    __device__ double2 operator[](int t) const { return cmul(t1[t >> 6], t2[t & 63]); }
+};
+
+__device__ void lds_fft(double2 *X, int m, const TwLds tw)
 {
     const int T = blockDim.x, tid = threadIdx.x;
     for (int Ns = 1; Ns < m;) {
@@ -475,14 +474,22 @@ __device__ void lds_fft(double2 *X, int m, const double2 *__restrict__ tw)
 //   X_k = w_k sum_j (x_j w_j) conj(w_{k-j}),  w_j = e^{-i pi j^2 / n},
 // the convolution taken as IFFT(FFT(x w) * bh), bh = FFT of the wrapped
 // conj(w) sequence, and IFFT(Y) = conj(FFT(conj(Y))) / m.
+// LDS: m data values, then the twiddle tables T1 (m / 64 entries, at least
+// 1) and T2 (64).
+__host__ __device__ constexpr int fft_lds_entries(int m) { return m + (m >= 64 ? m / 64 : 1) + 64; }
+
 __global__ __launch_bounds__(1024) void k3_fft_lines(double2 *__restrict__ data, int n, int m,
-                                                     const double2 *__restrict__ tw,
+                                                     const double2 *__restrict__ twg,
                                                      const double2 *__restrict__ chirp,
                                                      const double2 *__restrict__ bh)
 {
     extern __shared__ double2 X[];
+    double2 *t1 = X + m, *t2 = t1 + (m >= 64 ? m / 64 : 1);
     double2 *row = data + (size_t)blockIdx.x * n;
     const int T = blockDim.x;
+    for (int k = threadIdx.x; k < 64; k += T) t2[k] = k < m ? twg[k] : make_double2(1.0, 0.0);
+    for (int k = threadIdx.x; k < (m >= 64 ? m / 64 : 1); k += T) t1[k] = twg[64 * k];
+    const TwLds tw{t1, t2};
     if (!chirp) {
         for (int k = threadIdx.x; k < m; k += T) X[k] = row[k];
         __syncthreads();
@@ -879,13 +886,13 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
         hipLaunchKernelGGL(k3_chirp, dim3((m + 255) / 256), dim3(256), 0, st, n, m, ch, bh);
         NFFT2D_LAUNCH_CHECK();
         const int T = std::min(1024, std::max(64, m / 4));
-        hipLaunchKernelGGL(k3_fft_lines, dim3(1), dim3(T), sizeof(double2) * m, st, bh, m, m, tw,
+        hipLaunchKernelGGL(k3_fft_lines, dim3(1), dim3(T), sizeof(double2) * fft_lds_entries(m), st, bh, m, m, tw,
                            (const double2 *)nullptr, (const double2 *)nullptr);
         NFFT2D_LAUNCH_CHECK();
         return NFFT2D_OK;
     };
     NFFT2D_HIP_TRY(hipFuncSetAttribute((const void *)k3_fft_lines, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)sizeof(double2) * kMaxFft));
+                                       (int)sizeof(double2) * fft_lds_entries(kMaxFft)));
     if ((rc = tables(p.Mrx, mx, twx, chx, bhx))) return rc;
     if ((rc = tables(p.Mry, my, twy, chy, bhy))) return rc;
 
@@ -898,7 +905,8 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
                      const double2 *bh) -> int {
         const int T = std::min(1024, std::max(64, m / 4));
         const bool blue = !is_pow2(n);
-        hipLaunchKernelGGL(k3_fft_lines, dim3(count), dim3(T), sizeof(double2) * m, st, data, n, m, tw,
+        hipLaunchKernelGGL(k3_fft_lines, dim3(count), dim3(T), sizeof(double2) * fft_lds_entries(m), st, data, n, m,
+                           tw,
                            blue ? ch : (const double2 *)nullptr, blue ? bh : (const double2 *)nullptr);
         NFFT2D_LAUNCH_CHECK();
         return NFFT2D_OK;
