@@ -615,7 +615,8 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
     return NFFT2D_OK;
 }
 
-static unsigned sub_blocks(const nfft2d_plan *p) { return (unsigned)p->subcap; }
+// the subproblem capacity rounded up to a multiple of 64 (the XCD-ordered kernels deal runs of 8)
+static unsigned sub_blocks(const nfft2d_plan *p) { return (unsigned)((p->subcap + 63) & ~63); }
 
 template <typename T, int MODE>
 static int gather(nfft2d_plan *p, void *out, void *d1, void *d2, const void *g, void *gx1, void *gx2, hipStream_t s)

@@ -58,6 +58,15 @@ __device__ __forceinline__ int xcd_block(int b, int n)
     return (b & 7) * (n >> 3) + (b >> 3);
 }
 
+// For work of uneven cost in logical order (subproblems of a point set's
+// bins): every XCD takes runs of 8 consecutive logical blocks, the runs dealt
+// round-robin, so that neighbouring subproblems share an L2 while the XCDs
+// stay evenly loaded.  Grid: a multiple of 64.
+__device__ __forceinline__ int xcd_runs(int b)
+{
+    return (b & ~63) | ((b & 7) << 3) | ((b >> 3) & 7);
+}
+
 // Row-group / column-block sizes of the FFT passes.
 template <int N> constexpr int rows_per_block() { return N >= 1024 ? 1 : 1024 / N; }
 template <int N> constexpr int cols_per_block() { return N >= 2048 ? 1 : 2048 / N; }
@@ -1315,8 +1324,9 @@ k_gather_pairs(const float2* __restrict__ Gt, BinGeom geo, const PtRec<float>* _
                const int4* __restrict__ sub, const int* __restrict__ nsub, WinArgs wa, float2* __restrict__ out)
 {
     const int ns = *nsub;
-    if ((int)blockIdx.x >= ns) return;
-    gather_pairs_body<NSUB>(blockIdx.x, ns, Gt, geo, pts, sub, wa, out);
+    const int s = xcd_runs(blockIdx.x);   // runs of neighbouring subproblems share an L2
+    if (s >= ns) return;
+    gather_pairs_body<NSUB>(s, ns, Gt, geo, pts, sub, wa, out);
 }
 
 template <typename T, int MODE>
@@ -1684,8 +1694,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int* __restrict__ nsub, const int* __restrict__ subbeg, const int* __restrict__ tot, int maxpts, BinGeom geo,
     WinArgs wa, float2* __restrict__ scratch)
 {
-    const int s = blockIdx.x;
-    if (s >= *nsub) return;
+    const int ns = *nsub;
+    const int s = xcd_runs(blockIdx.x);
+    if (s >= ns) return;
     spread_halo_body<SORTED>(s, sub[s], pts, data, subbeg, tot, maxpts, geo, wa, scratch);
 }
 
@@ -1878,7 +1889,7 @@ __global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict_
                                                       const int* __restrict__ subbeg, BinGeom geo, int nb2,
                                                       cplx<T>* __restrict__ Gt)
 {
-    spread_merge_body<T, R1, R2>(blockIdx.x, scratch, subbeg, geo, nb2, Gt);
+    spread_merge_body<T, R1, R2>(xcd_block(blockIdx.x, gridDim.x), scratch, subbeg, geo, nb2, Gt);
 }
 
 }  // namespace nfft2d
