@@ -332,6 +332,15 @@ int NUFFT_2D_GPU_host(const double *x, const double *y, const double *s, const d
 int NUFFT_2D_GPU_params(const double *x, const double *y, const double *s, const double *t, double eps, int len_in,
                         int len_out, double *params);
 
+/* ------------------------------------------------------------------------
+ * Testing hooks (no counterpart in the reference).  Every device allocation of
+ * the library is numbered in order; nfft2d_debug_fail_alloc(n) makes the n-th
+ * allocation from now (0 = the next) fail as if the device were out of
+ * memory, once; n < 0 disables it.  nfft2d_debug_alloc_count() returns how
+ * many allocations were attempted so far.  For tests of the error paths. */
+void nfft2d_debug_fail_alloc(long long n);
+long long nfft2d_debug_alloc_count(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,8 @@ _SIGS = {
     "Calculate_cuFFT_plan_Z2Z_NFFT2_2D": (None, [_i, _i]),
     "Destroy_cuFFT_plan_NFFT2_2D": (None, []),
     "NDFT2_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i]),
+    "nfft2d_debug_fail_alloc": (None, [ctypes.c_longlong]),
+    "nfft2d_debug_alloc_count": (ctypes.c_longlong, []),
     "NUFFT_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i, _i]),
     "NUFFT_2D_GPU_host": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i, _i]),
     "NUFFT_2D_GPU_params": (_i, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _vp]),

@@ -72,6 +72,31 @@ def build_reference_calls(verbose: bool = False) -> str:
     return REFCALLS_BIN
 
 
+HOSTCHK_SRC = os.path.join(ROOT, "tests", "cpp", "host_checks.cpp")
+HOSTCHK_BIN = os.path.join(ROOT, "tests", "cpp", "host_checks")
+
+
+def build_host_checks(verbose: bool = False) -> str:
+    """Compile tests/cpp/host_checks.cpp together with the library sources, the host
+    code under AddressSanitizer (device code is built as for the library)."""
+    tmp = HOSTCHK_BIN + f".tmp{os.getpid()}"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O1", "-g", "-std=c++17", "-fPIC",
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
+           "-I", os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument", "-o", tmp, HOSTCHK_SRC]
+    cmd += [os.path.join(CSRC, s) for s in SOURCES]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise RuntimeError("hipcc failed on host_checks.cpp:\n" + res.stdout[-8000:])
+    os.replace(tmp, HOSTCHK_BIN)
+    return HOSTCHK_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_reference_calls(verbose=True))
+    print(build_host_checks(verbose=True))

@@ -285,8 +285,8 @@ extern "C" int nfft2d_ndft_plan_create(nfft2d_ndft_plan **plan, int N1, int N2, 
     p->nch = (N1 + ndft::kChunkK1 - 1) / ndft::kChunkK1;
     p->nks = (N2 + 8 * ndft::kKsPad - 1) / (8 * ndft::kKsPad) * ndft::kKsPad;
     const size_t gbytes = (size_t)p->nch * p->nks * kNdftC * 64 * ndft_slice_bytes(precision);
-    if (hipMalloc(&p->G, gbytes) != hipSuccess || hipMalloc((void **)&p->scale, sizeof(float) * N1) != hipSuccess ||
-        hipMalloc((void **)&p->inv_scale, sizeof(float) * N1) != hipSuccess) {
+    if (nfft2d_malloc(&p->G, gbytes) != hipSuccess || nfft2d_malloc(&p->scale, sizeof(float) * N1) != hipSuccess ||
+        nfft2d_malloc(&p->inv_scale, sizeof(float) * N1) != hipSuccess) {
         (void)hipGetLastError();
         nfft2d_ndft_plan_destroy(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft2d_ndft_plan_create: device allocation of %zu bytes failed", gbytes);

@@ -14,6 +14,7 @@
 #include <type_traits>
 #include <vector>
 #include <algorithm>
+#include <atomic>
 
 #include "../../include/nfft2d.h"
 #include "nfft2d_internal.hpp"
@@ -37,6 +38,23 @@ int nfft2d_fail(int code, const char *fmt, ...)
 }
 
 extern "C" const char *nfft2d_last_error(void) { return g_last_error.c_str(); }
+
+// ============================================================== device allocations
+static std::atomic<long long> g_alloc_seq{0};     // allocations attempted so far
+static std::atomic<long long> g_alloc_fail{-1};   // sequence number of the one to fail, or -1
+
+hipError_t nfft2d_malloc_raw(void **ptr, size_t bytes)
+{
+    const long long k = g_alloc_seq.fetch_add(1);
+    if (k == g_alloc_fail.load()) {
+        *ptr = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(ptr, bytes);
+}
+
+extern "C" long long nfft2d_debug_alloc_count(void) { return g_alloc_seq.load(); }
+extern "C" void nfft2d_debug_fail_alloc(long long n) { g_alloc_fail.store(n < 0 ? -1 : g_alloc_seq.load() + n); }
 extern "C" int nfft2d_version(void) { return 100; }
 
 // ============================================================== FP64 window math (host)
@@ -294,7 +312,7 @@ static int upload_tables(nfft2d_plan *p)
         }
         void *&twd = dim == 0 ? p->tw1 : p->tw2;
         void *&Dd = dim == 0 ? p->D1 : p->D2;
-        if (hipMalloc(&twd, sizeof(C) * 2 * N) != hipSuccess || hipMalloc(&Dd, sizeof(T) * N) != hipSuccess)
+        if (nfft2d_malloc(&twd, sizeof(C) * 2 * N) != hipSuccess || nfft2d_malloc(&Dd, sizeof(T) * N) != hipSuccess)
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan tables: hipMalloc failed");
         NFFT2D_HIP_TRY(hipMemcpy(twd, tw.data(), sizeof(C) * 2 * N, hipMemcpyHostToDevice));
         NFFT2D_HIP_TRY(hipMemcpy(Dd, D.data(), sizeof(T) * N, hipMemcpyHostToDevice));
@@ -337,7 +355,7 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
     if (dtype == NFFT2D_C128) {
         WinTables64 wt;
         make_window_tables64(wt);
-        if (hipMalloc(&p->w64, sizeof(wt)) != hipSuccess) {
+        if (nfft2d_malloc(&p->w64, sizeof(wt)) != hipSuccess) {
             free_plan(p);
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the FP64 window tables failed");
         }
@@ -347,13 +365,13 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
         }
         p->wa.w64 = (const WinTables64 *)p->w64;
     }
-    if (hipMalloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
-        hipMalloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
-        hipMalloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||
-        hipMalloc(&p->nsub, sizeof(int)) != hipSuccess ||
-        hipMalloc(&p->subbeg, sizeof(int) * (p->nbins + 1)) != hipSuccess ||
-        hipMalloc(&p->gexcl, sizeof(int2) * p->nbins) != hipSuccess ||
-        hipMalloc(&p->gsum, sizeof(int2) * ((p->nbins + kColBins - 1) / kColBins)) != hipSuccess) {
+    if (nfft2d_malloc(&p->grid, p->esz * (size_t)p->L1 * p->L2) != hipSuccess ||
+        nfft2d_malloc(&p->tmp, p->esz * (size_t)p->N1 * p->L2) != hipSuccess ||
+        nfft2d_malloc(&p->tot, sizeof(int) * p->nbins) != hipSuccess ||
+        nfft2d_malloc(&p->nsub, sizeof(int)) != hipSuccess ||
+        nfft2d_malloc(&p->subbeg, sizeof(int) * (p->nbins + 1)) != hipSuccess ||
+        nfft2d_malloc(&p->gexcl, sizeof(int2) * p->nbins) != hipSuccess ||
+        nfft2d_malloc(&p->gsum, sizeof(int2) * ((p->nbins + kColBins - 1) / kColBins)) != hipSuccess) {
         free_plan(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan_create: hipMalloc of the grid failed");
     }
@@ -427,9 +445,9 @@ static int ensure_point_buffers(nfft2d_plan *p, int M)
     if (M > p->Mcap || subcap > p->subcap) {
         free_points(p);
         const int cap = std::max(M, 1);
-        if (hipMalloc(&p->pts, sizeof(PtRec<T>) * cap) != hipSuccess ||
-            hipMalloc(&p->rank, sizeof(int) * cap) != hipSuccess ||
-            hipMalloc(&p->sub, sizeof(int4) * std::max(subcap, 1)) != hipSuccess) {
+        if (nfft2d_malloc(&p->pts, sizeof(PtRec<T>) * cap) != hipSuccess ||
+            nfft2d_malloc(&p->rank, sizeof(int) * cap) != hipSuccess ||
+            nfft2d_malloc(&p->sub, sizeof(int4) * std::max(subcap, 1)) != hipSuccess) {
             free_points(p);
             p->M = -1;
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "setpts: hipMalloc for %d points failed", M);
@@ -442,7 +460,7 @@ static int ensure_point_buffers(nfft2d_plan *p, int M)
     const long long hneed = (long long)nblk * p->nbins;
     if (hneed > p->histcap) {
         free_hist(p);
-        if (hipMalloc(&p->hist, sizeof(int) * hneed) != hipSuccess) {
+        if (nfft2d_malloc(&p->hist, sizeof(int) * hneed) != hipSuccess) {
             p->M = -1;
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "setpts: hipMalloc of the bin histogram failed");
         }
@@ -658,7 +676,7 @@ static int spread_impl(nfft2d_plan *p, const void *data, bool sorted, hipStream_
     if (mfma_spread_ok(p)) {
         if (p->subcap > p->scratchcap) {
             free_scratch(p);
-            if (hipMalloc(&p->scratch, sizeof(C) * kTileCells * (size_t)p->subcap) != hipSuccess)
+            if (nfft2d_malloc(&p->scratch, sizeof(C) * kTileCells * (size_t)p->subcap) != hipSuccess)
                 return nfft2d_fail(NFFT2D_ERR_ALLOC, "spread: hipMalloc of %d tiles failed", p->subcap);
             p->scratchcap = p->subcap;
         }
@@ -802,7 +820,7 @@ extern "C" int nfft2d_type2_lsq(nfft2d_plan *p, void *r, void *gx1, void *gx2, c
         p->resid = nullptr;
         p->residcap = 0;
         p->resid_valid = false;
-        if (hipMalloc(&p->resid, p->esz * (size_t)p->Mcap) != hipSuccess)
+        if (nfft2d_malloc(&p->resid, p->esz * (size_t)p->Mcap) != hipSuccess)
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "type2_lsq: hipMalloc of the residual (%d points) failed", p->Mcap);
         p->residcap = p->Mcap;
     }
@@ -1168,7 +1186,7 @@ extern "C" int nfft1d_plan_create(nfft1d_plan **out, int N)
     p->device = device;
     make_window_args(p->wa);
     const auto &tb = noalloc_tables(N);        // the complex64 tables of upload_tables<float>
-    if (hipMalloc(&p->tw, sizeof(float2) * 2 * N) != hipSuccess || hipMalloc(&p->D, sizeof(float) * N) != hipSuccess) {
+    if (nfft2d_malloc(&p->tw, sizeof(float2) * 2 * N) != hipSuccess || nfft2d_malloc(&p->D, sizeof(float) * N) != hipSuccess) {
         free_plan1d(p);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft1d_plan_create: hipMalloc of the tables failed");
     }
@@ -1196,8 +1214,8 @@ static int plan1d_rows(nfft1d_plan *p, int B)
     hfree(p->ones); hfree(p->grid);
     p->ones = p->grid = nullptr;
     p->rows = 0;
-    if (hipMalloc(&p->ones, sizeof(float) * B) != hipSuccess ||
-        hipMalloc(&p->grid, sizeof(float2) * (size_t)B * p->L) != hipSuccess)
+    if (nfft2d_malloc(&p->ones, sizeof(float) * B) != hipSuccess ||
+        nfft2d_malloc(&p->grid, sizeof(float2) * (size_t)B * p->L) != hipSuccess)
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "nfft1d: hipMalloc for %d lines failed", B);
     std::vector<float> one(B, 1.0f);
     NFFT2D_HIP_TRY(hipMemcpy(p->ones, one.data(), sizeof(float) * B, hipMemcpyHostToDevice));
@@ -1307,7 +1325,7 @@ static constexpr int kBatchGatherWaveBlocks = 256 * 4;   // 4 blocks of 4 waves 
 static void batch_free(nfft2d_batch *b)
 {
     if (!b) return;
-    for (auto *p : b->plans) nfft2d_plan_destroy(p);
+    for (auto *p : b->plans) free_plan(p);   // null for plans a failed create never made
     hfree(b->ddesc); hfree(b->own); hfree(b->base); hfree(b->bsub);
     b->fk.destroy();
     delete b;
@@ -1400,10 +1418,10 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     });
     add_stage(b->off_merge, b->n_merge, [&](int t) { return b->hdesc[t].nbins; });
     b->bsubcap = std::max<long long>(subcap, 1);
-    if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * count) != hipSuccess ||
-        hipMalloc(&b->own, sizeof(int2) * own.size()) != hipSuccess ||
-        hipMalloc(&b->base, sizeof(int) * (count + 1)) != hipSuccess ||
-        hipMalloc(&b->bsub, sizeof(int4) * b->bsubcap) != hipSuccess) {
+    if (nfft2d_malloc(&b->ddesc, sizeof(BatchDesc) * count) != hipSuccess ||
+        nfft2d_malloc(&b->own, sizeof(int2) * own.size()) != hipSuccess ||
+        nfft2d_malloc(&b->base, sizeof(int) * (count + 1)) != hipSuccess ||
+        nfft2d_malloc(&b->bsub, sizeof(int4) * b->bsubcap) != hipSuccess) {
         (void)hipGetLastError();
         batch_free(b);
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_create: device allocation failed");
@@ -1547,7 +1565,7 @@ extern "C" int nfft2d_batch_set_io_type1(nfft2d_batch *b, const nfft2d_c64 *cons
         nfft2d_plan *p = b->plans[t];
         if (p->subcap > p->scratchcap) {
             free_scratch(p);
-            if (hipMalloc(&p->scratch, sizeof(float2) * kTileCells * (size_t)p->subcap) != hipSuccess)
+            if (nfft2d_malloc(&p->scratch, sizeof(float2) * kTileCells * (size_t)p->subcap) != hipSuccess)
                 return nfft2d_fail(NFFT2D_ERR_ALLOC, "batch_set_io_type1: hipMalloc of the spread tiles failed");
             p->scratchcap = p->subcap;
         }

@@ -25,3 +25,13 @@ int nfft2d_check_device(int device);
         hipError_t e_ = hipGetLastError();                                                                 \
         if (e_ != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
     } while (0)
+
+// Every device allocation of the library goes through here.  A test hook
+// (nfft2d_debug_fail_alloc) can make one chosen allocation fail, so that each
+// error path runs.
+hipError_t nfft2d_malloc_raw(void **ptr, size_t bytes);
+template <typename T>
+inline hipError_t nfft2d_malloc(T **ptr, size_t bytes)
+{
+    return nfft2d_malloc_raw(reinterpret_cast<void **>(ptr), bytes);
+}

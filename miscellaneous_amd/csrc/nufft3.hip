@@ -730,7 +730,7 @@ int ensure(Workspace &ws, size_t bytes)
         ws.size = 0;
     }
     const size_t want = bytes + bytes / 4;
-    if (hipMalloc(&ws.ptr, want) != hipSuccess) {
+    if (nfft2d_malloc(&ws.ptr, want) != hipSuccess) {
         (void)hipGetLastError();
         ws.ptr = nullptr;
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "type-3 workspace of %zu bytes", want);
@@ -957,7 +957,7 @@ extern "C" int NUFFT_2D_GPU_host(const double *x, const double *y, const double 
     const size_t ni = (size_t)len_in, no = (size_t)len_out;
     void *buf = nullptr;
     const size_t bytes = al(16 * ni) + al(16 * ni) + al(16 * no) * 2;
-    if (hipMalloc(&buf, bytes) != hipSuccess) {
+    if (nfft2d_malloc(&buf, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return nfft2d_fail(NFFT2D_ERR_ALLOC, "NUFFT_2D_GPU_host: %zu bytes", bytes);
     }
