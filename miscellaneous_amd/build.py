@@ -80,7 +80,7 @@ def build_host_checks(verbose: bool = False) -> str:
     """Compile tests/cpp/host_checks.cpp together with the library sources, the host
     code under AddressSanitizer (device code is built as for the library)."""
     tmp = HOSTCHK_BIN + f".tmp{os.getpid()}"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O1", "-g", "-std=c++17", "-fPIC",
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O1", "-gline-tables-only", "-std=c++17", "-fPIC",
            "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
            "-I", os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument", "-o", tmp, HOSTCHK_SRC]

@@ -12,12 +12,14 @@
 //                d_far_field, 2 * Nu, 2 * Nv, M_x * M_y, (2 * Nu) * (2 * Nv));      Synthesis.cu:397,510
 // with (U, V) the integer meshgrid of Synthesis.cu:85-90 (u, v = -Nu .. Nu-1).
 // Inputs and outputs go to the binary file named by argv[1]; the test that
-// runs this program checks them against the CPU oracle.
+// runs this program checks them against the CPU oracle.  With --type3 it
+// makes the type-3 call of the reference's MEX gateway instead (below).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "nfft2d_overloads.hpp"
@@ -49,10 +51,60 @@ template <typename T> static T *dev_copy(const std::vector<T> &h)
     return d;
 }
 
+// The type-3 call of NFFT/NFFT3/C/NFFT3_2D_C_mex.cpp:18-65 with NUFFT_2D_CPU
+// replaced by NUFFT_2D_GPU_host (same arguments; fftw_complex is double[2]):
+// host arrays x, y (len_in), s, t (len_out), f as interleaved (re, im) built
+// from separate real and imaginary parts (:37-46), eps = 1e-9.  Inputs and
+// outputs go to `path`.
+static int type3_mex_call(const char *path)
+{
+    typedef double fftw_complex[2];
+    const int len_in = 3000, len_out = 2000;
+    std::vector<double> x(len_in), y(len_in), s(len_out), t(len_out), fr(len_in), fi(len_in);
+    for (int i = 0; i < len_in; ++i) {
+        x[i] = -6 + 12 * urand();
+        y[i] = -2 + 7 * urand();
+        fr[i] = urand() - 0.5;
+        fi[i] = urand() - 0.5;
+    }
+    for (int i = 0; i < len_out; ++i) {
+        s[i] = -9 + 14 * urand();
+        t[i] = -4 + 8 * urand();
+    }
+    std::vector<fftw_complex> f(len_in), F(len_out);
+    for (int i = 0; i < len_in; i++) {
+        f[i][0] = fr[i];
+        f[i][1] = fi[i];
+    }
+    const double eps[1] = {1e-9};
+    const int rc = NUFFT_2D_GPU_host(x.data(), y.data(), s.data(), t.data(), (const nfft2d_c128 *)f.data(),
+                                     (nfft2d_c128 *)F.data(), eps[0], len_in, len_out);
+    if (rc != NFFT2D_OK) {
+        std::fprintf(stderr, "NUFFT_2D_GPU_host: %s\n", nfft2d_last_error());
+        return 2;
+    }
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return 1;
+    const int32_t hdr[2] = {len_in, len_out};
+    std::fwrite(hdr, sizeof(hdr), 1, fp);
+    std::fwrite(x.data(), sizeof(double), len_in, fp);
+    std::fwrite(y.data(), sizeof(double), len_in, fp);
+    std::fwrite(s.data(), sizeof(double), len_out, fp);
+    std::fwrite(t.data(), sizeof(double), len_out, fp);
+    std::fwrite(f.data(), sizeof(fftw_complex), len_in, fp);
+    std::fwrite(F.data(), sizeof(fftw_complex), len_out, fp);
+    std::fwrite(eps, sizeof(double), 1, fp);
+    std::fclose(fp);
+    std::printf("reference_calls: NUFFT_2D_GPU_host (the NUFFT_2D_CPU call of NFFT3_2D_C_mex.cpp:65), %d -> %d: done\n",
+                len_in, len_out);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 3 && std::string(argv[1]) == "--type3") return type3_mex_call(argv[2]);
     if (argc != 2) {
-        std::fprintf(stderr, "usage: %s <output file>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <output file> | --type3 <output file>\n", argv[0]);
         return 1;
     }
     const int Nu = 44, Nv = 44, M_x = 44, M_y = 44, M = M_x * M_y, N = (2 * Nu) * (2 * Nv);

@@ -57,3 +57,34 @@ def test_reference_call_syntax(tmp_path):
     assert e_f < 1e-5 and e_d < 1e-9 and e_n < 1e-12
     # the two directions agree on the meshgrid (the reference's #ifdef NFFT / NDFT switch, Synthesis.cu:394-398)
     assert oracle.rel_l2(far_d, far_n) < 1e-9
+
+
+def test_type3_mex_call(tmp_path):
+    """NFFT/NFFT3/C/NFFT3_2D_C_mex.cpp:65 with NUFFT_2D_CPU replaced by NUFFT_2D_GPU_host: the same host
+    arrays and fftw_complex buffers; F against the oracle's restatement of NUFFT_2D_CPU and the direct sum."""
+    assert os.path.exists(BIN), "tests/cpp/reference_calls is not built: run __graft_entry__.build()"
+    out = tmp_path / "type3.bin"
+    print("case reference_calls --type3 3000 -> 2000", flush=True)
+    res = subprocess.run([BIN, "--type3", str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout
+    raw = out.read_bytes()
+    n_in, n_out = np.frombuffer(raw[:8], dtype=np.int32)
+    off = 8
+
+    def take(dtype, n):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=n, offset=off)
+        off += a.nbytes
+        return a
+
+    x, y, s, t = take(np.float64, n_in), take(np.float64, n_in), take(np.float64, n_out), take(np.float64, n_out)
+    f, F = take(np.complex128, n_in), take(np.complex128, n_out)
+    eps = take(np.float64, 1)[0]
+    assert off == len(raw)
+    R, _ = oracle.nufft3_2d(x, y, s, t, f, eps)
+    e_r = oracle.rel_l2(F, R)
+    e_d = oracle.rel_l2(F, oracle.ndft3_2d(x, y, s, t, f))
+    print(f"NUFFT_2D_GPU_host vs restatement {e_r:.2e}, vs direct sum {e_d:.2e}")
+    assert e_r < 1e-11 and e_d < 10 * eps
