@@ -10,8 +10,8 @@
 //      are finite).  The host reads them back -- the call's one
 //      synchronisation -- and derives every parameter exactly as :214-267.
 //   2. k3_prep_in / k3_prep_out: per input its pre-modulated sample and grid
-//      cell, per output its cell; both are counted into 16 x 16-cell bins and
-//      sorted by bin (k3_scan_*, k3_scatter).
+//      cell, per output its cell; both are counted into bins (inputs 8 x 8
+//      cells, outputs 16 x 16 cells and row) and sorted (k3_scan_*, k3_scatter*).
 //   3. k3_spread: one block per 16 x 16 tile of the grid sums
 //      c_k ey_k(j) ex_k(i) over the inputs within reach of the tile, four
 //      inputs per v_mfma_f64_16x16x4f64 (the Gaussian of :76 is separable);
@@ -40,7 +40,8 @@
 
 namespace nufft3 {
 
-constexpr int kTile = 16;              // grid tile and bin edge, cells
+constexpr int kTile = 16;              // grid tile and output bin edge, cells
+constexpr int kInBin = 8;              // input bin edge, cells (finer: fewer out-of-reach candidates per tile)
 constexpr int kMaxMsp = 42;            // LDS patch (16 + 2 msp)^2 x 16 B <= 160 000 B
 constexpr int kMaxFft = 8192;          // longest LDS-resident FFT (128 KB)
 constexpr int kExtentBlocks = 1024;
@@ -48,7 +49,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 struct Par {
     double xb, yb, sb, tb, kkx, kky, Dx, Dy, b, t1, fact;
-    int msp, Mrx, Mry, nbx, nby;
+    int msp, Mrx, Mry, nbx, nby;   // grid, and its 16 x 16-cell tiles (spread) / output bins (interpolation)
+    int ibx, iby;                   // 8 x 8-cell input bins
 };
 
 struct alignas(16) InRec {     // one input, in bin order
@@ -168,9 +170,9 @@ __global__ __launch_bounds__(256) void k3_prep_in(const double *__restrict__ x0,
     r.pad0 = r.pad1 = 0;
     // the extents put every cell in [Mrx / 4 - 1, 3 Mrx / 4]; the bound keeps
     // a rounding at the edge from leaving the grid
-    const int nb = p.nbx * p.nby;
+    const int nb = p.ibx * p.iby;
     const bool ok = r.nno >= 0 && r.nno < p.Mrx && r.mmo >= 0 && r.mmo < p.Mry;
-    const int bin = ok ? bin_of(r.nno, r.mmo, p) : nb;   // bin nb is never read
+    const int bin = ok ? (r.mmo / kInBin) * p.ibx + r.nno / kInBin : nb;   // bin nb is never read
     rec[k] = r;
     kr[k] = make_int2(bin, atomicAdd(&cnt[bin], 1));
 }
@@ -265,12 +267,29 @@ __global__ __launch_bounds__(256) void k3_scatter(const R *__restrict__ rec, con
     out[st[e.x] + e.y] = rec[k];
 }
 
+// The inputs' scatter also writes their cells, in the same order, to an
+// 8-byte array: the spread's reach test reads only that.
+__global__ __launch_bounds__(256) void k3_scatter_in(const InRec *__restrict__ rec, const int2 *__restrict__ kr,
+                                                     int n, const int *__restrict__ st, InRec *__restrict__ out,
+                                                     int2 *__restrict__ cells)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int2 e = kr[k];
+    const InRec r = rec[k];
+    const int pos = st[e.x] + e.y;
+    out[pos] = r;
+    cells[pos] = make_int2(r.nno, r.mmo);
+}
+
 // ------------------------------------------------------------ 3. Gaussian gridding
 // Block per 16 x 16 tile (tx, ty) of the row-major Mry x Mrx grid: cells
 // i in [16 tx, 16 tx + 16), j in [16 ty, 16 ty + 16).  An input k reaches cell
 // (i, j) when |i - nno_k| <= msp and |j - mmo_k| <= msp (:65-72).  The 4 waves
-// take turns over the 64-input chunks of the bins within ceil(msp / 16) of
-// the tile and sum their partial tiles in LDS at the end, in wave order.
+// take turns over the 64-input chunks of the 8 x 8-cell input bins within
+// ceil(msp / 8) of the tile (the reach test reads the 8-byte cell array; only
+// inputs within reach load their record) and sum their partial tiles in LDS
+// at the end, in wave order.
 // In a chunk, each input that reaches the tile writes its 16 row and 16
 // column Gaussian factors for the tile to a slot in LDS (zero outside its
 // reach), from two exponentials per dimension and a product recurrence (fast
@@ -287,6 +306,7 @@ constexpr int kChunk = 64;        // candidate inputs per wave and round
 constexpr int kRow = kTile + 1;   // padded slot row (doubles): lane-strided writes spread over the banks
 
 __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__restrict__ pts,
+                                                                const int2 *__restrict__ cells,
                                                                 const int *__restrict__ st, Par p,
                                                                 double2 *__restrict__ G)
 {
@@ -303,32 +323,36 @@ __global__ __launch_bounds__(64 * kSpreadWaves) void k3_spread(const InRec *__re
     const int r = lane & 15, ks = lane >> 4;
     const double oy = (double)(j0 - p.Mry / 2), ox = (double)(i0 - p.Mrx / 2);
     const double kap = exp(-2.0 * p.t1);
-    const int msp = p.msp, reach = (msp + kTile - 1) / kTile;
+    const int msp = p.msp, reach = (msp + kInBin - 1) / kInBin;   // in input bins
     const int rows = min(kTile, p.Mry - j0), cols = min(kTile, p.Mrx - i0);   // cells inside the grid
     d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    const int by0 = max(0, ty - reach), by1 = min(p.nby - 1, ty + reach);
-    const int bx0 = max(0, tx - reach), bx1 = min(p.nbx - 1, tx + reach);
-    const InRec zero = {0.0, 0.0, 0.0, 0.0, -(1 << 29), -(1 << 29), 0, 0};   // reaches nothing
+    constexpr int kSub = kTile / kInBin;   // input bins per tile side
+    const int by0 = max(0, kSub * ty - reach), by1 = min(p.iby - 1, kSub * ty + kSub - 1 + reach);
+    const int bx0 = max(0, kSub * tx - reach), bx1 = min(p.ibx - 1, kSub * tx + kSub - 1 + reach);
+    const int2 far = make_int2(-(1 << 29), -(1 << 29));   // reaches nothing
     for (int by = by0; by <= by1; ++by) {
-        const int s0 = st[by * p.nbx + bx0], s1 = st[by * p.nbx + bx1 + 1];   // the row's bins are contiguous
+        const int s0 = st[by * p.ibx + bx0], s1 = st[by * p.ibx + bx1 + 1];   // the row's bins are contiguous
         int base = s0 + kChunk * w;
-        InRec nxt = base + lane < s1 ? pts[base + lane] : zero;
+        int2 nxt = base + lane < s1 ? cells[base + lane] : far;
         for (; base < s1; base += kChunk * kSpreadWaves) {
-            const InRec e = nxt;
+            const int2 cell = nxt;
             const int after = base + kChunk * kSpreadWaves + lane;
-            nxt = after < s1 ? pts[after] : zero;
-            const bool in = e.nno + msp >= i0 && e.nno - msp < i0 + kTile && e.mmo + msp >= j0 &&
-                            e.mmo - msp < j0 + kTile;
+            nxt = after < s1 ? cells[after] : far;
+            const bool in = cell.x + msp >= i0 && cell.x - msp < i0 + kTile && cell.y + msp >= j0 &&
+                            cell.y - msp < j0 + kTile;
             const unsigned long long mask = __ballot(in);
             if (mask == 0ull) continue;
             if (in) {
+                const InRec *ep = pts + base + lane;   // only inputs within reach load their record
+                const double2 xy = *reinterpret_cast<const double2 *>(&ep->xt);
+                const double2 cv = *reinterpret_cast<const double2 *>(&ep->re);
                 const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                           __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                sh.s.c[w][pos] = make_double2(e.re, e.im);
+                sh.s.c[w][pos] = cv;
                 // rows: reach [mmo - msp, mmo + msp] relative to j0, cells inside the grid
-                const int ylo = max(0, e.mmo - msp - j0), yhi = min(rows - 1, e.mmo + msp - j0);
-                const int xlo = max(0, e.nno - msp - i0), xhi = min(cols - 1, e.nno + msp - i0);
-                const double uy = e.yt - oy, ux = e.xt - ox;
+                const int ylo = max(0, cell.y - msp - j0), yhi = min(rows - 1, cell.y + msp - j0);
+                const int xlo = max(0, cell.x - msp - i0), xhi = min(cols - 1, cell.x + msp - i0);
+                const double uy = xy.y - oy, ux = xy.x - ox;
                 double gy = exp(-p.t1 * (uy * uy)), ry = exp(2.0 * p.t1 * uy - p.t1);
                 double gx = exp(-p.t1 * (ux * ux)), rx = exp(2.0 * p.t1 * ux - p.t1);
                 double *py = sh.s.ey[w][pos], *px = sh.s.ex[w][pos];
@@ -824,16 +848,19 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
                            p.Mrx, p.Mry, mx, my, kMaxFft);
     p.nbx = (p.Mrx + kTile - 1) / kTile;
     p.nby = (p.Mry + kTile - 1) / kTile;
-    const int nb = p.nbx * p.nby;
-    const int nko = nb * kTile;   // output keys: bin and the row inside it
+    p.ibx = (p.Mrx + kInBin - 1) / kInBin;
+    p.iby = (p.Mry + kInBin - 1) / kInBin;
+    const int nb = p.nbx * p.nby;       // tiles / output bins
+    const int nki = p.ibx * p.iby;      // input keys: 8 x 8-cell bins
+    const int nko = nb * kTile;         // output keys: bin and the row inside it
     const size_t cells = (size_t)p.Mrx * p.Mry;
 
     // workspace: counts, starts, records, the grid and its transpose, FFT tables
     size_t need = 0;
     {
-        const size_t sz[] = {al(sizeof(int) * (nb + 2)) * 2 + al(sizeof(int) * (nko + 2)) * 2,
-                             al(sizeof(int) * ((nb + 1) / 1024 + 2)) + al(sizeof(int) * ((nko + 1) / 1024 + 2)),
-                             al(sizeof(InRec) * len_in) * 2, al(sizeof(int2) * len_in),
+        const size_t sz[] = {al(sizeof(int) * (nki + 2)) * 2 + al(sizeof(int) * (nko + 2)) * 2,
+                             al(sizeof(int) * ((nki + 1) / 1024 + 2)) + al(sizeof(int) * ((nko + 1) / 1024 + 2)),
+                             al(sizeof(InRec) * len_in) * 2, al(sizeof(int2) * len_in) * 2,
                              al(sizeof(OutRec) * len_out) * 2, al(sizeof(int2) * len_out),
                              al(sizeof(double2) * cells) * 2,
                              al(sizeof(double2) * mx) * 2 + al(sizeof(double2) * p.Mrx),
@@ -842,11 +869,11 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     }
     if ((rc = ensure(ws, need))) return rc;
     Carve cv{ws.ptr};
-    int *cnt_in = cv.take<int>(nb + 2), *cnt_out = cv.take<int>(nko + 2);
-    int *st_in = cv.take<int>(nb + 2), *st_out = cv.take<int>(nko + 2);
-    int *tot_in = cv.take<int>((nb + 1) / 1024 + 2), *tot_out = cv.take<int>((nko + 1) / 1024 + 2);
+    int *cnt_in = cv.take<int>(nki + 2), *cnt_out = cv.take<int>(nko + 2);
+    int *st_in = cv.take<int>(nki + 2), *st_out = cv.take<int>(nko + 2);
+    int *tot_in = cv.take<int>((nki + 1) / 1024 + 2), *tot_out = cv.take<int>((nko + 1) / 1024 + 2);
     InRec *rin = cv.take<InRec>(len_in), *sin_ = cv.take<InRec>(len_in);
-    int2 *kin = cv.take<int2>(len_in);
+    int2 *kin = cv.take<int2>(len_in), *cin = cv.take<int2>(len_in);
     OutRec *rout = cv.take<OutRec>(len_out), *sout = cv.take<OutRec>(len_out);
     int2 *kout = cv.take<int2>(len_out);
     double2 *G = cv.take<double2>(cells), *GT = cv.take<double2>(cells);
@@ -854,7 +881,7 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     double2 *twy = cv.take<double2>(my), *bhy = cv.take<double2>(my), *chy = cv.take<double2>(p.Mry);
 
     hipStream_t st = nullptr;
-    NFFT2D_HIP_TRY(hipMemsetAsync(cnt_in, 0, al(sizeof(int) * (nb + 2)) + al(sizeof(int) * (nko + 2)), st));
+    NFFT2D_HIP_TRY(hipMemsetAsync(cnt_in, 0, al(sizeof(int) * (nki + 2)) + al(sizeof(int) * (nko + 2)), st));
     const unsigned gin = (unsigned)((len_in + 255) / 256), gout = (unsigned)((len_out + 255) / 256);
     hipLaunchKernelGGL(k3_prep_in, dim3(gin), dim3(256), 0, st, x, y, reinterpret_cast<const double2 *>(f), len_in,
                        p, rin, kin, cnt_in);
@@ -862,9 +889,9 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     hipLaunchKernelGGL(k3_prep_out, dim3(gout), dim3(256), 0, st, s, t, len_out, p, rout, kout, cnt_out,
                        reinterpret_cast<double2 *>(F));
     NFFT2D_LAUNCH_CHECK();
-    // keys: nb bins (+ 1 out-of-grid) for the inputs, nko bin rows (+ 1) for the outputs
+    // keys: nki bins (+ 1 out-of-grid) for the inputs, nko bin rows (+ 1) for the outputs
     for (int side = 0; side < 2; ++side) {
-        const int n = (side ? nko : nb) + 1, segs = (n + 1023) / 1024;
+        const int n = (side ? nko : nki) + 1, segs = (n + 1023) / 1024;
         int *cnt = side ? cnt_out : cnt_in, *start = side ? st_out : st_in, *tot = side ? tot_out : tot_in;
         hipLaunchKernelGGL(k3_scan_local, dim3(segs), dim3(1024), 0, st, cnt, n, start, tot);
         NFFT2D_LAUNCH_CHECK();
@@ -873,7 +900,7 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
         hipLaunchKernelGGL(k3_scan_add, dim3(n / 256 + 1), dim3(256), 0, st, start, n, tot);
         NFFT2D_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k3_scatter<InRec>, dim3(gin), dim3(256), 0, st, rin, kin, len_in, st_in, sin_);
+    hipLaunchKernelGGL(k3_scatter_in, dim3(gin), dim3(256), 0, st, rin, kin, len_in, st_in, sin_, cin);
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k3_scatter<OutRec>, dim3(gout), dim3(256), 0, st, rout, kout, len_out, st_out, sout);
     NFFT2D_LAUNCH_CHECK();
@@ -897,7 +924,7 @@ int run(const double *x, const double *y, const double *s, const double *t, cons
     if ((rc = tables(p.Mry, my, twy, chy, bhy))) return rc;
 
     // spread
-    hipLaunchKernelGGL(k3_spread, dim3(nb), dim3(64 * kSpreadWaves), 0, st, sin_, st_in, p, G);
+    hipLaunchKernelGGL(k3_spread, dim3(nb), dim3(64 * kSpreadWaves), 0, st, sin_, cin, st_in, p, G);
     NFFT2D_LAUNCH_CHECK();
 
     // 2-D DFT: rows (length Mrx), transpose, rows of the transpose (length Mry)
