@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "nfft2d_internal.hpp"
 #include "nfft2d_kernels.hpp"
 #include "nfft1d_kernels.hpp"
+#include "nfft2d_czt.hpp"
 
 using namespace nfft2d;
 
@@ -237,11 +239,27 @@ struct nfft2d_plan_s {
     BinGeom geo;
     WinArgs wa;
     Fork fk;                              // setpts beside the grid passes (nfft2d_setpts_type2)
+    // chirp-z tables of a dimension whose N is not a power of two (null otherwise):
+    // [cin2 N][cout2 2N][hhat2 m][cin1 2N][cout1 N][hhat1 m][T1 m/64][T2 64]
+    void *czt[2] = {nullptr, nullptr};
+    int cztm[2] = {0, 0};
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 static int max_fft_n(int dtype) { return dtype == NFFT2D_C64 ? 4096 : 2048; }
 static bool fast_path(int N, int dtype) { return is_pow2(N) && N >= 8 && N <= max_fft_n(dtype); }
+// Lengths that are not powers of two take the chirp-z passes from 128 on (below, the
+// O(N^2) direct passes cost less than two m-point FFTs)
+static constexpr int kCztMinN = 128;
+// Largest chirp-z line: its m-point FFT (m >= 3N - 1) in LDS, at most 16384 complex64 / 8192 complex128
+template <typename T> constexpr int kCztMaxN = sizeof(T) == 4 ? 4096 : 2048;
+static bool czt_path(int N, int dtype) { return !fast_path(N, dtype) && N >= kCztMinN; }
+static int czt_m(int N)
+{
+    int m = 1;
+    while (m < 3 * N - 1) m <<= 1;   // P + Q - 1 = N + 2N - 1
+    return m;
+}
 
 static void hfree(void *ptr)
 {
@@ -273,6 +291,10 @@ static void free_hist(nfft2d_plan *p)
 static void free_plan(nfft2d_plan *p)
 {
     if (!p) return;
+    for (void *&c : p->czt) {
+        hfree(c);
+        c = nullptr;
+    }
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(p->device);
@@ -288,6 +310,133 @@ static void free_plan(nfft2d_plan *p)
     hfree(p->gexcl); hfree(p->gsum);
     (void)hipSetDevice(cur);
     delete p;
+}
+
+// In-place radix-2 FFT on the host, e^{-2 pi i jk / n}, n a power of two.
+static void host_fft(std::vector<std::complex<double>> &a)
+{
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                const double ang = -2.0 * kPi * (double)k / (double)len;
+                const std::complex<double> w(std::cos(ang), std::sin(ang));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// The chirp-z tables of one dimension (N not a power of two), in FP64 on the
+// host: c[k] = e^{-i pi k^2 / L}, L = 2N (k^2 reduced mod 2L exactly); type-2
+// passes p0 = -N/2, q0 = 0 (P = N, Q = L), type-1 passes p0 = 0, q0 = -N/2
+// (P = L, Q = N); the kernel conj(c[d + q0 - p0]) at d mod m, d in [1 - P, Q),
+// transformed; the FFT's two twiddle tables.
+template <typename T>
+static int czt_buffer(int N, void **out, int *mout)
+{
+    using C = cplx<T>;
+    const int L = 2 * N, m = czt_m(N), nt1 = m >= 64 ? m / 64 : 1;
+    auto chirp = [&](long long k) {
+        const long long q = ((k * k) % (2LL * L) + 2LL * L) % (2LL * L);
+        const double ang = -kPi * (double)q / (double)L;
+        return std::complex<double>(std::cos(ang), std::sin(ang));
+    };
+    std::vector<C> h;
+    h.reserve((size_t)3 * L + 2 * (size_t)m + nt1 + 64);
+    auto put = [&](std::complex<double> v) {
+        C c;
+        c.x = (T)v.real();
+        c.y = (T)v.imag();
+        h.push_back(c);
+    };
+    auto pass = [&](int P, int Q, int p0, int q0) {
+        for (int i = 0; i < P; ++i) put(chirp(i + p0));
+        for (int i = 0; i < Q; ++i) put(chirp(i + q0));
+        std::vector<std::complex<double>> ker(m, 0.0);
+        for (int d = 1 - P; d < Q; ++d) ker[(size_t)((d % m) + m) % m] = std::conj(chirp((long long)d + q0 - p0));
+        host_fft(ker);
+        for (int i = 0; i < m; ++i) put(ker[i]);
+    };
+    pass(N, L, -N / 2, 0);   // type-2
+    pass(L, N, 0, -N / 2);   // type-1
+    for (int j = 0; j < nt1; ++j) put(std::polar(1.0, -2.0 * kPi * 64.0 * j / m));
+    for (int j = 0; j < 64; ++j) put(std::polar(1.0, -2.0 * kPi * (double)j / m));
+    if (nfft2d_malloc(out, sizeof(C) * h.size()) != hipSuccess)
+        return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan tables: hipMalloc of the chirp-z tables failed");
+    NFFT2D_HIP_TRY(hipMemcpy(*out, h.data(), sizeof(C) * h.size(), hipMemcpyHostToDevice));
+    *mout = m;
+    return NFFT2D_OK;
+}
+template <typename T>
+static int upload_czt(nfft2d_plan *p, int dim, int N)
+{
+    return czt_buffer<T>(N, &p->czt[dim], &p->cztm[dim]);
+}
+
+// The chirp-z tables of a line length N (buffer of czt_buffer) for its type-2
+// (t2 = true) or type-1 pass.
+template <typename T>
+static CztTables<T> czt_tables_of(const void *buf, int N, int m, bool t2)
+{
+    using C = cplx<T>;
+    const int L = 2 * N;
+    const C *b = (const C *)buf;
+    CztTables<T> t;
+    t.m = m;
+    const C *t2base = b + 3 * L + 2 * m;   // after both passes' chirps and kernels
+    t.t1 = t2base;
+    t.t2 = t2base + (m >= 64 ? m / 64 : 1);
+    if (t2) {
+        t.P = N; t.Q = L;
+        t.cin = b; t.cout = b + N; t.hhat = b + N + L;
+    } else {
+        const C *o = b + N + L + m;
+        t.P = L; t.Q = N;
+        t.cin = o; t.cout = o + L; t.hhat = o + L + N;
+    }
+    return t;
+}
+template <typename T>
+static CztTables<T> czt_tables(const nfft2d_plan *p, int dim, bool t2)
+{
+    return czt_tables_of<T>(p->czt[dim], dim == 0 ? p->N1 : p->N2, p->cztm[dim], t2);
+}
+
+// Launches one chirp-z pass: `lines` lines (one block each).
+template <typename T, int SIGN, bool SCALE_IN, int U>
+static int czt_launch_u(const CztTables<T> &tb, int lines, const cplx<T> *in, long long in_ls, long long in_es,
+                        cplx<T> *out, long long out_ls, long long out_es, const T *sl, const T *se, hipStream_t s)
+{
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [] {
+        attr = hipFuncSetAttribute((const void *)k_czt<T, SIGN, SCALE_IN, U>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(cplx<T>) * czt_lds_entries(czt_m(kCztMaxN<T>))));
+    });
+    if (attr != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "chirp-z pass: %s", hipGetErrorString(attr));
+    const int nt = std::min(1024, std::max(64, tb.m / 4));
+    hipLaunchKernelGGL((k_czt<T, SIGN, SCALE_IN, U>), dim3(lines), dim3(nt), sizeof(cplx<T>) * czt_lds_entries(tb.m),
+                       s, in, in_ls, in_es, out, out_ls, out_es, sl, se, tb);
+    NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+template <typename T, int SIGN, bool SCALE_IN>
+static int czt_launch(const CztTables<T> &tb, int lines, const cplx<T> *in, long long in_ls, long long in_es,
+                      cplx<T> *out, long long out_ls, long long out_es, const T *sl, const T *se, hipStream_t s)
+{
+    // at 1024 threads a thread holds 8 values up to m = 8192, 16 at m = 16384
+    if (tb.m > 8192)
+        return czt_launch_u<T, SIGN, SCALE_IN, 4>(tb, lines, in, in_ls, in_es, out, out_ls, out_es, sl, se, s);
+    return czt_launch_u<T, SIGN, SCALE_IN, 2>(tb, lines, in, in_ls, in_es, out, out_ls, out_es, sl, se, s);
 }
 
 template <typename T>
@@ -316,6 +465,10 @@ static int upload_tables(nfft2d_plan *p)
             return nfft2d_fail(NFFT2D_ERR_ALLOC, "plan tables: hipMalloc failed");
         NFFT2D_HIP_TRY(hipMemcpy(twd, tw.data(), sizeof(C) * 2 * N, hipMemcpyHostToDevice));
         NFFT2D_HIP_TRY(hipMemcpy(Dd, D.data(), sizeof(T) * N, hipMemcpyHostToDevice));
+        if (czt_path(N, p->dtype)) {
+            int rc = upload_czt<T>(p, dim, N);
+            if (rc) return rc;
+        }
     }
     return NFFT2D_OK;
 }
@@ -330,8 +483,7 @@ extern "C" int nfft2d_plan_create(nfft2d_plan **out, int N1, int N2, int dtype, 
     const int maxN = dtype == NFFT2D_C64 ? 4096 : 2048;
     if (N1 <= 0 || N2 <= 0 || (N1 & 1) || (N2 & 1) || N1 > maxN || N2 > maxN)
         return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: N1, N2 must be even and in [2, %d] (got %d, %d)", maxN, N1, N2);
-    if ((!fast_path(N1, dtype) && N1 > 2048) || (!fast_path(N2, dtype) && N2 > 2048))
-        return nfft2d_fail(NFFT2D_ERR_SIZE, "plan_create: non-power-of-two N above 2048 is not supported");
+    static_assert(kCztMaxN<float> >= 4096 && kCztMaxN<double> >= 2048, "every even N up to the maximum has passes");
     int device = 0;
     NFFT2D_HIP_TRY(hipGetDevice(&device));      // before the allocation: nothing to free on failure
     nfft2d_plan *p = new (std::nothrow) nfft2d_plan_s();
@@ -572,6 +724,9 @@ static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
                                    D2, tw2, p->N1);
             }
         });
+    } else if (p->czt[1]) {   // rows: f[k1][*] (N2) -> Tm[k1][*] (L2), pre-compensated
+        int rc = czt_launch<T, -1, true>(czt_tables<T>(p, 1, true), p->N1, f, p->N2, 1, Tm, p->L2, 1, D1, D2, s);
+        if (rc) return rc;
     } else {
         hipLaunchKernelGGL((k_t2_rows_direct<T, -1>), dim3(p->N1), dim3(256), sizeof(C) * p->N2, s, f, Tm, D1, D2, tw2,
                            p->N1, p->N2);
@@ -586,6 +741,10 @@ static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
                                    p->L2);
             }
         });
+    } else if (p->czt[0]) {   // columns: Tm[*][n2] (N1) -> G[n2][*] (L1)
+        int rc = czt_launch<T, -1, true>(czt_tables<T>(p, 0, true), p->L2, Tm, 1, p->L2, G, p->L1, 1, (const T *)nullptr,
+                                         (const T *)nullptr, s);
+        if (rc) return rc;
     } else {
         hipLaunchKernelGGL((k_t2_cols_direct<T, -1>), dim3(p->L2), dim3(256), sizeof(C) * p->N1, s, Tm, G, tw1, p->N1,
                            p->L2);
@@ -611,6 +770,10 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
                                    tw1, p->L2);
             }
         });
+    } else if (p->czt[0]) {   // columns: G[n2][*] (L1) -> Tp[*][n2] (N1)
+        int rc = czt_launch<T, SIGN, false>(czt_tables<T>(p, 0, false), p->L2, G, p->L1, 1, Tp, 1, p->L2,
+                                            (const T *)nullptr, (const T *)nullptr, s);
+        if (rc) return rc;
     } else {
         hipLaunchKernelGGL((k_t1_cols_direct<T, SIGN>), dim3(p->L2), dim3(256), sizeof(C) * p->L1, s, G, Tp, tw1, p->N1,
                            p->L2);
@@ -625,6 +788,9 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
                                    D2, tw2, p->N1);
             }
         });
+    } else if (p->czt[1]) {   // rows: Tp[k1][*] (L2) -> F[k1][*] (N2), deconvolved
+        int rc = czt_launch<T, SIGN, false>(czt_tables<T>(p, 1, false), p->N1, Tp, p->L2, 1, F, p->N2, 1, D1, D2, s);
+        if (rc) return rc;
     } else {
         hipLaunchKernelGGL((k_t1_rows_direct<T, SIGN>), dim3(p->N1), dim3(256), sizeof(C) * p->L2, s, Tp, F, D1, D2,
                            tw2, p->N1, p->N2);
@@ -1156,6 +1322,8 @@ struct nfft1d_plan_s {
     int N = 0, L = 0, device = 0, rows = 0;   // rows: lines the grid and the unit row factors hold
     void *tw = nullptr, *D = nullptr;         // 2N twiddles, N deconvolution factors S / phi(xi_k)
     void *ones = nullptr, *grid = nullptr;    // rows x 1.0f (the row passes' per-row factor), rows x L grid
+    void *czt = nullptr;                      // chirp-z tables when N is not a power of two (from 128 on)
+    int cztm = 0;
     WinArgs wa;
 };
 
@@ -1165,7 +1333,7 @@ static void free_plan1d(nfft1d_plan *p)
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(p->device);
-    hfree(p->tw); hfree(p->D); hfree(p->ones); hfree(p->grid);
+    hfree(p->tw); hfree(p->D); hfree(p->ones); hfree(p->grid); hfree(p->czt);
     (void)hipSetDevice(cur);
     delete p;
 }
@@ -1194,6 +1362,13 @@ extern "C" int nfft1d_plan_create(nfft1d_plan **out, int N)
         hipMemcpy(p->D, tb.second.data(), sizeof(float) * N, hipMemcpyHostToDevice) != hipSuccess) {
         free_plan1d(p);
         return nfft2d_fail(NFFT2D_ERR_HIP, "nfft1d_plan_create: upload of the tables failed");
+    }
+    if (czt_path(N, NFFT2D_C64)) {
+        const int rc = czt_buffer<float>(N, &p->czt, &p->cztm);
+        if (rc) {
+            free_plan1d(p);
+            return rc;
+        }
     }
     *out = p;
     return NFFT2D_OK;
@@ -1255,6 +1430,10 @@ extern "C" int nfft1d_type2(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64
             hipLaunchKernelGGL((k_t2_rows<float, N, -1>), dim3(grid1d(batch, R)), dim3(R * (N / 8)), 0, s, f, G, ones, D,
                                tw, batch);
         });
+    } else if (p->czt) {
+        if ((rc = czt_launch<float, -1, true>(czt_tables_of<float>(p->czt, p->N, p->cztm, true), batch, f, p->N, 1, G,
+                                              p->L, 1, ones, D, s)))
+            return rc;
     } else {
         hipLaunchKernelGGL((k_t2_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->N, s, f, G, ones, D,
                            tw, batch, p->N);
@@ -1287,6 +1466,10 @@ extern "C" int nfft1d_type1(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64
             hipLaunchKernelGGL((k_t1_rows<float, N, -1>), dim3(grid1d(batch, R)), dim3(R * (N / 8)), 0, s, G, F, ones, D,
                                tw, batch);
         });
+    } else if (p->czt) {
+        if ((rc = czt_launch<float, -1, false>(czt_tables_of<float>(p->czt, p->N, p->cztm, false), batch, G, p->L, 1,
+                                               F, p->N, 1, ones, D, s)))
+            return rc;
     } else {
         hipLaunchKernelGGL((k_t1_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->L, s, G, F, ones,
                            D, tw, batch, p->N);

@@ -48,7 +48,7 @@ def test_version(lib):
     ((64, 64, 0, 6, 2, 1), -1, "flags"),
     ((8192, 64, 0, 6, 2, 0), -2, "4096"),
     ((4096, 64, 1, 6, 2, 0), -2, "2048"),
-    ((3000, 64, 0, 6, 2, 0), -2, "power-of-two"),
+    ((3000, 64, 1, 6, 2, 0), -2, "2048"),
 ])
 def test_plan_create_validation(lib, args, code, needle):
     h = ctypes.c_void_p()

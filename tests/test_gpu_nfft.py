@@ -608,3 +608,53 @@ def test_autograd_skips_the_sort_for_unchanged_points():
     plan.setpts(x2.detach().contiguous(), x1.detach().contiguous())   # re-pointed elsewhere
     y3 = m.nfft2d_type2(f, x1, x2, plan)
     assert torch.equal(y3, y2)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("N1,N2", [(1000, 1000), (130, 500), (512, 250), (2046, 128), (1998, 64)])
+def test_chirp_z_sizes(N1, N2, dtype):
+    """Sizes that are not powers of two, from 128 on: the FFT passes are chirp-z transforms
+    (Bluestein, at the power of two >= 3N - 1).  Type-2, type-1 and the adjoint (the e^{+}
+    passes) against FP64 direct sums, points over 1.5 periods."""
+    rng = np.random.default_rng(N1 * 3 + N2)
+    M = 40000
+    x1 = (rng.uniform(-0.5, 0.5, M) * N1 * 1.5)
+    x2 = (rng.uniform(-0.5, 0.5, M) * N2 * 1.5)
+    cdt = np.complex64 if dtype == torch.complex64 else np.complex128
+    rdt = np.float32 if dtype == torch.complex64 else np.float64
+    x1, x2 = x1.astype(rdt), x2.astype(rdt)
+    f = cn01(rng, (N1, N2), cdt)
+    d = cn01(rng, M, cdt)
+    tol = TOL32 if dtype == torch.complex64 else 1e-9
+    print(f"case chirp-z {N1}x{N2} {dtype}", flush=True)
+    plan = m.Plan(N1, N2, dtype, device=DEV).setpts(T(x1), T(x2))
+    y = plan.type2(T(f)).cpu().numpy()
+    sel = rng.choice(M, 1500, replace=False)
+    e2 = oracle.rel_l2(y[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel]))
+    F = plan.type1(T(d)).cpu().numpy()
+    i1, i2 = rng.integers(0, N1, 400), rng.integers(0, N2, 400)
+    e1 = oracle.rel_l2(F[i1, i2], oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2))
+    A = plan.adjoint(T(d)).cpu().numpy()   # sum_l d_l e^{+2 pi i (k1 x1 / N1 + k2 x2 / N2)}
+    ea = oracle.rel_l2(A[i1, i2], np.conj(oracle.ndft_type1_modes(np.conj(d).astype(np.complex128), x1, x2, N1,
+                                                                     N2, i1, i2)))
+    print(f"  rel-L2 type-2 {e2:.2e}, type-1 {e1:.2e}, adjoint {ea:.2e}")
+    assert e2 < tol and e1 < tol and ea < tol
+
+
+def test_chirp_z_complex64_up_to_4096():
+    """Complex64 plans take every even N up to 4096 (chirp-z lines of 16,384 points at N = 3000)."""
+    rng = np.random.default_rng(77)
+    N1, N2, M = 3000, 130, 20000
+    x1 = (rng.uniform(-0.5, 0.5, M) * N1).astype(np.float32)
+    x2 = (rng.uniform(-0.5, 0.5, M) * N2).astype(np.float32)
+    f = cn01(rng, (N1, N2))
+    d = cn01(rng, M)
+    print("case chirp-z 3000x130 complex64", flush=True)
+    plan = m.Plan(N1, N2, torch.complex64, device=DEV).setpts(T(x1), T(x2))
+    y = plan.type2(T(f)).cpu().numpy()
+    sel = rng.choice(M, 1000, replace=False)
+    assert oracle.rel_l2(y[sel], oracle.ndft_type2(f.astype(np.complex128), x1[sel], x2[sel])) < TOL32
+    F = plan.type1(T(d)).cpu().numpy()
+    i1, i2 = rng.integers(0, N1, 300), rng.integers(0, N2, 300)
+    assert oracle.rel_l2(F[i1, i2], oracle.ndft_type1_modes(d.astype(np.complex128), x1, x2, N1, N2, i1, i2)) < TOL32
+
