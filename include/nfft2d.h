@@ -117,7 +117,10 @@ int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, const double *u
 
 /* A plan holds the twiddle and deconvolution tables, the oversampled grid and
  * the tile-sorted points.  dtype: NFFT2D_C64 or NFFT2D_C128.  K must be 6 and
- * c must be 2 (the reference's constants); flags must be 0. */
+ * c must be 2 (the reference's constants); flags must be 0.  N1, N2 even, at
+ * most 4096 (complex64) or 2048 (complex128), any even value: powers of two
+ * take pruned Stockham passes, other lengths from 128 on chirp-z (Bluestein)
+ * passes, shorter ones direct O(N^2) passes. */
 int nfft2d_plan_create(nfft2d_plan **plan, int N1, int N2, int dtype, int K, int c, unsigned flags);
 int nfft2d_plan_destroy(nfft2d_plan *plan);
 
@@ -232,7 +235,8 @@ int nfft2d_ndft_type2(nfft2d_ndft_plan *plan, nfft2d_c64 *result, const nfft2d_c
 /* Batched 1-D NFFTs, complex64, K = 6, c = 2: `batch` lines per call, each
  * line with N modes and M points, stored line after line (line b: modes at
  * b*N, points and samples at b*M).  N even, at most 2048, or a power of two up
- * to 4096 (type-1: at most 2048); coordinates in grid units with period N.
+ * to 4096 (type-1: at most 2048); other lengths from 128 on take chirp-z
+ * (Bluestein) passes; coordinates in grid units with period N.
  *   nfft1d_type2  BP_FBP_FFBP/NFFT1_1D.m:1 (called once per pulse in
  *                 BP_FBP_FFBP/BP_NFFT.m:18):
  *                 result[b][l] = sum_k data[b][k + N/2] e^{-2 pi i k x[b][l] / N}
