@@ -629,11 +629,48 @@ def bench_type3(args, dev):
         ph = torch.outer(s[sel[c:c + 8]], x) + torch.outer(t[sel[c:c + 8]], y)
         ref[c:c + 8] = torch.polar(torch.ones_like(ph), -ph) @ f
     rel = ((F[sel] - ref).norm() / ref.norm()).item()
+    # the host-array form (NUFFT_2D_CPU's own contract): PCIe copies in and out included
+    hx, hy, hs, ht, hf = (a.cpu().numpy() for a in (x, y, s, t, f))
+    m.NUFFT_2D_host(hx, hy, hs, ht, hf, eps)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        m.NUFFT_2D_host(hx, hy, hs, ht, hf, eps)
+    sec_host = (time.perf_counter() - t0) / 3
     return {"config": f"type-3 NUFFT (NUFFT_2D_CPU's algorithm, FP64), {n} inputs uniform in [-50, 50]^2 -> {n} "
                       f"outputs uniform in [-10, 10]^2, eps {eps}: grid {par['Mrx']} x {par['Mry']}, "
                       f"{2 * par['msp'] + 1}^2 Gaussian taps per point",
             "ms_per_call": sec * 1e3, "value": 2 * n / sec / 1e6, "unit": "Mpts/s (inputs + outputs)",
-            "rel_l2_128": rel, "eps": eps}
+            "rel_l2_128": rel, "eps": eps,
+            "host_arrays": {"ms_per_call": sec_host * 1e3, "value": 2 * n / sec_host / 1e6,
+                            "note": "NUFFT_2D_GPU_host: host arrays in and out, the PCIe copies included"}}
+
+
+def bench_non_pow2(args, dev):
+    """Sizes that are not powers of two (chirp-z FFT passes): type-2 and type-1 at 1000 x 1000 modes against
+    1024 x 1024, 1M uniform points, complex64, eager calls (points set once)."""
+    import miscellaneous_amd as m
+
+    res = {}
+    gen = torch.Generator().manual_seed(SEED + 13)
+    M = 1 << 20
+    for N in (1000, 1024):
+        x = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+        y = ((torch.rand(M, generator=gen) - 0.5) * N).to(dev)
+        f = cn01(gen, (N, N), dev)
+        d = cn01(gen, (M,), dev)
+        p = m.Plan(N, N, torch.complex64, device=dev).setpts(x, y)
+        out = {}
+        for name, fn in (("type2_ms", lambda: p.type2(f)), ("type1_ms", lambda: p.type1(d))):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                fn()
+            torch.cuda.synchronize()
+            out[name] = (time.perf_counter() - t0) / 20 * 1e3
+        res[str(N)] = out
+    res["config"] = "type-2 / type-1 NFFT, N x N modes, M = 1048576 uniform points, complex64; 1000 runs chirp-z passes"
+    return res
 
 
 def bench_ndft_dropin(args, dev):
@@ -837,6 +874,7 @@ def main():
         extra["ndft2_2d_dropin"] = bench_ndft_dropin(args, dev)
         extra["autograd"] = bench_autograd(args, dev)
         extra["type3"] = bench_type3(args, dev)
+        extra["non_pow2"] = bench_non_pow2(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]
