@@ -255,12 +255,17 @@ def bench_batch(args, dev):
     ys = [torch.from_numpy(b).to(dev) for _, b in pts]
     outs = [torch.empty(mm, dtype=torch.complex64, device=dev) for (_, _, mm) in sizes]
     b = m.Batch(sizes, device=dev).set_points(xs, ys).set_io(fs, outs)
-    t0 = time.perf_counter()
-    choice = b.autotune(reps=3)
-    tune_s = time.perf_counter() - t0
 
     def step():             # tile sort on the batch's second stream, beside the grid passes
         b.setpts_type2()
+
+    g0 = capture(step)      # the defaults, before tuning: subproblems of 256, direct cut-off 8, 1024 blocks
+    s_default = time_graph(g0, max(args.steps // 4, 10), 3)
+    del g0
+    t0 = time.perf_counter()
+    choice = b.autotune(reps=5)
+    tune_s = time.perf_counter() - t0
+    tune = b.tuning()
 
     def step_serial():
         b.setpts()
@@ -297,9 +302,10 @@ def bench_batch(args, dev):
     same = all(torch.equal(a, c) for a, c in zip(outs, outs2))
     total_m = sum(mm for (_, _, mm) in sizes)
     nbytes = sum(200 * n1 * n2 + 16 * mm for (n1, n2, mm) in sizes)
-    tuned = {}
-    for (n1, _, _), c in zip(sizes, choice):
+    tuned, direct = {}, {}
+    for (n1, _, _), c, d in zip(sizes, choice, tune["direct"]):
         tuned[str(n1)] = c
+        direct[str(n1)] = d
     del plans, g2
     return {"config": "256 type-2 NFFTs, N uniform in {64,128,256,512} (square), M log-uniform in [4096, 262144], "
                       "half uniform / half Gaussian (sigma 0.05, wrapped) points, complex64, K=6, c=2; "
@@ -308,7 +314,9 @@ def bench_batch(args, dev):
             "ms_per_step": s_batch * 1e3, "one_stream_ms_per_step": s_serial * 1e3, "plain_ms_per_step": s_plain * 1e3, "speedup_vs_plain": s_plain / s_batch,
             "identical_to_plain": same, "rel_l2_max_16x256": worst,
             "hbm_frac": nbytes / s_batch / (HBM_PEAK_GBS * 1e9), "algorithmic_bytes": nbytes,
-            "autotuned_maxpts_by_N": tuned, "autotune_s": tune_s}
+            "autotuned_maxpts_by_N": tuned, "autotuned_direct_cutoff_by_N": direct,
+            "autotuned_gather_blocks": tune["gather_blocks"], "default_tuning_ms_per_step": s_default * 1e3,
+            "autotune_s": tune_s}
 
 
 def bench_ndft(args, dev):
@@ -508,18 +516,21 @@ def bench_knab(args, dev):
     for _ in range(3):
         out = call()
     torch.cuda.synchronize()
-    reps = max(args.steps // 10, 10)
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    reps = max(args.steps // 5, 20)
+    times = []
+    for _ in range(reps):          # each call synchronises once itself (the grid sizes go to the host)
+        t0 = time.perf_counter()
         out = call()
-    torch.cuda.synchronize()
-    s = (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    s = float(np.median(times))
     ref = sig(RO, CO)
     rel = ((out.to(torch.complex128) - ref).norm() / ref.norm()).item()
     return {"config": "knabNFFT2D: 512x512 polar samples (B = 0.5/T in both variables, P = 18), 1024x1024 "
                       "output points; spectrum FFT + window + zero padding (a = b = 2) + type-2 NFFT on "
-                      "1024x1024 modes; eager calls",
+                      "1024x1024 modes; eager calls, host-timed one by one",
             "value": RO.numel() / s / 1e6, "unit": "Mpts/s", "ms_per_call": s * 1e3,
+            "ms_per_call_mean": float(np.mean(times)) * 1e3, "calls": reps,
             "rel_l2_vs_bandlimited_signal": rel}
 
 
@@ -734,9 +745,31 @@ def bench_autograd(args, dev):
         step()
     torch.cuda.synchronize()
     s = (time.perf_counter() - t0) / reps
+
+    # the same step (forward, loss, backward) captured whole into one graph and replayed
+    def graphed_step():
+        loss = ((m.nfft2d_type2(f, x1, x2, plan) - y_obs).abs() ** 2).sum()
+        loss.backward()
+
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            f.grad = x1.grad = x2.grad = None
+            graphed_step()
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize()
+    f.grad = x1.grad = x2.grad = None
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        graphed_step()
+    s_graph = time_graph(g, reps, 3)
+    del g
     return {"config": "256x256 modes, M=262144 uniform; loss ||nfft2d_type2(f, x1, x2) - y||^2 through "
                       "torch.autograd (grad f and x); eager steps, the tile sort skipped for unchanged points",
-            "steps_per_s": 1.0 / s, "ms_per_step": s * 1e3}
+            "steps_per_s": 1.0 / s, "ms_per_step": s * 1e3,
+            "graphed": {"steps_per_s": 1.0 / s_graph, "ms_per_step": s_graph * 1e3,
+                        "note": "forward, loss and backward captured into one graph (torch.cuda.graph), replayed"}}
 
 
 def cpu_baseline(t2, seconds: float):
@@ -784,6 +817,23 @@ def cpu_baseline(t2, seconds: float):
                        f"(NFFT/NFFT2_2D.cu), FP64, and 1 thread on M/8 points", legs=legs)
 
 
+def measure_copy_bandwidth(dev, nbytes=1 << 30, reps=10):
+    """A plain device-to-device copy of 1 GiB, timed with HIP events: (read + write bytes) / time, in GB/s,
+    the STREAM-like figure beside the 8 TB/s spec."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    return gbs
+
+
 def profile_workload(args):
     """A few eager calls of one workload, for rocprofv3 counter passes (no timing of its own)."""
     import miscellaneous_amd as m
@@ -822,6 +872,23 @@ def profile_workload(args):
     print(f"profile workload {args.profile}: {args.steps} calls done", flush=True)
 
 
+# The workloads of the JSON line beside the headline and the gradient, in run order.
+EXTRA_LEGS = {
+    "batch": bench_batch,
+    "ndft": bench_ndft,
+    "large_1024": bench_large,
+    "type1_polar512": bench_type1,
+    "fp64_polar512": bench_fp64,
+    "knab_polar_interp": bench_knab,
+    "pso_generation": bench_pso,
+    "nfft_1d": bench_1d,
+    "ndft2_2d_dropin": bench_ndft_dropin,
+    "autograd": bench_autograd,
+    "type3": bench_type3,
+    "non_pow2": bench_non_pow2,
+}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -841,9 +908,16 @@ def main():
                          "BF16x3 NDFT at 256^2, M = 65536, the FP64 type-3 at 1M -> 1M, or a 256 MiB device "
                          "copy) and exit: the command "
                          "the rocprofv3 counter runs profile")
+    ap.add_argument("--only", choices=sorted(EXTRA_LEGS),
+                    help="run one of the extra workloads alone and print its JSON object")
     args = ap.parse_args()
     if args.profile:
         profile_workload(args)
+        return
+    if args.only:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps({args.only: EXTRA_LEGS[args.only](args, dev)}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
@@ -863,26 +937,27 @@ def main():
     gr = None if args.no_grad else bench_grad(args, dev)
     extra = {}
     if not args.no_extra:
-        extra["batch"] = bench_batch(args, dev)
-        extra["ndft"] = bench_ndft(args, dev)
-        extra["large_1024"] = bench_large(args, dev)
-        extra["type1_polar512"] = bench_type1(args, dev)
-        extra["fp64_polar512"] = bench_fp64(args, dev)
-        extra["knab_polar_interp"] = bench_knab(args, dev)
-        extra["pso_generation"] = bench_pso(args, dev)
-        extra["nfft_1d"] = bench_1d(args, dev)
-        extra["ndft2_2d_dropin"] = bench_ndft_dropin(args, dev)
-        extra["autograd"] = bench_autograd(args, dev)
-        extra["type3"] = bench_type3(args, dev)
-        extra["non_pow2"] = bench_non_pow2(args, dev)
+        for name, leg in EXTRA_LEGS.items():
+            extra[name] = leg(args, dev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(t2, args.cpu_seconds)
 
     M = t2["M"]
     achieved = t2["gather_bytes"] / (t2["gather_ms"] * 1e-3) / 1e9
-    traffic = None
+    traffic, tj = None, {}
     if os.path.exists(args.traffic):
         with open(args.traffic) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            tj = json.load(fh)
+        traffic = tj.get("bytes_per_launch")
+    secondary = None
+    if tj.get("valu_insts_per_launch"):
+        # VALU issue floor: a wave-64 FP32 VALU instruction occupies its SIMD 4 cycles; 1024 SIMDs at the
+        # 2.4 GHz peak engine clock (no packed FP32 here, so 16 lanes per SIMD per cycle)
+        issue_us = tj["valu_insts_per_launch"] * 4 / (1024 * 2.4e3)
+        secondary = {"bound": "valu_issue", "valu_insts_per_launch": tj["valu_insts_per_launch"],
+                     "floor_us": issue_us, "achieved_us": t2["gather_ms"] * 1e3,
+                     "frac": issue_us / (t2["gather_ms"] * 1e3),
+                     "note": "rocprofv3 SQ_INSTS_VALU of the same kernel and workload x 4 cycles / (1024 SIMDs x 2.4 GHz)"}
+    copy_gbs = measure_copy_bandwidth(dev)
     out = {
         "metric": "2D type-2 NFFT Mpts/s at rel-L2<=1e-5 (512^2, M=1M polar)",
         "value": M / t2["s_full"] / 1e6,
@@ -910,7 +985,8 @@ def main():
         "type2_hbm_frac": t2["type2_bytes"] / t2["s_exec"] / (HBM_PEAK_GBS * 1e9),
         "roofline": {"kernel": "k_gather_pairs<2> (stage c)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": t2["gather_bytes"]},
+                     "algorithmic_bytes": t2["gather_bytes"], "measured_copy_GBs": copy_gbs,
+                     "secondary": secondary},
     }
     if gr is not None:
         out["grad"] = {"steps_per_s": 1.0 / gr["s"], "ms_per_step": gr["s"] * 1e3,

@@ -278,10 +278,21 @@ int nfft1d_type1(nfft1d_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, 
  *   setpts_type2 setpts + type2, the tile sort on a second stream owned by the
  *                batch, concurrent with the grid passes (as
  *                nfft2d_setpts_type2; the first call creates the stream);
- *   autotune     choose the subproblem size (64, 128 or 256 points) per size
- *                class (N1, N2) by timing reps setpts_type2 passes of the
- *                whole batch per candidate; synchronizes the stream;
- *   set_maxpts / get_maxpts  the subproblem size of transform t ([64, 256]). */
+ *   autotune     choose, by timing reps setpts_type2 passes of the whole
+ *                batch per candidate, one parameter at a time: per size class
+ *                (N1, N2) the subproblem size (64, 128 or 256 points), then per
+ *                size class the direct-gather cut-off (0, 4, 8 or 16 points),
+ *                then the width of the persistent gather (512, 1024 or 2048
+ *                blocks); no choice changes a result bit; synchronizes the
+ *                stream;
+ *   set_maxpts / get_maxpts  the subproblem size of transform t ([64, 256]);
+ *   set_direct / get_direct  transform t's direct-gather cut-off ([0, 16]):
+ *                its subproblems of at most that many points are gathered
+ *                point by point from the grid instead of through an LDS tile
+ *                (default 8); like set_maxpts, it takes effect at the next
+ *                setpts;
+ *   set_gather_blocks / get_gather_blocks  blocks of the persistent gather
+ *                (default 1024). */
 typedef struct nfft2d_batch_s nfft2d_batch;
 int nfft2d_batch_create(nfft2d_batch **batch, int count, const int *N1, const int *N2, const int *M, unsigned flags);
 int nfft2d_batch_destroy(nfft2d_batch *batch);
@@ -294,6 +305,10 @@ int nfft2d_batch_setpts_type2(nfft2d_batch *batch, void *stream);
 int nfft2d_batch_autotune(nfft2d_batch *batch, void *stream, int reps);
 int nfft2d_batch_set_maxpts(nfft2d_batch *batch, int t, int maxpts);
 int nfft2d_batch_get_maxpts(const nfft2d_batch *batch, int t);
+int nfft2d_batch_set_direct(nfft2d_batch *batch, int t, int npts);
+int nfft2d_batch_get_direct(const nfft2d_batch *batch, int t);
+int nfft2d_batch_set_gather_blocks(nfft2d_batch *batch, int blocks);
+int nfft2d_batch_get_gather_blocks(const nfft2d_batch *batch);
 /* Type-1 of every transform (the reference's NFFT2_2D_GPU, NFFT/NFFT2_2D.cu:329,
  * once per transform; e.g. once per particle of a PSO generation,
  * Synthesis/PSO/PSO.cu:348 -> Synthesis.cu:395): result[t][k1][k2] =
@@ -335,6 +350,53 @@ int NUFFT_2D_GPU_host(const double *x, const double *y, const double *s, const d
                       nfft2d_c128 *F, double eps, int len_in, int len_out);
 int NUFFT_2D_GPU_params(const double *x, const double *y, const double *s, const double *t, double eps, int len_in,
                         int len_out, double *params);
+
+/* ------------------------------------------------------------------------
+ * Band-limited polar interpolation, the steps around its type-2 NFFT
+ * (BP_FBP_FFBP/knabNFFT2D.m:1-79 and rectNFFT2D.m:1-77, MATLAB functions with
+ * no C interface in the reference; the Python knabNFFT2D / rectNFFT2D call
+ * these with the 2-D FFT of the samples and a type-2 plan in between).
+ * Device arrays, FP64 coordinates, launched on `stream`.  rho_out and cos_out
+ * are R x C; out_strides (host, 4 entries, in elements: rho_out's row and
+ * column strides, then cos_out's) lets meshgrid views in without a copy, null
+ * means both contiguous.
+ *   nfft2d_knab_stats     out[0..9] = mean step of the rho axis and of the
+ *                         cos alpha axis (mean of the differences), their first
+ *                         values, two zeros, then min, max of the rho axis and
+ *                         of the cos alpha axis (knabNFFT2D.m:28-40, :76-77);
+ *                         out[NFFT2D_KNAB_STATS + 4 b ...] = min, max of rho_out
+ *                         and min, max of cos_out over block b's share, for b <
+ *                         NFFT2D_KNAB_STAT_BLOCKS (the caller folds them into
+ *                         the extents).  The axes are read with element
+ *                         strides.  NaN propagates.
+ *   nfft2d_knab_spectrum  F (PN x QM, dtype) from the caller's 2-D spectrum
+ *                         S (M x N, cos alpha x rho): the band of 2 kgn + 1 by
+ *                         2 kgm + 1 frequencies centred at (PN/2, QM/2) holds
+ *                         conj(S[(v - kgm) mod M][(u - kgn) mod N] w), w =
+ *                         Gn[u] Gm[v] scale (FP64, rounded once), or w = scale
+ *                         when Gn and Gm are null; zero elsewhere
+ *                         (knabNFFT2D.m:58-70, the window of KnabWindow.m).
+ *   nfft2d_knab_points    x1, x2 (R2 x C2, float for NFFT2D_C64, double for
+ *                         NFFT2D_C128) = wrap((rho_out - r0) / h1, P1) and
+ *                         wrap((cos_out - c0) / h2, P2), wrap(x, P) = x -
+ *                         P rint(x / P) in FP64; entries beyond the caller's
+ *                         R x C take 0 (the padded branch, knabNFFT2D.m:45-56).
+ *   nfft2d_knab_finish    result (R x C) = conj(y[r C2 + c]) / scale, or
+ *                         extrap where rho_out[0][c] is outside [rho_min,
+ *                         rho_max] or cos_out[r][0] outside [cos_min, cos_max]
+ *                         (knabNFFT2D.m:74-79). */
+enum { NFFT2D_KNAB_STATS = 10, NFFT2D_KNAB_STAT_BLOCKS = 256 };
+int nfft2d_knab_stats(const double *rho_axis, int n_rho, long long rho_stride, const double *cos_axis, int n_cos,
+                      long long cos_stride, const double *rho_out, const double *cos_out, int R, int C,
+                      const long long *out_strides, double *out, void *stream);
+int nfft2d_knab_spectrum(int dtype, const void *S, int M, int N, int kgn, int kgm, const double *Gn, const double *Gm,
+                         double scale, void *F, int PN, int QM, void *stream);
+int nfft2d_knab_points(int dtype, const double *rho_out, const double *cos_out, int R, int C,
+                       const long long *out_strides, int R2, int C2, double r0, double c0, double h1, double h2,
+                       double P1, double P2, void *x1, void *x2, void *stream);
+int nfft2d_knab_finish(int dtype, const void *y, int R, int C, int C2, double scale, const double *rho_out,
+                       const double *cos_out, const long long *out_strides, double rho_min, double rho_max,
+                       double cos_min, double cos_max, double extrap_re, double extrap_im, void *result, void *stream);
 
 /* ------------------------------------------------------------------------
  * Testing hooks (no counterpart in the reference).  Every device allocation of

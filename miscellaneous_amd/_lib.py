@@ -21,6 +21,8 @@ NFFT2D_C128 = 1
 NFFT2D_NDFT_FP32 = 0
 NFFT2D_NDFT_BF16X3 = 1
 NFFT2D_NDFT_FP16 = 2
+NFFT2D_KNAB_STATS = 10
+NFFT2D_KNAB_STAT_BLOCKS = 256
 
 _ERR_NAMES = {-1: "NFFT2D_ERR_ARG", -2: "NFFT2D_ERR_SIZE", -3: "NFFT2D_ERR_HIP", -4: "NFFT2D_ERR_ALLOC",
               -5: "NFFT2D_ERR_STATE", -6: "NFFT2D_ERR_UNSUPPORTED"}
@@ -53,6 +55,12 @@ _SIGS = {
     "NUFFT_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i, _i]),
     "NUFFT_2D_GPU_host": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i, _i]),
     "NUFFT_2D_GPU_params": (_i, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _vp]),
+    "nfft2d_knab_stats": (_i, [_vp, _i, ctypes.c_longlong, _vp, _i, ctypes.c_longlong, _vp, _vp, _i, _i, _vp, _vp,
+                               _vp]),
+    "nfft2d_knab_spectrum": (_i, [_i, _vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_double, _vp, _i, _i, _vp]),
+    "nfft2d_knab_points": (_i, [_i, _vp, _vp, _i, _i, _vp, _i, _i] + [ctypes.c_double] * 6 + [_vp, _vp, _vp]),
+    "nfft2d_knab_finish": (_i, [_i, _vp, _i, _i, _i, ctypes.c_double, _vp, _vp, _vp] + [ctypes.c_double] * 6
+                           + [_vp, _vp]),
     "nfft2d_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _i, ctypes.c_uint]),
     "nfft2d_plan_destroy": (_i, [_vp]),
     "nfft2d_setpts": (_i, [_vp, _vp, _vp, _i, _vp]),
@@ -95,6 +103,10 @@ _SIGS = {
     "nfft2d_batch_setpts_type1": (_i, [_vp, _vp]),
     "nfft2d_batch_set_maxpts": (_i, [_vp, _i, _i]),
     "nfft2d_batch_get_maxpts": (_i, [_vp, _i]),
+    "nfft2d_batch_set_direct": (_i, [_vp, _i, _i]),
+    "nfft2d_batch_get_direct": (_i, [_vp, _i]),
+    "nfft2d_batch_set_gather_blocks": (_i, [_vp, _i]),
+    "nfft2d_batch_get_gather_blocks": (_i, [_vp]),
 }
 
 _lib = None

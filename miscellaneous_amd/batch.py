@@ -125,10 +125,25 @@ class Batch:
             check(self._lib.nfft2d_batch_setpts_type1(self._h, _stream(self.device)), "nfft2d_batch_setpts_type1")
 
     def autotune(self, reps: int = 3) -> list:
-        """Choose the subproblem size per size class by timing; returns the per-transform choice."""
+        """Choose by timing the subproblem size and the direct-gather cut-off per size class and the
+        width of the persistent gather (``tuning()``); returns the per-transform subproblem sizes."""
         with torch.cuda.device(self.device):
             check(self._lib.nfft2d_batch_autotune(self._h, _stream(self.device), int(reps)), "nfft2d_batch_autotune")
         return self.maxpts()
+
+    def tuning(self) -> dict:
+        """The batch's current tuning: per-transform subproblem sizes and direct-gather cut-offs, and the
+        number of blocks of the persistent gather."""
+        return {"maxpts": self.maxpts(),
+                "direct": [self._lib.nfft2d_batch_get_direct(self._h, t) for t in range(self.count)],
+                "gather_blocks": self._lib.nfft2d_batch_get_gather_blocks(self._h)}
+
+    def set_direct(self, t: int, npts: int) -> None:
+        """Subproblems of transform t with at most npts (0..16) points skip the LDS tile."""
+        check(self._lib.nfft2d_batch_set_direct(self._h, int(t), int(npts)), "nfft2d_batch_set_direct")
+
+    def set_gather_blocks(self, blocks: int) -> None:
+        check(self._lib.nfft2d_batch_set_gather_blocks(self._h, int(blocks)), "nfft2d_batch_set_gather_blocks")
 
     def maxpts(self) -> list:
         return [self._lib.nfft2d_batch_get_maxpts(self._h, t) for t in range(self.count)]

@@ -13,7 +13,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libnfft2d.so")
-SOURCES = ["nfft2d_api.hip", "ndft2d.hip", "nufft3.hip"]
+SOURCES = ["nfft2d_api.hip", "ndft2d.hip", "nufft3.hip", "knab.hip"]
 ARCH = os.environ.get("NFFT2D_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1483,6 +1483,8 @@ extern "C" int nfft1d_type1(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64
 // (kernels in nfft2d_batch.hpp).
 #include "nfft2d_batch.hpp"
 
+static constexpr int kBatchGatherWaveBlocks = 256 * 4;   // 4 blocks of 4 waves per CU (LDS: 38 KB each)
+
 struct nfft2d_batch_s {
     int count = 0, device = 0;
     std::vector<nfft2d_plan *> plans;
@@ -1498,12 +1500,13 @@ struct nfft2d_batch_s {
     bool have_points = false, have_io = false, have_io1 = false;
     bool sorted = false;                  // the subproblem list matches the points and sizes (batch_setpts ran)
     Fork fk;                              // setpts beside the grid passes (nfft2d_batch_setpts_type2)
+    std::vector<int> dmax;                // per transform: direct-gather cut-off (BatchDesc::dmax)
+    int gather_blocks = kBatchGatherWaveBlocks;   // width of the persistent wave gather
 };
 
 static int batch_rows_per_block(int N) { return N >= 1024 ? 1 : 1024 / N; }
 static int batch_cols_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
 static constexpr int kBatchMinMaxpts = 64;   // smallest subproblem size the autotuner tries
-static constexpr int kBatchGatherWaveBlocks = 256 * 4;   // 4 blocks of 4 waves per CU (LDS: 38 KB each)
 
 static void batch_free(nfft2d_batch *b)
 {
@@ -1533,6 +1536,7 @@ static void batch_fill_desc(nfft2d_batch *b, int t)
     d.scratch = (float2 *)p->scratch;
     d.r1 = merge_radius(p->L1);
     d.r2 = merge_radius(p->L2);
+    d.dmax = b->dmax.empty() ? kDirectDefault : b->dmax[t];
 }
 
 static int batch_upload(nfft2d_batch *b)
@@ -1561,6 +1565,7 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     (void)hipGetDevice(&b->device);
     b->plans.assign(count, nullptr);
     b->hdesc.assign(count, BatchDesc{});
+    b->dmax.assign(count, kDirectDefault);
     std::vector<int2> own;
     long long subcap = 0;
     for (int t = 0; t < count; ++t) {
@@ -1672,6 +1677,38 @@ extern "C" int nfft2d_batch_get_maxpts(const nfft2d_batch *b, int t)
     return b->hdesc[t].maxpts;
 }
 
+extern "C" int nfft2d_batch_set_direct(nfft2d_batch *b, int t, int npts)
+{
+    if (!b || t < 0 || t >= b->count) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_direct: bad batch or index");
+    if (npts < 0 || npts > kDirectCap)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_direct: the cut-off must be in [0, %d]", kDirectCap);
+    b->dmax[t] = npts;
+    b->hdesc[t].dmax = npts;
+    b->sorted = false;                    // the subproblem list carries the cut-off: rebuilt by the next setpts
+    return batch_upload(b);
+}
+
+extern "C" int nfft2d_batch_get_direct(const nfft2d_batch *b, int t)
+{
+    if (!b || t < 0 || t >= b->count) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_get_direct: bad batch or index");
+    return b->dmax[t];
+}
+
+extern "C" int nfft2d_batch_set_gather_blocks(nfft2d_batch *b, int blocks)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_gather_blocks: null batch");
+    if (blocks < 1 || blocks > 65536)
+        return nfft2d_fail(NFFT2D_ERR_ARG, "batch_set_gather_blocks: %d outside [1, 65536]", blocks);
+    b->gather_blocks = blocks;
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_batch_get_gather_blocks(const nfft2d_batch *b)
+{
+    if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_get_gather_blocks: null batch");
+    return b->gather_blocks;
+}
+
 extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
 {
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_setpts: null batch");
@@ -1706,11 +1743,12 @@ static int batch_grid(nfft2d_batch *b, hipStream_t s)
 
 static int batch_gather(nfft2d_batch *b, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_gather_w, dim3(kBatchGatherWaveBlocks), dim3(256), 0, s, b->ddesc, b->bsub,
-                       b->base + b->count, b->plans[0]->wa);
+    hipLaunchKernelGGL(kb_gather_w, dim3(b->gather_blocks), dim3(256), 0, s, b->ddesc, b->bsub, b->base + b->count,
+                       b->plans[0]->wa);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_gather_direct, dim3((unsigned)((b->bsubcap + 255 / kDirectMax) / (256 / kDirectMax))),
-                       dim3(256), 0, s, b->ddesc, b->bsub, b->base + b->count, b->plans[0]->wa);
+    constexpr int per_block = 256 / kDirectLanes;   // list entries per block
+    hipLaunchKernelGGL(kb_gather_direct, dim3((unsigned)((b->bsubcap + per_block - 1) / per_block)), dim3(256), 0, s,
+                       b->ddesc, b->bsub, b->base + b->count, b->plans[0]->wa);
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }
@@ -1808,9 +1846,15 @@ extern "C" int nfft2d_batch_setpts_type2(nfft2d_batch *b, void *stream)
     return batch_gather(b, s);
 }
 
-// Subproblem size per size class (N1, N2), chosen by timing the whole batch
-// (setpts + type-2) with every candidate for one class at a time, the other
-// classes held fixed.  Synchronizes the stream; leaves the best sizes set.
+// The batch's tuning, chosen by timing the whole batch (setpts + type-2),
+// one parameter at a time (coordinate descent), the others held:
+//   1. per size class (N1, N2): the subproblem size, 64, 128 or 256 points;
+//   2. per size class: the direct-gather cut-off, 0, 4, 8 or 16 points
+//      (subproblems up to it skip the LDS tile);
+//   3. for the batch: the width of the persistent wave gather, 512, 1024 or
+//      2048 blocks.
+// None of them changes a result bit.  Synchronizes the stream; leaves the best
+// values set.
 extern "C" int nfft2d_batch_autotune(nfft2d_batch *b, void *stream, int reps)
 {
     if (!b) return nfft2d_fail(NFFT2D_ERR_ARG, "batch_autotune: null batch");
@@ -1818,37 +1862,73 @@ extern "C" int nfft2d_batch_autotune(nfft2d_batch *b, void *stream, int reps)
         return nfft2d_fail(NFFT2D_ERR_STATE, "batch_autotune: set the points and the input/output arrays first");
     if (reps < 1) reps = 3;
     hipStream_t s = (hipStream_t)stream;
-    static const int cand[] = {64, 128, 256};
+    static const int cand_mp[] = {64, 128, 256};
+    static const int cand_dm[] = {0, 4, 8, 16};
+    static const int cand_gw[] = {512, 1024, 2048};
     std::map<std::pair<int, int>, std::vector<int>> classes;
     for (int t = 0; t < b->count; ++t) classes[{b->hdesc[t].N1, b->hdesc[t].N2}].push_back(t);
     hipEvent_t e0, e1;
     NFFT2D_HIP_TRY(hipEventCreate(&e0));
     NFFT2D_HIP_TRY(hipEventCreate(&e1));
     int rc = NFFT2D_OK;
+    // the faster of two timings of reps passes each, after a warm-up pass
     auto time_once = [&](float &ms) -> int {
-        int r = nfft2d_batch_setpts_type2(b, s);    // warm-up
+        int r = nfft2d_batch_setpts_type2(b, s);
         if (r) return r;
-        if (hipEventRecord(e0, s) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event record");
-        for (int k = 0; k < reps && !r; ++k) r = nfft2d_batch_setpts_type2(b, s);
-        if (r) return r;
-        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-            return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event timing failed");
+        ms = 1e30f;
+        for (int trial = 0; trial < 2; ++trial) {
+            float t = 0.f;
+            if (hipEventRecord(e0, s) != hipSuccess) return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event record");
+            for (int k = 0; k < reps && !r; ++k) r = nfft2d_batch_setpts_type2(b, s);
+            if (r) return r;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                return nfft2d_fail(NFFT2D_ERR_HIP, "batch_autotune: event timing failed");
+            ms = std::min(ms, t);
+        }
         return NFFT2D_OK;
+    };
+    auto set_mp = [&](const std::vector<int> &ts, int mp) {
+        for (int t : ts) { b->plans[t]->maxpts = mp; b->hdesc[t].maxpts = mp; }
+        b->sorted = false;                        // the next setpts rebuilds the subproblem list
+        return batch_upload(b);
+    };
+    auto set_dm = [&](const std::vector<int> &ts, int dm) {
+        for (int t : ts) { b->dmax[t] = dm; b->hdesc[t].dmax = dm; }
+        b->sorted = false;
+        return batch_upload(b);
     };
     for (auto &kv : classes) {
         float best = 1e30f;
         int best_mp = b->hdesc[kv.second[0]].maxpts;
-        for (int mp : cand) {
-            for (int t : kv.second) { b->plans[t]->maxpts = mp; b->hdesc[t].maxpts = mp; }
-            if ((rc = batch_upload(b))) break;
+        for (int mp : cand_mp) {
             float ms = 0.f;
-            if ((rc = time_once(ms))) break;
+            if ((rc = set_mp(kv.second, mp)) || (rc = time_once(ms))) break;
             if (ms < best) { best = ms; best_mp = mp; }
         }
+        if (rc || (rc = set_mp(kv.second, best_mp))) break;
+    }
+    for (auto &kv : classes) {
         if (rc) break;
-        for (int t : kv.second) { b->plans[t]->maxpts = best_mp; b->hdesc[t].maxpts = best_mp; }
-        if ((rc = batch_upload(b))) break;
+        float best = 1e30f;
+        int best_dm = b->dmax[kv.second[0]];
+        for (int dm : cand_dm) {
+            float ms = 0.f;
+            if ((rc = set_dm(kv.second, dm)) || (rc = time_once(ms))) break;
+            if (ms < best) { best = ms; best_dm = dm; }
+        }
+        if (rc || (rc = set_dm(kv.second, best_dm))) break;
+    }
+    if (!rc) {
+        float best = 1e30f;
+        int best_gw = b->gather_blocks;
+        for (int gw : cand_gw) {
+            float ms = 0.f;
+            b->gather_blocks = gw;
+            if ((rc = time_once(ms))) break;
+            if (ms < best) { best = ms; best_gw = gw; }
+        }
+        b->gather_blocks = best_gw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -39,6 +39,9 @@ struct BatchDesc {
     float2* F1;
     float2* scratch;
     int r1, r2;
+    // type-2 gather: subproblems of at most dmax points are read point by point
+    // from the grid (kb_gather_direct), larger ones through an LDS tile
+    int dmax;
 };
 
 // Largest N of the batched FFT passes.
@@ -144,7 +147,8 @@ __global__ __launch_bounds__(1024) void kb_sub_prefix(const BatchDesc* __restric
 }
 
 // Scatter of every transform's points, and each transform's subproblems into
-// the batch list: entry base[t] + s = (bin, start, count, t).
+// the batch list: entry base[t] + s = (bin, start, count, t), with ~t (< 0) in
+// place of t when count <= the transform's direct-gather cut-off dmax.
 __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ bd, const int2* __restrict__ own,
                                                   const int* __restrict__ base, int4* __restrict__ bsub)
 {
@@ -157,8 +161,11 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ 
         const int pos = gb.x + e.x, sp = base[o.x] + gb.y + e.y;
         d.subbeg[i] = gb.y + e.y;                            // the bin's first subproblem in its transform
         const int ns = (c + d.maxpts - 1) / d.maxpts;
-        for (int k = 0; k < ns; ++k)
-            bsub[sp + k] = make_int4(i, pos + k * d.maxpts, min(d.maxpts, c - k * d.maxpts), o.x);
+        for (int k = 0; k < ns; ++k) {
+            const int n = min(d.maxpts, c - k * d.maxpts);
+            // the transform index, bit-complemented for the subproblems the type-2 gathers lane by lane
+            bsub[sp + k] = make_int4(i, pos + k * d.maxpts, n, n <= d.dmax ? ~o.x : o.x);
+        }
     }
     if (i >= d.M) return;
     int m1, m2;
@@ -222,8 +229,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int s = blockIdx.x;
     if (s >= base[count]) return;
     const int4 e = bsub[s];
-    const BatchDesc& t = bd[e.w];
-    spread_halo_body<false>(s - base[e.w], make_int4(e.x, e.y, e.z, 0), t.pts, t.d1, t.subbeg, t.tot, t.maxpts, t.geo,
+    const int tr = e.w < 0 ? ~e.w : e.w;
+    const BatchDesc& t = bd[tr];
+    spread_halo_body<false>(s - base[tr], make_int4(e.x, e.y, e.z, 0), t.pts, t.d1, t.subbeg, t.tot, t.maxpts, t.geo,
                             wa, t.scratch);
 }
 
@@ -273,15 +281,20 @@ __global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ 
 #ifndef NFFT2D_BATCH_GATHER_WAVES
 #define NFFT2D_BATCH_GATHER_WAVES 4
 #endif
-// Subproblems of at most kDirectMax points are gathered by kb_gather_direct,
-// one lane per point, the others by kb_gather_w (the batch of 256 uneven
-// transforms: 1.89 ms per setpts + type-2 with 8, 1.91 with 4, 1.97 with 16,
-// 2.03 with every subproblem on the waves).
-constexpr int kDirectMax = 8;
+// Subproblems of at most BatchDesc::dmax points (per transform, at most
+// kDirectCap) are gathered by kb_gather_direct, kDirectLanes lanes per list
+// entry, the others by kb_gather_w.  A fixed cut-off of 8 gave, on the batch
+// of 256 uneven transforms, 1.89 ms per setpts + type-2 against 1.91 with 4,
+// 1.97 with 16 and 2.03 with every subproblem on the waves; the batch's
+// autotuner picks the cut-off per size class.
+constexpr int kDirectLanes = 8;
+constexpr int kDirectCap = 16;
+constexpr int kDirectDefault = 8;
 
 // Persistent gather over the batch's subproblem list (total = base[count]),
 // one wave per subproblem: wave g = 4 blockIdx.x + w takes subproblems g,
-// g + 4 gridDim.x, ... (those of more than kDirectMax points).
+// g + 4 gridDim.x, ... (those of more than their transform's dmax points,
+// marked by a non-negative transform index in the list).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATCH_GATHER_WAVES, 8))) void
 kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub, const int* __restrict__ total, WinArgs wa)
 {
@@ -290,26 +303,29 @@ kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub, con
     const int n = *total;
     for (int s = blockIdx.x * 4 + w; s < n; s += gridDim.x * 4) {
         const int4 d = bsub[s];
-        if (d.z <= kDirectMax) continue;
+        if (d.w < 0) continue;                           // kb_gather_direct's
         const BatchDesc& t = bd[d.w];
         gather_wave_pairs(d, lds[w], t.Gt, t.geo, t.pts, wa, t.out);
     }
 }
 
-// The subproblems of at most kDirectMax points: kDirectMax threads per list
-// entry, thread q computing point q of the entry with direct_point.
+// The subproblems of at most dmax points: kDirectLanes threads per list entry,
+// thread q computing points q and q + kDirectLanes of the entry with
+// direct_point (the same bits as the wave's sums).
 __global__ __launch_bounds__(256) void kb_gather_direct(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub,
                                                         const int* __restrict__ total, WinArgs wa)
 {
-    const int s = blockIdx.x * (256 / kDirectMax) + (int)threadIdx.x / kDirectMax;
-    const int q = (int)threadIdx.x % kDirectMax;
+    const int s = blockIdx.x * (256 / kDirectLanes) + (int)threadIdx.x / kDirectLanes;
+    const int q = (int)threadIdx.x % kDirectLanes;
     if (s >= *total) return;
     const int4 d = bsub[s];
-    if (d.z > kDirectMax || q >= d.z) return;
-    const BatchDesc& t = bd[d.w];
-    const PtRec<float> rec = t.pts[d.y + q];
+    if (d.w >= 0 || q >= d.z) return;                    // kb_gather_w's, or past the entry's points
+    const BatchDesc& t = bd[~d.w];
     const int O1 = (d.x % t.geo.nb1) * kBin, O2 = (d.x / t.geo.nb1) * kBin;
-    t.out[rec.idx] = direct_point(rec, O1, O2, t.Gt, t.geo, wa);
+    for (int k = q; k < d.z; k += kDirectLanes) {
+        const PtRec<float> rec = t.pts[d.y + k];
+        t.out[rec.idx] = direct_point(rec, O1, O2, t.Gt, t.geo, wa);
+    }
 }
 
 }  // namespace nfft2d

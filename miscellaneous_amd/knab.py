@@ -19,8 +19,13 @@ then output points outside the input grid get ``ExtrapVal``.  ``rectNFFT2D``
 (``BP_FBP_FFBP/rectNFFT2D.m:1-77``) is the same with a rectangular window.
 
 Here the type-2 runs on the library (``Plan.setpts_type2``: tile sort, grid
-passes and gather; complex64, or complex128 for complex128 samples).  The input spectrum is a plain 2-D FFT of the
-caller's samples (``torch.fft.fft2``, as the reference calls MATLAB's
+passes and gather; complex64, or complex128 for complex128 samples), and the
+steps around it are the library's ``nfft2d_knab_*`` kernels: one reduction for
+the grid steps, origins and extents (one transfer to the host, which sizes the
+transform), one pass that windows, pads and conjugates the spectrum, one that
+scales and wraps the output points, one that conjugates, scales, crops and
+marks points outside the input grid.  The input spectrum is a plain 2-D FFT of
+the caller's samples (``torch.fft.fft2``, as the reference calls MATLAB's
 ``fft2``), and the window tables are computed on the host in FP64 once per
 (T, B, P, N): the reference integrates ``ConvWindow`` with MATLAB's adaptive
 ``quad``; here the substitution f = B/2 sin(theta) removes the square-root
@@ -28,17 +33,20 @@ edge of the integrand and a 64-point Gauss-Legendre rule integrates the
 remaining smooth function to FP64 rounding.
 
 Sizes: a N and b M must be even (the library's even-size plans; the reference
-also has an odd branch), and at most 4096 (2048 when not a power of two).
+also has an odd branch), and at most 4096 (complex64) or 2048 (complex128).
 """
 from __future__ import annotations
 
+import ctypes
 import functools
 import math
 
 import numpy as np
 import torch
 
-from .nfft import Plan
+from . import _lib
+from ._lib import NFFT2D_C64, NFFT2D_C128, NFFT2D_KNAB_STAT_BLOCKS, NFFT2D_KNAB_STATS, check
+from .nfft import Plan, _ptr, _stream
 
 __all__ = ["knabNFFT2D", "rectNFFT2D", "knab_window"]
 
@@ -87,15 +95,14 @@ _plans: dict = {}
 _windows: dict = {}
 
 
-def _window_tensor(wn: tuple, wm: tuple, scale: float, dtype, device: torch.device) -> torch.Tensor:
-    """scale * Gn (x) Gm as a complex device tensor, kept per (knab_window arguments, scale, dtype, device)."""
-    key = (wn, wm, scale, dtype, device)
+def _window_tables(w: tuple, device: torch.device) -> torch.Tensor:
+    """The FP64 window G of knab_window(*w) as a device tensor, kept per (arguments, device)."""
+    key = (w, device)
     g = _windows.get(key)
     if g is None:
-        if len(_windows) >= 16:
+        if len(_windows) >= 32:
             _windows.clear()
-        Gn, Gm = knab_window(*wn)[0], knab_window(*wm)[0]
-        g = _windows[key] = torch.from_numpy(np.outer(Gn, Gm) * scale).to(device).to(dtype)
+        g = _windows[key] = torch.from_numpy(knab_window(*w)[0]).to(device)
     return g
 
 
@@ -154,37 +161,47 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
     if min(RHO_OUT.shape) < 2:   # the output spacing (knabNFFT2D.m:36-37) divides by len - 1
         raise ValueError("RHO_OUT must be at least 2-by-2.")
     f64 = torch.float64
-    rho_axis, cos_axis = RHO_IN[0, :].to(f64), COSALPHA_IN[:, 0].to(f64)
-    ro, co = RHO_OUT.to(f64), COSALPHA_OUT.to(f64)
-    # the grid steps, origins and output extents in one transfer to the host
-    hs = torch.stack([torch.diff(rho_axis).mean(), torch.diff(cos_axis).mean(), rho_axis[0], cos_axis[0],
-                      ro.max() - ro.min(), co.max() - co.min()]).cpu().tolist()
-    T1, T2, r0, c0, rext, cext = hs
+    lib = _lib.load_library()
+    st = _stream(dev)
+    cdt = torch.complex128 if INPUT_DATA.dtype == torch.complex128 else torch.complex64
+    rdt = torch.float64 if cdt == torch.complex128 else torch.float32
+    ct = NFFT2D_C128 if cdt == torch.complex128 else NFFT2D_C64
+    rax, cax = RHO_IN[0, :], COSALPHA_IN[:, 0]
+    rax, cax = (rax if rax.dtype == f64 else rax.to(f64)), (cax if cax.dtype == f64 else cax.to(f64))
+    ro, co = RHO_OUT.to(f64), COSALPHA_OUT.to(f64)   # no copy for FP64 input: strided reads, meshgrid views too
     len1, len2 = ro.shape
+    strides = (ctypes.c_longlong * 4)(ro.stride(0), ro.stride(1), co.stride(0), co.stride(1))
+    stats = torch.empty(NFFT2D_KNAB_STATS + 4 * NFFT2D_KNAB_STAT_BLOCKS, dtype=f64, device=dev)
+    with torch.cuda.device(dev):
+        # the grid steps, origins, output extents and input ranges in one transfer to the host
+        check(lib.nfft2d_knab_stats(_ptr(rax), rax.numel(), rax.stride(0), _ptr(cax), cax.numel(), cax.stride(0),
+                                    _ptr(ro), _ptr(co), len1, len2, strides, _ptr(stats), st), "nfft2d_knab_stats")
+        hs = stats.cpu().numpy()
+    T1, T2, r0, c0, _, _, rmin, rmax, cmin, cmax = hs[:NFFT2D_KNAB_STATS].tolist()
+    part = hs[NFFT2D_KNAB_STATS:].reshape(-1, 4)       # per block: min, max of ro; min, max of co
+    nan_r, nan_c = np.isnan(part[:, :2]).any(), np.isnan(part[:, 2:]).any()
+    rext = math.nan if nan_r else float(part[:, 1].max() - part[:, 0].min())
+    cext = math.nan if nan_c else float(part[:, 3].max() - part[:, 2].min())
     Nn = rext / (T1 * (1 - 1 / len2))
     Mm = cext / (T2 * (1 - 1 / len1))
-    rho_s, cos_s = ro - r0, co - c0
     if Nn < 1 or math.isnan(Nn):
         Nn = 1.0
     if Mm < 1 or math.isnan(Mm):
         Mm = 1.0
     a, b = _round(len2 / Nn), _round(len1 / Mm)
-    if a < 2:                          # extra output points at the origin, cropped at the end
+    R2, C2 = len1, len2                # extra output points at the origin when a or b < 2, cropped at the end
+    if a < 2:
         a = 2
-        ii = _round(a * Nn) - rho_s.shape[1]
-        rho_s = torch.nn.functional.pad(rho_s, (0, ii))
-        cos_s = torch.nn.functional.pad(cos_s, (0, ii))
+        C2 = _round(a * Nn)
     if b < 2:
         b = 2
-        ii = _round(b * Mm) - rho_s.shape[0]
-        rho_s = torch.nn.functional.pad(rho_s, (0, 0, 0, ii))
-        cos_s = torch.nn.functional.pad(cos_s, (0, 0, 0, ii))
-    cdt = torch.complex128 if INPUT_DATA.dtype == torch.complex128 else torch.complex64
-    rdt = torch.float64 if cdt == torch.complex128 else torch.float32
-    data = INPUT_DATA.to(cdt).t()                    # N (rho) x M (cos alpha)
-    N, M = data.shape
+        R2 = _round(b * Mm)
+    data = INPUT_DATA.to(cdt).contiguous()           # M (cos alpha) x N (rho)
+    M, N = data.shape
     if (a * N) % 2 or (b * M) % 2:
         raise ValueError(f"{window}NFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
+    if C2 < len2 or R2 < len1:
+        raise ValueError(f"{window}NFFT2D: padded output grid {R2} x {C2} smaller than the output")
     if window == "knab":
         wn, wm = (T1, float(B1), int(P), N), (T2, float(B2), int(P), M)
         kgn, kgm = knab_window(*wn)[1], knab_window(*wm)[1]
@@ -193,27 +210,29 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
         kgm = int(math.floor((1 / T2 - float(B2) / 2) / (1 / (Mm * T2))))
     if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M or kgn < 0 or kgm < 0:
         raise ValueError(f"{window}NFFT2D: the window is wider than the padded spectrum")
-    # the spectrum, repeated periodically to 2 kg + 1 frequencies and windowed (Selva eq. 25)
-    sF = torch.fft.fft2(data)
-    i_n = torch.remainder(torch.arange(-kgn, kgn + 1, device=dev), N)
-    i_m = torch.remainder(torch.arange(-kgm, kgm + 1, device=dev), M)
-    sF = sF.index_select(0, i_n).index_select(1, i_m)
+    # the spectrum of the samples (cos alpha x rho); the kernel below reads it transposed
+    S = torch.fft.fft2(data).contiguous()
+    F = torch.empty(a * N, b * M, dtype=cdt, device=dev)
+    x1 = torch.empty(R2 * C2, dtype=rdt, device=dev)
+    x2 = torch.empty(R2 * C2, dtype=rdt, device=dev)
+    res = torch.empty(len1, len2, dtype=cdt, device=dev)
     if window == "knab":
-        sF = sF * _window_tensor(wn, wm, float(N * a) * float(M * b), cdt, dev)
+        Gn, Gm = _window_tables(wn, dev), _window_tables(wm, dev)
+        scale = float(N * a) * float(M * b)
     else:   # G = 1 and the samples divided by N M (rectNFFT2D.m:70)
-        sF = sF * (float(a) * float(b))
-    # zero padding to a N x b M, frequency 0 at (a N / 2, b M / 2): the library's centred layout
-    F = torch.zeros(a * N, b * M, dtype=cdt, device=dev)
-    F[a * N // 2 - kgn:a * N // 2 + kgn + 1, b * M // 2 - kgm:b * M // 2 + kgm + 1] = sF
-    # points in units of the padded grid, wrapped to one period in FP64 before any FP32 cast
-    x1 = rho_s.reshape(-1) / (T1 / a)
-    x2 = cos_s.reshape(-1) / (T2 / b)
-    x1 = (x1 - a * N * torch.round(x1 / (a * N))).to(rdt)
-    x2 = (x2 - b * M * torch.round(x2 / (b * M))).to(rdt)
-    y = _plan(a * N, b * M, cdt, dev).setpts_type2(x1, x2, torch.conj_physical(F))
-    res = (y.conj() / float(a * b * N * M)).reshape(rho_s.shape)[:len1, :len2].clone()
-    xout = (ro[0, :] < rho_axis.min()) | (ro[0, :] > rho_axis.max())
-    yout = (co[:, 0] < cos_axis.min()) | (co[:, 0] > cos_axis.max())
-    res[:, xout] = ExtrapVal
-    res[yout, :] = ExtrapVal
+        Gn = Gm = None
+        scale = float(a) * float(b)
+    ex = complex(ExtrapVal)
+    with torch.cuda.device(dev):
+        # repeated periodically to 2 kg + 1 frequencies, windowed (Selva eq. 25), zero padded to a N x b M
+        # with frequency 0 at (a N / 2, b M / 2) (the library's centred layout) and conjugated
+        check(lib.nfft2d_knab_spectrum(ct, _ptr(S), M, N, kgn, kgm, _ptr(Gn), _ptr(Gm), scale, _ptr(F), a * N, b * M,
+                                       st), "nfft2d_knab_spectrum")
+        # points in units of the padded grid, wrapped to one period in FP64 before the cast
+        check(lib.nfft2d_knab_points(ct, _ptr(ro), _ptr(co), len1, len2, strides, R2, C2, r0, c0, T1 / a, T2 / b,
+                                     float(a * N), float(b * M), _ptr(x1), _ptr(x2), st), "nfft2d_knab_points")
+    y = _plan(a * N, b * M, cdt, dev).setpts_type2(x1, x2, F)
+    with torch.cuda.device(dev):
+        check(lib.nfft2d_knab_finish(ct, _ptr(y), len1, len2, C2, float(a * b * N * M), _ptr(ro), _ptr(co), strides,
+                                     rmin, rmax, cmin, cmax, ex.real, ex.imag, _ptr(res), st), "nfft2d_knab_finish")
     return res

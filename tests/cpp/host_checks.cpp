@@ -112,6 +112,25 @@ static void check_args()
                "NUFFT_2D_GPU_host(len_in < 0)");
     expect_err(NUFFT_2D_GPU_params(nullptr, nullptr, nullptr, nullptr, 1e-6, 10, 10, nullptr),
                "NUFFT_2D_GPU_params(null)");
+    expect_err(nfft2d_batch_set_direct(nullptr, 0, 8), "batch_set_direct(null)");
+    expect_err(nfft2d_batch_set_gather_blocks(nullptr, 1024), "batch_set_gather_blocks(null)");
+    const double *dp = reinterpret_cast<const double *>(8);
+    expect_err(nfft2d_knab_stats(nullptr, 8, 1, dp, 8, 1, dp, dp, 4, 4, nullptr, (double *)8, nullptr),
+               "knab_stats(null axis)");
+    expect_err(nfft2d_knab_stats(dp, 1, 1, dp, 8, 1, dp, dp, 4, 4, nullptr, (double *)8, nullptr),
+               "knab_stats(one-sample axis)");
+    expect_err(nfft2d_knab_spectrum(NFFT2D_C64, dp, 8, 8, 5, 2, nullptr, nullptr, 1.0, (void *)8, 8, 8, nullptr),
+               "knab_spectrum(band wider than the padding)");
+    expect_err(nfft2d_knab_spectrum(NFFT2D_C64, dp, 8, 8, 1, 1, dp, nullptr, 1.0, (void *)8, 8, 8, nullptr),
+               "knab_spectrum(one window table)");
+    expect_err(nfft2d_knab_points(7, dp, dp, 4, 4, nullptr, 4, 4, 0.0, 0.0, 1.0, 1.0, 8.0, 8.0, (void *)8, (void *)8,
+                                  nullptr),
+               "knab_points(bad dtype)");
+    expect_err(nfft2d_knab_points(NFFT2D_C64, dp, dp, 4, 4, nullptr, 3, 4, 0.0, 0.0, 1.0, 1.0, 8.0, 8.0, (void *)8,
+                                  (void *)8, nullptr),
+               "knab_points(padded grid smaller)");
+    expect_err(nfft2d_knab_finish(NFFT2D_C128, dp, 4, 4, 3, 1.0, dp, dp, nullptr, 0, 1, 0, 1, 0, 0, nullptr, nullptr),
+               "knab_finish(null result)");
 }
 
 // ------------------------------------------------------------------ full mode (GPU)

@@ -162,10 +162,15 @@ def test_batch_autotune_keeps_results():
     before = [o.clone() for o in to]
     choice = b.autotune(reps=2)
     assert all(c in (64, 128, 256) for c in choice)
+    tune = b.tuning()
+    print(f"case autotune: {tune}", flush=True)
+    assert tune["maxpts"] == choice
+    assert all(d in (0, 4, 8, 16) for d in tune["direct"])
+    assert tune["gather_blocks"] in (512, 1024, 2048)
     # one choice per size class
     by_class = {}
-    for (N1, N2, _), c in zip(SIZES, choice):
-        by_class.setdefault((N1, N2), set()).add(c)
+    for (N1, N2, _), c, d in zip(SIZES, choice, tune["direct"]):
+        by_class.setdefault((N1, N2), set()).add((c, d))
     assert all(len(v) == 1 for v in by_class.values())
     b.setpts()
     b.type2()
@@ -217,6 +222,45 @@ def test_batch_small_subproblems_lane_per_point():
         sel = np.sort(rng.choice(M, min(M, 500), replace=False))
         ref = oracle.ndft_type2(fs[t].astype(np.complex128), xs[t][sel], ys[t][sel])
         assert oracle.rel_l2(yb[sel], ref) <= 1e-5, t
+
+
+@pytest.mark.parametrize("direct,blocks", [(0, 1024), (16, 512), (4, 2048), (16, 1)])
+def test_batch_direct_cutoff_and_gather_width(direct, blocks):
+    """The direct-gather cut-off (subproblems of up to 16 points read lane by lane, two points per
+    lane above 8) and the width of the persistent gather change no result bit; bins of 0..20
+    points cross every cut-off."""
+    rng = np.random.default_rng(23)
+    sizes, fs, xs, ys = [], [], [], []
+    for N, shift in ((64, 0), (32, 7)):
+        nbins = ((2 * N) // 16) ** 2
+        counts = [(b + shift) % 21 for b in range(nbins)]
+        x1, x2 = points_per_bin(rng, N, counts)
+        sizes.append((N, N, len(x1)))
+        fs.append(cn01(rng, (N, N)))
+        xs.append(x1)
+        ys.append(x2)
+    print(f"case direct {direct} blocks {blocks}", flush=True)
+    b = m.Batch(sizes, device=DEV)
+    to = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in sizes]
+    b.set_points([T(x) for x in xs], [T(y) for y in ys]).set_io([T(f) for f in fs], to)
+    for t in range(len(sizes)):
+        b.set_direct(t, direct)
+    b.set_gather_blocks(blocks)
+    assert b.tuning()["direct"] == [direct] * len(sizes) and b.tuning()["gather_blocks"] == blocks
+    b.setpts()
+    b.type2()
+    torch.cuda.synchronize()
+    for t in range(len(sizes)):
+        assert np.array_equal(to[t].cpu().numpy(), per_plan(fs[t], xs[t], ys[t])), f"transform {t} {sizes[t]}"
+
+
+def test_batch_tuning_arguments_refused():
+    b = m.Batch([(16, 16, 10)], device=DEV)
+    for bad in (-1, 17):
+        with pytest.raises(m.NFFT2DError, match="cut-off"):
+            b.set_direct(0, bad)
+    with pytest.raises(m.NFFT2DError, match="outside"):
+        b.set_gather_blocks(0)
 
 
 def test_type2_refused_before_setpts():

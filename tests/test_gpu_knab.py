@@ -107,3 +107,28 @@ def test_fp64_samples_take_the_fp64_path(fn):
     e = rel(out.cpu().numpy(), ref)
     print(f"{fn} fp64 rel-L2 {e:.3e}")
     assert e < 1e-9
+
+
+@pytest.mark.parametrize("fn", ["knab", "rect"])
+def test_meshgrid_views_and_fp32_coordinates(fn):
+    """Output coordinates as torch.meshgrid views (stride 0 in one dimension) and in FP32: the
+    kernels read them in place; the result is bitwise that of contiguous FP64 copies of the same
+    values."""
+    n1, n2 = 80, 96
+    T1, T2 = 0.05, 0.002
+    f64 = torch.float64
+    rho = 100 + T1 * torch.arange(n2, dtype=f64, device=DEV)
+    cs = -0.1 + T2 * torch.arange(n1, dtype=f64, device=DEV)
+    RHO, COS = torch.meshgrid(rho, cs, indexing="xy")          # views, n1 x n2
+    assert RHO.stride(0) == 0 or COS.stride(1) == 0
+    rng = np.random.default_rng(33)
+    data = T((rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2))) / np.sqrt(2), torch.complex64)
+    ro = torch.linspace(float(rho[0]) - 2 * T1, float(rho[-1]), 150, device=DEV).to(torch.float32)
+    co = torch.linspace(float(cs[3]), float(cs[-1]) + T2, 170, device=DEV).to(torch.float32)
+    RO, CO = torch.meshgrid(ro, co, indexing="xy")
+    f = getattr(m, f"{fn}NFFT2D")
+    a = f(0.25, RHO, COS, data, RO, CO, 0.5 / T1, 0.5 / T2)
+    b = f(0.25, RHO.contiguous(), COS.contiguous(), data, RO.to(f64).contiguous(), CO.to(f64).contiguous(),
+          0.5 / T1, 0.5 / T2)
+    assert torch.equal(a, b)
+    assert (a == 0.25).any() and (a != 0.25).any()
