@@ -851,6 +851,17 @@ def profile_workload(args):
         x, y, s, t, f = type3_workload(dev)
         for _ in range(args.steps):
             m.NUFFT_2D(x, y, s, t, f, 1e-6)
+    elif args.profile == "ndft_dropin":
+        nu, M = 256, 65536
+        rng = np.random.default_rng(SEED + 9)
+        X = torch.from_numpy(rng.uniform(-nu / 2, nu / 2, M)).to(dev)
+        Y = torch.from_numpy(rng.uniform(-nu / 2, nu / 2, M)).to(dev)
+        d = torch.from_numpy((rng.standard_normal(M) + 1j * rng.standard_normal(M)) / np.sqrt(2)).to(dev)
+        n = torch.arange(nu * nu, device=dev)
+        u, v = (n // nu - nu // 2).double(), (n % nu - nu // 2).double()
+        out = torch.empty(nu * nu, dtype=torch.complex128, device=dev)
+        for _ in range(args.steps):
+            m.NDFT2_2D_GPU(0, X, Y, u, v, d, out, nu, nu, M, nu * nu)
     elif args.profile == "ndft_bf16x3":
         N, M = 256, 65536
         f = cn01(gen, (N, N), dev)
@@ -905,7 +916,7 @@ def main():
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "copy", "type3"],
+    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "ndft_dropin", "copy", "type3"],
                     help="run only --steps eager calls of one workload (polar 512^2 type-2 / type-1, the "
                          "BF16x3 NDFT at 256^2, M = 65536, the FP64 type-3 at 1M -> 1M, or a 256 MiB device "
                          "copy) and exit: the command "

@@ -3,12 +3,17 @@
 //   out[n] = sum_{m<M} in[m] exp(-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)).
 // The reference fills an explicit M x N FP64 kernel matrix (:44-50; 64 GiB at
 // M = N = 65536), writes it to a text file (:56) and calls cublasZgemv (:67).
-// Here nothing is materialised: on the integer meshgrid the reference's
-// caller passes, the sum is an FP64 complex GEMM on the matrix cores
-// (k_ndft_grid); for other (u, v) each thread owns one output n and streams
-// the points through LDS in chunks, accumulating in FP64 (k_ndft_direct).
+// Here nothing of that size is materialised: on the integer meshgrid the
+// reference's caller passes, the sum is an FP64 complex GEMM on the matrix
+// cores (k_ndft_grid + k_ndft_grid_sum, with a cached workspace of at most 64
+// partial output arrays); for other (u, v) each thread owns one output n and
+// streams the points through LDS in chunks, accumulating in FP64
+// (k_ndft_direct).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <new>
 
 #include "../../include/nfft2d.h"
@@ -104,17 +109,25 @@ __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__res
     }
 }
 
-// One 16 x 16 tile of outputs per block; the 16 waves of the block split the
-// points and their partial tiles are summed in LDS in wave order
-// (deterministic).  v_mfma_f64_16x16x4f64 takes 4 points per instruction:
-// lane l supplies A[i][k] at i = l % 16, k = l / 16 and B[k][j] at
-// j = l % 16; accumulator element r of lane l is C[4 r + l / 16][l % 16].  The
-// complex product is four real ones: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br.
-// Operands are exact FP64 phasors: the phase reduced to [0, 1) turns by
-// sincospi.  Persistent over the tiles (their count is known on the device
-// only).
-constexpr int kGridWaves = 16;
-constexpr int kGridBlocks = 512;
+// The grid case on v_mfma_f64_16x16x4f64 (4 points per instruction): lane l
+// supplies A[i][k] at i = l % 16, k = l / 16 and B[k][j] at j = l % 16;
+// accumulator element r of lane l is C[4 r + l / 16][l % 16].  The complex
+// product is four real ones: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br.
+//
+// A work item is one wave: a supertile of kTR x kTC MFMA tiles (32 x 64
+// outputs) over one range of the points.  Per K-step of 4 points a lane
+// evaluates its point's phasor at its first row and first column by an FP64
+// phase reduction and sincospi, and the phasors 16 rows / columns further on
+// by one complex product with the point's 16-step phasor each (rows and
+// columns are arithmetic progressions; the steps come from k_ndft_steps, once
+// per call), so that 8 MFMA tiles share 2 sincospi per lane instead of 16.  The partial supertile goes to range r's slice of a
+// workspace; k_ndft_grid_sum adds the ranges in a fixed order (the result does
+// not depend on the launch).  Work items are walked persistently (their count
+// is known on the device only).
+constexpr int kTR = 2, kTC = 4;                   // MFMA tiles per supertile: rows, columns
+constexpr int kSupRows = 16 * kTR, kSupCols = 16 * kTC;
+constexpr int kGridWavesTotal = 2048;             // waves of the persistent launch (2 per SIMD at 190 VGPRs)
+constexpr int kMaxRanges = 64;
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double2 phasor(double t)   // e^{-2 pi i t}
@@ -125,60 +138,161 @@ __device__ __forceinline__ double2 phasor(double t)   // e^{-2 pi i t}
     return make_double2(c, -s);
 }
 
-__global__ __launch_bounds__(64 * kGridWaves) void k_ndft_grid(const double *__restrict__ X,
-                                                             const double *__restrict__ Y,
-                                                             const double2 *__restrict__ in,
-                                                             double2 *__restrict__ out, double inv_Nu,
-                                                             double inv_Nv, int M)
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
 {
-    __shared__ double2 part[kGridWaves][16][16];
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Supertiles per grid (device side, from the detector's nu, nv).
+__device__ __forceinline__ int n_super(const GridInfo &g, int &stv)
+{
+    stv = (g.nv + kSupCols - 1) / kSupCols;
+    return ((g.nu + kSupRows - 1) / kSupRows) * stv;
+}
+
+// steps[m] = (e^{-2 pi i 16 du X_m / Nu}, e^{-2 pi i 16 dv Y_m / Nv}): the 16-row and 16-column
+// steps of point m's phasors, once per call instead of once per supertile.
+__global__ __launch_bounds__(kThreads) void k_ndft_steps(const double *__restrict__ X, const double *__restrict__ Y,
+                                                         double inv_Nu, double inv_Nv, int M,
+                                                         double4 *__restrict__ steps)
+{
     const GridInfo g = g_grid;
     if (!g.ok) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int q = lane & 15, k = lane >> 4;
-    const int tu = (g.nu + 15) / 16, tv = (g.nv + 15) / 16;
-    const int per = (((M + kGridWaves - 1) / kGridWaves) + 3) & ~3;   // points per wave, a multiple of 4
-    const int m_lo = min(M, w * per), m_hi = min(M, m_lo + per);
-    for (int tile = blockIdx.x; tile < tu * tv; tile += gridDim.x) {
-        const int i0 = (tile / tv) * 16, j0 = (tile % tv) * 16;
-        const double a = g.u0 + (i0 + q) * g.du, b = g.v0 + (j0 + q) * g.dv;   // this lane's row / column
-        const bool row_in = i0 + q < g.nu, col_in = j0 + q < g.nv;
-        d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-        for (int m0 = m_lo; m0 < m_hi; m0 += 4) {
-            const int m = m0 + k;
-            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    const int m = blockIdx.x * kThreads + threadIdx.x;
+    if (m >= M) return;
+    const double2 sa = phasor(16.0 * g.du * (X[m] * inv_Nu)), sb = phasor(16.0 * g.dv * (Y[m] * inv_Nv));
+    steps[m] = make_double4(sa.x, sa.y, sb.x, sb.y);
+}
+
+__global__ __launch_bounds__(256) void k_ndft_grid(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double2 *__restrict__ in, const double4 *__restrict__ steps,
+                                                   double2 *__restrict__ part, double inv_Nu, double inv_Nv, int M,
+                                                   int ranges, int chunk)
+{
+    const GridInfo g = g_grid;
+    if (!g.ok) return;
+    const int lane = threadIdx.x & 63, q = lane & 15, k = lane >> 4;
+    int stv;
+    const int items = n_super(g, stv) * ranges;
+    const long long N = (long long)g.nu * g.nv;
+    for (int item = blockIdx.x * 4 + (threadIdx.x >> 6); item < items; item += gridDim.x * 4) {
+        const int r = item % ranges, st = item / ranges;
+        const int i0 = (st / stv) * kSupRows, j0 = (st % stv) * kSupCols;
+        const int m_lo = min(M, r * chunk), m_hi = min(M, m_lo + chunk);
+        const double a0 = g.u0 + (i0 + q) * g.du, b0 = g.v0 + (j0 + q) * g.dv;   // this lane's first row / column
+        bool row_in[kTR], col_in[kTC];
+#pragma unroll
+        for (int t = 0; t < kTR; ++t) row_in[t] = i0 + 16 * t + q < g.nu;
+#pragma unroll
+        for (int t = 0; t < kTC; ++t) col_in[t] = j0 + 16 * t + q < g.nv;
+        d4 cr[kTR][kTC], ci[kTR][kTC];
+#pragma unroll
+        for (int a = 0; a < kTR; ++a)
+#pragma unroll
+            for (int b = 0; b < kTC; ++b) {
+                cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+                ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+        // the point data of the next K-step is loaded while the current one is
+        // multiplied (one load round trip per K-step would set the pace)
+        struct Pt {
+            double2 d;
+            double x, y;
+            double4 st;
+        };
+        auto load_pt = [&](int m) {
+            Pt p;
             if (m < m_hi) {
-                const double2 d = in[m];
-                const double2 pa = phasor(a * X[m] * inv_Nu);
-                const double2 pb = phasor(b * Y[m] * inv_Nv);
-                ar = row_in ? d.x * pa.x - d.y * pa.y : 0.0;
-                ai = row_in ? d.x * pa.y + d.y * pa.x : 0.0;
-                br = col_in ? pb.x : 0.0;
-                bi = col_in ? pb.y : 0.0;
+                p.d = in[m];
+                p.x = X[m];
+                p.y = Y[m];
+                p.st = steps[m];
+            } else {
+                p.d = make_double2(0.0, 0.0);
+                p.x = p.y = 0.0;
+                p.st = make_double4(1.0, 0.0, 1.0, 0.0);
             }
-            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
-            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, cr, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
-        }
+            return p;
+        };
+        Pt cur = load_pt(m_lo + k);
+        for (int m0 = m_lo; m0 < m_hi; m0 += 4) {
+            const Pt nxt = load_pt(m0 + 4 + k);
+            double ar[kTR], ai[kTR], br[kTC], bi[kTC];
+            {
+                // a point past the range has d = 0: its A operands vanish
+                const double xm = cur.x * inv_Nu, ym = cur.y * inv_Nv;
+                double2 pa = phasor(a0 * xm), pb = phasor(b0 * ym);
+                const double2 sa = make_double2(cur.st.x, cur.st.y), sb = make_double2(cur.st.z, cur.st.w);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) part[w][4 * r + k][q] = make_double2(cr[r], ci[r]);
-        __syncthreads();
-        if (threadIdx.x < 256) {
-            const int ii = threadIdx.x / 16, jj = threadIdx.x % 16;
-            double sr = 0.0, si = 0.0;
+                for (int t = 0; t < kTR; ++t) {
+                    const double2 v = zmul(cur.d, pa);
+                    ar[t] = row_in[t] ? v.x : 0.0;
+                    ai[t] = row_in[t] ? v.y : 0.0;
+                    if (t + 1 < kTR) pa = zmul(pa, sa);
+                }
 #pragma unroll
-            for (int ww = 0; ww < kGridWaves; ++ww) {
-                const double2 p = part[ww][ii][jj];
-                sr += p.x;
-                si += p.y;
+                for (int t = 0; t < kTC; ++t) {
+                    br[t] = col_in[t] ? pb.x : 0.0;
+                    bi[t] = col_in[t] ? pb.y : 0.0;
+                    if (t + 1 < kTC) pb = zmul(pb, sb);
+                }
             }
-            const int i = i0 + ii, j = j0 + jj;
-            if (i < g.nu && j < g.nv) out[g.vmajor ? (size_t)j * g.nu + i : (size_t)i * g.nv + j] = make_double2(sr, si);
+#pragma unroll
+            for (int a = 0; a < kTR; ++a)
+#pragma unroll
+                for (int b = 0; b < kTC; ++b) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+                }
+#pragma unroll
+            for (int a = 0; a < kTR; ++a)
+#pragma unroll
+                for (int b = 0; b < kTC; ++b) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], -bi[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                }
+            cur = nxt;
         }
-        __syncthreads();
+        double2 *dst = part + (size_t)r * N;
+#pragma unroll
+        for (int a = 0; a < kTR; ++a)
+#pragma unroll
+            for (int b = 0; b < kTC; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = i0 + 16 * a + 4 * e + k, j = j0 + 16 * b + q;
+                    if (i < g.nu && j < g.nv) dst[(size_t)i * g.nv + j] = make_double2(cr[a][b][e], ci[a][b][e]);
+                }
     }
 }
+
+// out[n] = sum over the ranges (in order) of the partial sums, in the caller's flattening.
+__global__ __launch_bounds__(kThreads) void k_ndft_grid_sum(const double2 *__restrict__ part, int ranges,
+                                                            double2 *__restrict__ out, int N)
+{
+    const GridInfo g = g_grid;
+    if (!g.ok) return;
+    const int n = blockIdx.x * kThreads + threadIdx.x;
+    if (n >= N) return;
+    const int i = g.vmajor ? n % g.nu : n / g.nv, j = g.vmajor ? n / g.nu : n % g.nv;
+    const size_t e = (size_t)i * g.nv + j;
+    double sr = 0.0, si = 0.0;
+    for (int r = 0; r < ranges; ++r) {
+        const double2 v = part[(size_t)r * N + e];
+        sr += v.x;
+        si += v.y;
+    }
+    out[n] = make_double2(sr, si);
+}
+
+// The partial sums' workspace, one per device, grown as needed (the
+// drop-in's calls run on the null stream, so consecutive calls are ordered).
+struct GridWs {
+    double2 *ptr = nullptr;
+    size_t count = 0;
+};
+std::mutex g_ws_mu;
+std::map<int, GridWs> g_ws;
 
 __global__ __launch_bounds__(kThreads) void k_ndft_direct(const double *__restrict__ X, const double *__restrict__ Y,
                                                           const double *__restrict__ u, const double *__restrict__ v,
@@ -228,10 +342,48 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
         return nfft2d_fail(NFFT2D_ERR_ARG, "NDFT2_2D_GPU: null pointer");
     // (u, v) a grid: FP64 MFMA contraction; otherwise the direct sum (each kernel
     // reads the detector's finding and returns at once when it is not its case)
+    // point ranges: enough work items for the whole chip at the supertile count a
+    // square grid of N outputs would have, each range at least 16 points
+    const long long nsup_est = std::max<long long>(1, (long long)N / (kSupRows * kSupCols));
+    int ranges = (int)std::min<long long>(kMaxRanges, std::max<long long>(1, kGridWavesTotal / nsup_est));
+    ranges = std::max(1, std::min(ranges, (M + 15) / 16));
+    const int chunk = ((M + ranges - 1) / ranges + 3) & ~3;   // a multiple of 4 points
+    const int dev = [] { int d = 0; (void)hipGetDevice(&d); return d; }();
+    double2 *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        GridWs &ws = g_ws[dev];
+        // partial sums, then (from an even double2 index: 32-byte aligned) the steps
+        const size_t need = (((size_t)ranges * (size_t)N + 1) & ~(size_t)1) + 2 * (size_t)M;
+        if (ws.count < need) {
+            if (ws.ptr) {
+                (void)hipDeviceSynchronize();
+                (void)hipFree(ws.ptr);
+                ws.ptr = nullptr;
+                ws.count = 0;
+            }
+            if (nfft2d_malloc(&ws.ptr, sizeof(double2) * need) != hipSuccess) {
+                (void)hipGetLastError();
+                ws.ptr = nullptr;
+                return nfft2d_fail(NFFT2D_ERR_ALLOC, "NDFT2_2D_GPU: workspace of %zu bytes", sizeof(double2) * need);
+            }
+            ws.count = need;
+        }
+        part = ws.ptr;
+    }
+    double4 *steps = reinterpret_cast<double4 *>(part + (((size_t)ranges * (size_t)N + 1) & ~(size_t)1));
     hipLaunchKernelGGL(k_grid_detect, dim3(1), dim3(kDetThreads), 0, nullptr, u, v, N);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ndft_grid, dim3(kGridBlocks), dim3(64 * kGridWaves), 0, nullptr, X, Y,
-                       reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), 1.0 / Nu, 1.0 / Nv, M);
+    if (M > 0) {
+        hipLaunchKernelGGL(k_ndft_steps, dim3((M + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, X, Y,
+                           1.0 / Nu, 1.0 / Nv, M, steps);
+        NFFT2D_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_ndft_grid, dim3(kGridWavesTotal / 4), dim3(256), 0, nullptr, X, Y,
+                       reinterpret_cast<const double2 *>(in), steps, part, 1.0 / Nu, 1.0 / Nv, M, ranges, chunk);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ndft_grid_sum, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, part, ranges,
+                       reinterpret_cast<double2 *>(out), N);
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_ndft_direct, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, X, Y, u, v,
                        reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), 1.0 / Nu, 1.0 / Nv, M,
