@@ -60,11 +60,17 @@ __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__res
     }
     __syncthreads();
     const double u0 = u[0], v0 = v[0];
-    // first change of u and of v
-    for (int n = 1 + t; n < N; n += kDetThreads) {
-        if (u[n] != u0) atomicMin(&s_run[0], n);
-        if (v[n] != v0) atomicMin(&s_run[1], n);
+    // first change of u and of v: each thread's first hit is its smallest
+    // (its indices increase), so it stops once it has both, and the block
+    // takes the minimum with one LDS atomic per thread and value (an atomic
+    // per element serialised the scan: 425 us at 65,536 outputs)
+    int fu = N, fv = N;
+    for (int n = 1 + t; n < N && (fu == N || fv == N); n += kDetThreads) {
+        if (fu == N && u[n] != u0) fu = n;
+        if (fv == N && v[n] != v0) fv = n;
     }
+    if (fu < N) atomicMin(&s_run[0], fu);
+    if (fv < N) atomicMin(&s_run[1], fv);
     __syncthreads();
     const int ru = s_run[0], rv = s_run[1];
     // u-major: v changes at once and u after nv = ru outputs; v-major: the converse
