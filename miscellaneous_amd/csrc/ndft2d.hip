@@ -33,9 +33,10 @@ constexpr int kThreads = 256;
 // is the complex product (nu x M) . (M x nv) of two phasor matrices: a GEMM on
 // the FP64 matrix cores instead of M N separate kernel evaluations.
 // k_grid_detect recognises the two flattenings of such a grid (u constant
-// over runs of nv outputs, or v constant over runs of nu) by exact FP64
-// comparison and leaves its finding in g_grid (one per device, as every
-// __device__ variable); k_ndft_grid runs when it found a grid, k_ndft_direct
+// over runs of nv outputs, or v constant over runs of nu) and k_grid_verify
+// checks every output against it by exact FP64 comparison; the finding stays
+// in g_grid (one per device, as every __device__ variable); k_ndft_grid runs
+// when it is a grid, k_ndft_direct
 // (any u, v) when it did not.  Both read the flag on the device, so the call
 // stays asynchronous and capturable.  Two NDFT2_2D_GPU calls that overlap on
 // one device (different streams) race on g_grid.
@@ -51,12 +52,11 @@ constexpr int kDetThreads = 1024;
 __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__restrict__ u,
                                                              const double *__restrict__ v, int N)
 {
-    __shared__ int s_run[2], s_bad;
+    __shared__ int s_run[2];
     const int t = threadIdx.x;
     if (t == 0) {
         s_run[0] = N;
         s_run[1] = N;
-        s_bad = 0;
     }
     __syncthreads();
     const double u0 = u[0], v0 = v[0];
@@ -94,16 +94,11 @@ __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__res
         du = nu > 1 ? u[1] - u0 : 0.0;
         dv = nv > 1 ? v[nu] - v0 : 0.0;
     }
-    if (vmajor >= 0) {
-        for (int n = t; n < N; n += kDetThreads) {
-            const int i = vmajor ? n % nu : n / nv, j = vmajor ? n / nu : n % nv;
-            if (u[n] != u0 + i * du || v[n] != v0 + j * dv) s_bad = 1;
-        }
-    }
-    __syncthreads();
+    // every output is then checked against the grid by k_grid_verify (all of
+    // the chip: one block's loop over N outputs took 76 us at N = 65,536)
     if (t == 0) {
         GridInfo g;
-        g.ok = vmajor >= 0 && !s_bad && nu * nv == N;
+        g.ok = vmajor >= 0 && nu * nv == N;
         g.vmajor = vmajor > 0;
         g.nu = nu;
         g.nv = nv;
@@ -113,6 +108,19 @@ __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__res
         g.dv = dv;
         g_grid = g;
     }
+}
+
+// Clears g_grid.ok when an output is not the grid's: u[n] = u0 + i du and
+// v[n] = v0 + j dv, exactly, for n's (i, j) in the detected flattening.
+__global__ __launch_bounds__(kThreads) void k_grid_verify(const double *__restrict__ u, const double *__restrict__ v,
+                                                          int N)
+{
+    const GridInfo g = g_grid;
+    if (!g.ok) return;
+    const int n = blockIdx.x * kThreads + threadIdx.x;
+    if (n >= N) return;
+    const int i = g.vmajor ? n % g.nu : n / g.nv, j = g.vmajor ? n / g.nu : n % g.nv;
+    if (u[n] != g.u0 + i * g.du || v[n] != g.v0 + j * g.dv) g_grid.ok = 0;
 }
 
 // The grid case on v_mfma_f64_16x16x4f64 (4 points per instruction): lane l
@@ -379,6 +387,8 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
     }
     double4 *steps = reinterpret_cast<double4 *>(part + (((size_t)ranges * (size_t)N + 1) & ~(size_t)1));
     hipLaunchKernelGGL(k_grid_detect, dim3(1), dim3(kDetThreads), 0, nullptr, u, v, N);
+    NFFT2D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_grid_verify, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, u, v, N);
     NFFT2D_LAUNCH_CHECK();
     if (M > 0) {
         hipLaunchKernelGGL(k_ndft_steps, dim3((M + kThreads - 1) / kThreads), dim3(kThreads), 0, nullptr, X, Y,
