@@ -109,7 +109,12 @@ void Destroy_cuFFT_plan_NFFT2_2D(void);
  *   out[n] = sum_{m<M} in[m] e^{-2 pi i (u[n] X[m] / Nu + v[n] Y[m] / Nv)},  n < N,
  * in FP64.  `handle` took a cublasHandle_t; it is accepted and ignored.  The
  * reference materialises the M x N kernel matrix and writes it to a text file
- * (:44,56) before its gemv (:67); neither happens here. */
+ * (:44,56) before its gemv (:67); neither happens here.  Outputs on a meshgrid
+ * (u, v arithmetic progressions, either flattening; detected on the device)
+ * take the FP64 matrix-core contraction, any other (u, v) the direct sum.  Runs
+ * on the legacy default stream.  A workspace of at most 64 N + 2 M complex
+ * values is kept per device and grown when a call needs more (the growth
+ * synchronises the device); concurrent calls on one device serialise on it. */
 int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, const double *u, const double *v,
                  const nfft2d_c128 *in, nfft2d_c128 *out, int Nu, int Nv, int M, int N);
 

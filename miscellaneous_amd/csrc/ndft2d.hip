@@ -364,8 +364,10 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
     const int chunk = ((M + ranges - 1) / ranges + 3) & ~3;   // a multiple of 4 points
     const int dev = [] { int d = 0; (void)hipGetDevice(&d); return d; }();
     double2 *part = nullptr;
+    // held until the launches are queued: a call from another host thread that
+    // grows the workspace frees the old one only after a device synchronise
+    std::lock_guard<std::mutex> lk(g_ws_mu);
     {
-        std::lock_guard<std::mutex> lk(g_ws_mu);
         GridWs &ws = g_ws[dev];
         // partial sums, then (from an even double2 index: 32-byte aligned) the steps
         const size_t need = (((size_t)ranges * (size_t)N + 1) & ~(size_t)1) + 2 * (size_t)M;
