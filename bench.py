@@ -851,6 +851,19 @@ def profile_workload(args):
         x, y, s, t, f = type3_workload(dev)
         for _ in range(args.steps):
             m.NUFFT_2D(x, y, s, t, f, 1e-6)
+    elif args.profile == "pso":
+        P, Nu, Nv, M = 250, 44, 44, 1936
+        rng = np.random.default_rng(SEED + 7)
+        gen = m.FarFieldGeneration(P, Nu, Nv, M, device=dev)
+        g = np.stack(np.meshgrid(np.arange(44), np.arange(44)), -1).reshape(-1, 2).astype(np.float64)
+        xy = (g[None] - 21.5) * 1.9 + rng.uniform(-0.3, 0.3, (P, M, 2))
+        gen.X.copy_(torch.from_numpy(xy[..., 0].astype(np.float32)))
+        gen.Y.copy_(torch.from_numpy(xy[..., 1].astype(np.float32)))
+        gen.excitations.copy_(cn01(torch.Generator().manual_seed(SEED + 7), (P, M), dev))
+        gen.filter.copy_(torch.from_numpy(rng.uniform(0.5, 1.5, 4 * Nu * Nv).astype(np.float32)))
+        graph = gen.capture()
+        for _ in range(args.steps):
+            graph.replay()
     elif args.profile == "ndft_dropin":
         nu, M = 256, 65536
         rng = np.random.default_rng(SEED + 9)
@@ -916,7 +929,7 @@ def main():
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "ndft_dropin", "copy", "type3"],
+    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "ndft_dropin", "pso", "copy", "type3"],
                     help="run only --steps eager calls of one workload (polar 512^2 type-2 / type-1, the "
                          "BF16x3 NDFT at 256^2, M = 65536, the FP64 type-3 at 1M -> 1M, or a 256 MiB device "
                          "copy) and exit: the command "

@@ -728,7 +728,7 @@ static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
         int rc = czt_launch<T, -1, true>(czt_tables<T>(p, 1, true), p->N1, f, p->N2, 1, Tm, p->L2, 1, D1, D2, s);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL((k_t2_rows_direct<T, -1>), dim3(p->N1), dim3(256), sizeof(C) * p->N2, s, f, Tm, D1, D2, tw2,
+        hipLaunchKernelGGL((k_t2_rows_direct<T, -1>), dim3(p->N1), dim3(256), sizeof(C) * 3 * p->N2, s, f, Tm, D1, D2, tw2,
                            p->N1, p->N2);
     }
     NFFT2D_LAUNCH_CHECK();
@@ -746,7 +746,7 @@ static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
                                          (const T *)nullptr, s);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL((k_t2_cols_direct<T, -1>), dim3(p->L2), dim3(256), sizeof(C) * p->N1, s, Tm, G, tw1, p->N1,
+        hipLaunchKernelGGL((k_t2_cols_direct<T, -1>), dim3(p->L2), dim3(256), sizeof(C) * 3 * p->N1, s, Tm, G, tw1, p->N1,
                            p->L2);
     }
     NFFT2D_LAUNCH_CHECK();
@@ -775,7 +775,7 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
                                             (const T *)nullptr, (const T *)nullptr, s);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL((k_t1_cols_direct<T, SIGN>), dim3(p->L2), dim3(256), sizeof(C) * p->L1, s, G, Tp, tw1, p->N1,
+        hipLaunchKernelGGL((k_t1_cols_direct<T, SIGN>), dim3(p->L2), dim3(256), sizeof(C) * 2 * p->L1, s, G, Tp, tw1, p->N1,
                            p->L2);
     }
     NFFT2D_LAUNCH_CHECK();
@@ -792,7 +792,7 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
         int rc = czt_launch<T, SIGN, false>(czt_tables<T>(p, 1, false), p->N1, Tp, p->L2, 1, F, p->N2, 1, D1, D2, s);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL((k_t1_rows_direct<T, SIGN>), dim3(p->N1), dim3(256), sizeof(C) * p->L2, s, Tp, F, D1, D2,
+        hipLaunchKernelGGL((k_t1_rows_direct<T, SIGN>), dim3(p->N1), dim3(256), sizeof(C) * 2 * p->L2, s, Tp, F, D1, D2,
                            tw2, p->N1, p->N2);
     }
     NFFT2D_LAUNCH_CHECK();
@@ -1435,7 +1435,7 @@ extern "C" int nfft1d_type2(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64
                                               p->L, 1, ones, D, s)))
             return rc;
     } else {
-        hipLaunchKernelGGL((k_t2_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->N, s, f, G, ones, D,
+        hipLaunchKernelGGL((k_t2_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * 3 * p->N, s, f, G, ones, D,
                            tw, batch, p->N);
     }
     NFFT2D_LAUNCH_CHECK();
@@ -1471,7 +1471,7 @@ extern "C" int nfft1d_type1(nfft1d_plan *p, nfft2d_c64 *result, const nfft2d_c64
                                                F, p->N, 1, ones, D, s)))
             return rc;
     } else {
-        hipLaunchKernelGGL((k_t1_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * p->L, s, G, F, ones,
+        hipLaunchKernelGGL((k_t1_rows_direct<float, -1>), dim3(batch), dim3(256), sizeof(float2) * 2 * p->L, s, G, F, ones,
                            D, tw, batch, p->N);
     }
     NFFT2D_LAUNCH_CHECK();

@@ -195,7 +195,8 @@ __global__ __launch_bounds__(128) void kb_t2_rows(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N2)) {
-        t2_rows_direct_body<float, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1, d.N2);
+        t2_rows_direct_body<float, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1, d.N2,
+                                       3 * d.N2 <= batch_lds_rows() ? lds + d.N2 : nullptr);
         return;
     }
     batch_dispatch_n(d.N2, [&](auto n) {
@@ -210,7 +211,8 @@ __global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N1)) {
-        t2_cols_direct_body<float, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.N1, d.L2);
+        t2_cols_direct_body<float, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.N1, d.L2,
+                                       3 * d.N1 <= batch_lds_cols() ? lds + d.N1 : nullptr);
         return;
     }
     batch_dispatch_n(d.N1, [&](auto n) {
@@ -254,7 +256,8 @@ __global__ __launch_bounds__(256) void kb_t1_cols(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N1)) {
-        t1_cols_direct_body<float, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.N1, d.L2);
+        t1_cols_direct_body<float, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.N1, d.L2,
+                                       4 * d.N1 <= batch_lds_cols() ? lds + 2 * d.N1 : nullptr);
         return;
     }
     batch_dispatch_n(d.N1, [&](auto n) {
@@ -269,7 +272,8 @@ __global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N2)) {
-        t1_rows_direct_body<float, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1, d.N2);
+        t1_rows_direct_body<float, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1, d.N2,
+                                       4 * d.N2 <= batch_lds_rows() ? lds + 2 * d.N2 : nullptr);
         return;
     }
     batch_dispatch_n(d.N2, [&](auto n) {

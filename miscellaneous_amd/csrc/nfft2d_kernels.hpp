@@ -280,30 +280,66 @@ __global__ __launch_bounds__(rows_per_block<N>() * (N / 8)) void k_t1_rows(
 }
 
 // ================================================================= direct-DFT passes
-// Same four passes for any even N (not a power of two, or N < 8): each output
-// is a direct sum over the line, staged in LDS, with twiddle index
-// (k n) mod L into the 2N-entry table.  O(N^2) per line; used for the
-// reference's non-power-of-two sizes (e.g. 2Nu = 88 in Synthesis/PSO).
+// Twiddle table of a direct pass: staged into LDS when the caller has room
+// (lds_tw non-null), else read from global memory.
+template <typename T>
+__device__ __forceinline__ const cplx<T>* stage_twiddles(const cplx<T>* __restrict__ tw, cplx<T>* lds_tw, int L)
+{
+    if (!lds_tw) return tw;
+    for (int k = threadIdx.x; k < L; k += blockDim.x) lds_tw[k] = tw[k];
+    return lds_tw;
+}
+
+// sum_{p < P} x[p] w[(e0 + p step) mod L], in four interleaved partial sums
+// (independent FMA chains), added as (s0 + s1) + (s2 + s3).
+template <int SIGN, typename T>
+__device__ __forceinline__ cplx<T> direct_sum(const cplx<T>* x, int P, const cplx<T>* w, int e0, int step, int L)
+{
+    using C = cplx<T>;
+    C s0 = mk((T)0, (T)0), s1 = s0, s2 = s0, s3 = s0;
+    int e = e0;
+    auto adv = [&]() {
+        e += step;
+        e -= e >= L ? L : 0;
+    };
+    int p = 0;
+    for (; p + 3 < P; p += 4) {
+        s0 = cadd(s0, cmul(x[p], twid<SIGN>(w, e)));
+        adv();
+        s1 = cadd(s1, cmul(x[p + 1], twid<SIGN>(w, e)));
+        adv();
+        s2 = cadd(s2, cmul(x[p + 2], twid<SIGN>(w, e)));
+        adv();
+        s3 = cadd(s3, cmul(x[p + 3], twid<SIGN>(w, e)));
+        adv();
+    }
+    for (; p < P; ++p) {
+        s0 = cadd(s0, cmul(x[p], twid<SIGN>(w, e)));
+        adv();
+    }
+    return cadd(cadd(s0, s1), cadd(s2, s3));
+}
+
+// Direct DFT passes for line lengths the Stockham passes do not take: each
+// output a sum over the line staged in LDS, with twiddle index (k n) mod L
+// into the 2N-entry table advanced term by term.  O(N^2) per line; used for
+// the reference's non-power-of-two sizes below the chirp-z threshold (e.g.
+// 2Nu = 88 in Synthesis/PSO).  lds_tw: room for the L twiddles in LDS, or null.
 template <typename T, int SIGN>
 __device__ __forceinline__ void t2_rows_direct_body(int i1, cplx<T>* line, const cplx<T>* __restrict__ f,
                                                     cplx<T>* __restrict__ Tm, const T* __restrict__ D1,
                                                     const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1,
-                                                    int N2)
+                                                    int N2, cplx<T>* lds_tw = nullptr)
 {
-    using C = cplx<T>;
     const int L2 = 2 * N2;
+    const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L2);
     for (int i2 = threadIdx.x; i2 < N2; i2 += blockDim.x)
         line[i2] = cscale(f[(size_t)i1 * N2 + i2], D1[i1] * D2[i2]);
     __syncthreads();
-    for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x) {
-        C acc = mk((T)0, (T)0);
-        for (int i2 = 0; i2 < N2; ++i2) {
-            const int k2 = i2 - N2 / 2;
-            const long long e = ((long long)(k2 < 0 ? k2 + L2 : k2) * n2) % L2;
-            acc = cadd(acc, cmul(line[i2], twid<SIGN>(tw, (int)e)));
-        }
-        Tm[(size_t)i1 * L2 + n2] = acc;
-    }
+    // twiddle index ((k2 mod L2) n2) mod L2 for k2 = i2 - N2/2: starts at (L2 - N2/2) n2, steps by n2
+    for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x)
+        Tm[(size_t)i1 * L2 + n2] =
+            direct_sum<SIGN, T>(line, N2, w, (int)(((long long)(L2 - N2 / 2) * n2) % L2), n2, L2);
 }
 
 template <typename T, int SIGN>
@@ -312,27 +348,22 @@ __global__ __launch_bounds__(256) void k_t2_rows_direct(
     const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
 {
     extern __shared__ unsigned char smem_raw[];
-    t2_rows_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), f, Tm, D1, D2, tw, N1, N2);
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    t2_rows_direct_body<T, SIGN>(blockIdx.x, lds, f, Tm, D1, D2, tw, N1, N2, lds + N2);
 }
 
 template <typename T, int SIGN>
 __device__ __forceinline__ void t2_cols_direct_body(int n2, cplx<T>* line, const cplx<T>* __restrict__ Tm,
                                                     cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int N1,
-                                                    int L2)
+                                                    int L2, cplx<T>* lds_tw = nullptr)
 {
-    using C = cplx<T>;
     const int L1 = 2 * N1;
+    const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L1);
     for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) line[i1] = Tm[(size_t)i1 * L2 + n2];
     __syncthreads();
-    for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x) {
-        C acc = mk((T)0, (T)0);
-        for (int i1 = 0; i1 < N1; ++i1) {
-            const int k1 = i1 - N1 / 2;
-            const long long e = ((long long)(k1 < 0 ? k1 + L1 : k1) * n1) % L1;
-            acc = cadd(acc, cmul(line[i1], twid<SIGN>(tw, (int)e)));
-        }
-        Gt[(size_t)n2 * L1 + n1] = acc;
-    }
+    for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x)
+        Gt[(size_t)n2 * L1 + n1] =
+            direct_sum<SIGN, T>(line, N1, w, (int)(((long long)(L1 - N1 / 2) * n1) % L1), n1, L1);
 }
 
 template <typename T, int SIGN>
@@ -340,24 +371,23 @@ __global__ __launch_bounds__(256) void k_t2_cols_direct(
     const cplx<T>* __restrict__ Tm, cplx<T>* __restrict__ Gt, const cplx<T>* __restrict__ tw, int N1, int L2)
 {
     extern __shared__ unsigned char smem_raw[];
-    t2_cols_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Tm, Gt, tw, N1, L2);
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    t2_cols_direct_body<T, SIGN>(blockIdx.x, lds, Tm, Gt, tw, N1, L2, lds + N1);
 }
 
 template <typename T, int SIGN>
 __device__ __forceinline__ void t1_cols_direct_body(int n2, cplx<T>* line, const cplx<T>* __restrict__ Gt,
                                                     cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int N1,
-                                                    int L2)
+                                                    int L2, cplx<T>* lds_tw = nullptr)
 {
-    using C = cplx<T>;
     const int L1 = 2 * N1;
+    const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L1);
     for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x) line[n1] = Gt[(size_t)n2 * L1 + n1];
     __syncthreads();
     for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) {
         const int k1 = i1 - N1 / 2;
-        const long long kk = k1 < 0 ? k1 + L1 : k1;
-        C acc = mk((T)0, (T)0);
-        for (int n1 = 0; n1 < L1; ++n1) acc = cadd(acc, cmul(line[n1], twid<SIGN>(tw, (int)((kk * n1) % L1))));
-        Tp[(size_t)i1 * L2 + n2] = acc;
+        const int kk = k1 < 0 ? k1 + L1 : k1;
+        Tp[(size_t)i1 * L2 + n2] = direct_sum<SIGN, T>(line, L1, w, 0, kk, L1);
     }
 }
 
@@ -366,25 +396,24 @@ __global__ __launch_bounds__(256) void k_t1_cols_direct(
     const cplx<T>* __restrict__ Gt, cplx<T>* __restrict__ Tp, const cplx<T>* __restrict__ tw, int N1, int L2)
 {
     extern __shared__ unsigned char smem_raw[];
-    t1_cols_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Gt, Tp, tw, N1, L2);
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    t1_cols_direct_body<T, SIGN>(blockIdx.x, lds, Gt, Tp, tw, N1, L2, lds + 2 * N1);
 }
 
 template <typename T, int SIGN>
 __device__ __forceinline__ void t1_rows_direct_body(int i1, cplx<T>* line, const cplx<T>* __restrict__ Tp,
                                                     cplx<T>* __restrict__ F, const T* __restrict__ D1,
                                                     const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1,
-                                                    int N2)
+                                                    int N2, cplx<T>* lds_tw = nullptr)
 {
-    using C = cplx<T>;
     const int L2 = 2 * N2;
+    const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L2);
     for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x) line[n2] = Tp[(size_t)i1 * L2 + n2];
     __syncthreads();
     for (int i2 = threadIdx.x; i2 < N2; i2 += blockDim.x) {
         const int k2 = i2 - N2 / 2;
-        const long long kk = k2 < 0 ? k2 + L2 : k2;
-        C acc = mk((T)0, (T)0);
-        for (int n2 = 0; n2 < L2; ++n2) acc = cadd(acc, cmul(line[n2], twid<SIGN>(tw, (int)((kk * n2) % L2))));
-        F[(size_t)i1 * N2 + i2] = cscale(acc, D1[i1] * D2[i2]);
+        const int kk = k2 < 0 ? k2 + L2 : k2;
+        F[(size_t)i1 * N2 + i2] = cscale(direct_sum<SIGN, T>(line, L2, w, 0, kk, L2), D1[i1] * D2[i2]);
     }
 }
 
@@ -394,7 +423,8 @@ __global__ __launch_bounds__(256) void k_t1_rows_direct(
     const T* __restrict__ D2, const cplx<T>* __restrict__ tw, int N1, int N2)
 {
     extern __shared__ unsigned char smem_raw[];
-    t1_rows_direct_body<T, SIGN>(blockIdx.x, reinterpret_cast<cplx<T>*>(smem_raw), Tp, F, D1, D2, tw, N1, N2);
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    t1_rows_direct_body<T, SIGN>(blockIdx.x, lds, Tp, F, D1, D2, tw, N1, N2, lds + 2 * N2);
 }
 
 // ================================================================= setpts (bin sort)
