@@ -290,34 +290,62 @@ __device__ __forceinline__ const cplx<T>* stage_twiddles(const cplx<T>* __restri
     return lds_tw;
 }
 
-// sum_{p < P} x[p] w[(e0 + p step) mod L], in four interleaved partial sums
-// (independent FMA chains), added as (s0 + s1) + (s2 + s3).
+// sum_{p < P, p = r mod G} x[p] w[(e0 + p step) mod L] for this thread's r,
+// in four interleaved partial sums (independent FMA chains), added as
+// (s0 + s1) + (s2 + s3).
 template <int SIGN, typename T>
-__device__ __forceinline__ cplx<T> direct_sum(const cplx<T>* x, int P, const cplx<T>* w, int e0, int step, int L)
+__device__ __forceinline__ cplx<T> direct_sum(const cplx<T>* x, int P, const cplx<T>* w, int e0, int step, int L,
+                                              int r, int G)
 {
     using C = cplx<T>;
     C s0 = mk((T)0, (T)0), s1 = s0, s2 = s0, s3 = s0;
-    int e = e0;
+    int e = (int)((e0 + (long long)r * step) % L);
+    const int st = (int)(((long long)G * step) % L);
     auto adv = [&]() {
-        e += step;
+        e += st;
         e -= e >= L ? L : 0;
     };
-    int p = 0;
-    for (; p + 3 < P; p += 4) {
+    int p = r;
+    for (; p + 3 * G < P; p += 4 * G) {
         s0 = cadd(s0, cmul(x[p], twid<SIGN>(w, e)));
         adv();
-        s1 = cadd(s1, cmul(x[p + 1], twid<SIGN>(w, e)));
+        s1 = cadd(s1, cmul(x[p + G], twid<SIGN>(w, e)));
         adv();
-        s2 = cadd(s2, cmul(x[p + 2], twid<SIGN>(w, e)));
+        s2 = cadd(s2, cmul(x[p + 2 * G], twid<SIGN>(w, e)));
         adv();
-        s3 = cadd(s3, cmul(x[p + 3], twid<SIGN>(w, e)));
+        s3 = cadd(s3, cmul(x[p + 3 * G], twid<SIGN>(w, e)));
         adv();
     }
-    for (; p < P; ++p) {
+    for (; p < P; p += G) {
         s0 = cadd(s0, cmul(x[p], twid<SIGN>(w, e)));
         adv();
     }
     return cadd(cadd(s0, s1), cadd(s2, s3));
+}
+
+// All Nout outputs of a direct pass over the block: for short lines G = 2 or
+// 4 adjacent threads share an output (each sums every G-th term, then they
+// add their partials by cross-lane shuffles), so that the block's lanes stay
+// busy.  G depends on Nout only, never on the block size, so that every
+// launch shape (plans: 256 threads, batch row passes: 128) gives the same bits.
+// term(o, e0, step) gives output o's twiddle start and step; store(o, v).
+template <int SIGN, typename T, typename Term, typename Store>
+__device__ __forceinline__ void direct_outputs(const cplx<T>* x, int P, const cplx<T>* w, int L, int Nout,
+                                               Term&& term, Store&& store)
+{
+    const int NT = blockDim.x;
+    const int G = Nout <= 64 ? 4 : (Nout <= 128 ? 2 : 1);   // NT >= 128 >= G
+    const int r = (int)threadIdx.x % G;
+    for (int o = (int)threadIdx.x / G; o < Nout; o += NT / G) {
+        int e0, step;
+        term(o, e0, step);
+        cplx<T> v = direct_sum<SIGN, T>(x, P, w, e0, step, L, r, G);
+        for (int m = 1; m < G; m <<= 1) {
+            v.x += __shfl_xor(v.x, m, 64);
+            v.y += __shfl_xor(v.y, m, 64);
+        }
+        if (r == 0) store(o, v);
+    }
 }
 
 // Direct DFT passes for line lengths the Stockham passes do not take: each
@@ -337,9 +365,13 @@ __device__ __forceinline__ void t2_rows_direct_body(int i1, cplx<T>* line, const
         line[i2] = cscale(f[(size_t)i1 * N2 + i2], D1[i1] * D2[i2]);
     __syncthreads();
     // twiddle index ((k2 mod L2) n2) mod L2 for k2 = i2 - N2/2: starts at (L2 - N2/2) n2, steps by n2
-    for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x)
-        Tm[(size_t)i1 * L2 + n2] =
-            direct_sum<SIGN, T>(line, N2, w, (int)(((long long)(L2 - N2 / 2) * n2) % L2), n2, L2);
+    direct_outputs<SIGN, T>(
+        line, N2, w, L2, L2,
+        [&](int n2, int& e0, int& step) {
+            e0 = (int)(((long long)(L2 - N2 / 2) * n2) % L2);
+            step = n2;
+        },
+        [&](int n2, cplx<T> v) { Tm[(size_t)i1 * L2 + n2] = v; });
 }
 
 template <typename T, int SIGN>
@@ -361,9 +393,13 @@ __device__ __forceinline__ void t2_cols_direct_body(int n2, cplx<T>* line, const
     const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L1);
     for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) line[i1] = Tm[(size_t)i1 * L2 + n2];
     __syncthreads();
-    for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x)
-        Gt[(size_t)n2 * L1 + n1] =
-            direct_sum<SIGN, T>(line, N1, w, (int)(((long long)(L1 - N1 / 2) * n1) % L1), n1, L1);
+    direct_outputs<SIGN, T>(
+        line, N1, w, L1, L1,
+        [&](int n1, int& e0, int& step) {
+            e0 = (int)(((long long)(L1 - N1 / 2) * n1) % L1);
+            step = n1;
+        },
+        [&](int n1, cplx<T> v) { Gt[(size_t)n2 * L1 + n1] = v; });
 }
 
 template <typename T, int SIGN>
@@ -384,11 +420,14 @@ __device__ __forceinline__ void t1_cols_direct_body(int n2, cplx<T>* line, const
     const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L1);
     for (int n1 = threadIdx.x; n1 < L1; n1 += blockDim.x) line[n1] = Gt[(size_t)n2 * L1 + n1];
     __syncthreads();
-    for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) {
-        const int k1 = i1 - N1 / 2;
-        const int kk = k1 < 0 ? k1 + L1 : k1;
-        Tp[(size_t)i1 * L2 + n2] = direct_sum<SIGN, T>(line, L1, w, 0, kk, L1);
-    }
+    direct_outputs<SIGN, T>(
+        line, L1, w, L1, N1,
+        [&](int i1, int& e0, int& step) {
+            const int k1 = i1 - N1 / 2;
+            e0 = 0;
+            step = k1 < 0 ? k1 + L1 : k1;
+        },
+        [&](int i1, cplx<T> v) { Tp[(size_t)i1 * L2 + n2] = v; });
 }
 
 template <typename T, int SIGN>
@@ -410,11 +449,14 @@ __device__ __forceinline__ void t1_rows_direct_body(int i1, cplx<T>* line, const
     const cplx<T>* w = stage_twiddles<T>(tw, lds_tw, L2);
     for (int n2 = threadIdx.x; n2 < L2; n2 += blockDim.x) line[n2] = Tp[(size_t)i1 * L2 + n2];
     __syncthreads();
-    for (int i2 = threadIdx.x; i2 < N2; i2 += blockDim.x) {
-        const int k2 = i2 - N2 / 2;
-        const int kk = k2 < 0 ? k2 + L2 : k2;
-        F[(size_t)i1 * N2 + i2] = cscale(direct_sum<SIGN, T>(line, L2, w, 0, kk, L2), D1[i1] * D2[i2]);
-    }
+    direct_outputs<SIGN, T>(
+        line, L2, w, L2, N2,
+        [&](int i2, int& e0, int& step) {
+            const int k2 = i2 - N2 / 2;
+            e0 = 0;
+            step = k2 < 0 ? k2 + L2 : k2;
+        },
+        [&](int i2, cplx<T> v) { F[(size_t)i1 * N2 + i2] = cscale(v, D1[i1] * D2[i2]); });
 }
 
 template <typename T, int SIGN>
