@@ -264,11 +264,12 @@ int nfft1d_type1(nfft1d_plan *plan, nfft2d_c64 *result, const nfft2d_c64 *data, 
 /* ------------------------------------------------------------------ batches */
 
 /* Many complex64 transforms of different sizes (K = 6, c = 2; N1, N2 even in
- * [2, 1024]: powers of two from 8 run Stockham passes, other sizes direct
- * ones, so that a transform's result is bitwise its own plan's except for
- * sizes from 128 on that are not powers of two, where a plan runs chirp-z
- * passes and the two agree to rounding; any M), run as ONE kernel launch per
- * stage over all of them:
+ * [2, 1024]: powers of two from 8 run Stockham passes, other sizes the
+ * plan's chirp-z passes from 128 to 682 and direct ones otherwise, so that a
+ * transform's result is bitwise its own plan's except for sizes from 684 on
+ * that are not powers of two, where a plan runs chirp-z passes and the batch
+ * direct ones and the two agree to rounding; any M), run as ONE kernel launch
+ * per stage over all of them:
  * nfft2d_batch_setpts is 5 launches, nfft2d_batch_type2 and type1 4 each,
  * whatever the number of transforms.  Neither allocates nor records a
  * hipMemset, so both can be captured into one hipGraph and replayed.  There is

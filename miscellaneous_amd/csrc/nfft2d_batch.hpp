@@ -13,6 +13,7 @@
 // allocation, so the whole sequence can be captured into one hipGraph.
 #pragma once
 
+#include "nfft2d_czt.hpp"
 #include "nfft2d_kernels.hpp"
 
 namespace nfft2d {
@@ -42,6 +43,9 @@ struct BatchDesc {
     // type-2 gather: subproblems of at most dmax points are read point by point
     // from the grid (kb_gather_direct), larger ones through an LDS tile
     int dmax;
+    // chirp-z tables of the plan per dimension (null: Stockham or direct passes)
+    const void *czt1, *czt2;
+    int cztm1, cztm2;
 };
 
 // Largest N of the batched FFT passes.
@@ -182,10 +186,16 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ 
 
 // ---------------------------------------------------------------- grid passes
 // Row passes: 128 threads, block o.y of the transform's rows (R rows of N2 for
-// the Stockham passes, one row for the direct ones); column passes: 256
-// threads, likewise over the L2 columns.  The LDS buffer fits the largest N
-// of either form (the direct passes stage one line of up to 2 kBatchMaxN).
-constexpr int kBatchLineLds = 2 * kBatchMaxN;
+// the Stockham passes, one row for the chirp-z and direct ones); column
+// passes: 256 threads, likewise over the L2 columns.  The LDS buffer fits the
+// largest N of every form: the direct passes stage one line of up to
+// 2 kBatchMaxN, the chirp-z passes (the plans' for lengths up to
+// kBatchCztMaxN; longer ones take the direct passes here) one FFT of up to
+// 2048 points with its twiddles.
+constexpr int kBatchCztMaxM = 2048;
+constexpr int kBatchCztU = 4;   // values per thread: 4 U >= kBatchCztMaxM / 128
+constexpr int kBatchLineLds = 2 * kBatchMaxN > czt_lds_entries(kBatchCztMaxM) ? 2 * kBatchMaxN
+                                                                             : czt_lds_entries(kBatchCztMaxM);
 constexpr int batch_lds_rows() { return batch_rows_lds() > kBatchLineLds ? batch_rows_lds() : kBatchLineLds; }
 constexpr int batch_lds_cols() { return batch_cols_lds() > kBatchLineLds ? batch_cols_lds() : kBatchLineLds; }
 
@@ -195,8 +205,12 @@ __global__ __launch_bounds__(128) void kb_t2_rows(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N2)) {
-        t2_rows_direct_body<float, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1, d.N2,
-                                       3 * d.N2 <= batch_lds_rows() ? lds + d.N2 : nullptr);
+        if (d.czt2)   // row o.y: f[k1][*] (N2) -> Tm[k1][*] (L2), pre-compensated
+            czt_line<float, -1, true, kBatchCztU>(o.y, lds, d.f, d.N2, 1, d.Tm, 2 * d.N2, 1, d.D1, d.D2,
+                                                  czt_tables_of<float>(d.czt2, d.N2, d.cztm2, true));
+        else
+            t2_rows_direct_body<float, -1>(o.y, lds, d.f, d.Tm, d.D1, d.D2, d.tw2, d.N1, d.N2,
+                                           3 * d.N2 <= batch_lds_rows() ? lds + d.N2 : nullptr);
         return;
     }
     batch_dispatch_n(d.N2, [&](auto n) {
@@ -211,8 +225,13 @@ __global__ __launch_bounds__(256) void kb_t2_cols(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N1)) {
-        t2_cols_direct_body<float, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.N1, d.L2,
-                                       3 * d.N1 <= batch_lds_cols() ? lds + d.N1 : nullptr);
+        if (d.czt1)   // column o.y: Tm[*][n2] (N1) -> Gt[n2][*] (L1)
+            czt_line<float, -1, true, kBatchCztU>(o.y, lds, d.Tm, 1, d.L2, d.Gt, d.L1, 1, (const float*)nullptr,
+                                                  (const float*)nullptr,
+                                                  czt_tables_of<float>(d.czt1, d.N1, d.cztm1, true));
+        else
+            t2_cols_direct_body<float, -1>(o.y, lds, d.Tm, d.Gt, d.tw1, d.N1, d.L2,
+                                           3 * d.N1 <= batch_lds_cols() ? lds + d.N1 : nullptr);
         return;
     }
     batch_dispatch_n(d.N1, [&](auto n) {
@@ -256,8 +275,13 @@ __global__ __launch_bounds__(256) void kb_t1_cols(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N1)) {
-        t1_cols_direct_body<float, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.N1, d.L2,
-                                       4 * d.N1 <= batch_lds_cols() ? lds + 2 * d.N1 : nullptr);
+        if (d.czt1)   // column o.y: Gt[n2][*] (L1) -> Tm[*][n2] (N1)
+            czt_line<float, -1, false, kBatchCztU>(o.y, lds, d.Gt, d.L1, 1, d.Tm, 1, d.L2, (const float*)nullptr,
+                                                   (const float*)nullptr,
+                                                   czt_tables_of<float>(d.czt1, d.N1, d.cztm1, false));
+        else
+            t1_cols_direct_body<float, -1>(o.y, lds, d.Gt, d.Tm, d.tw1, d.N1, d.L2,
+                                           4 * d.N1 <= batch_lds_cols() ? lds + 2 * d.N1 : nullptr);
         return;
     }
     batch_dispatch_n(d.N1, [&](auto n) {
@@ -272,8 +296,12 @@ __global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ 
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     if (!batch_fft_n(d.N2)) {
-        t1_rows_direct_body<float, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1, d.N2,
-                                       4 * d.N2 <= batch_lds_rows() ? lds + 2 * d.N2 : nullptr);
+        if (d.czt2)   // row o.y: Tm[k1][*] (L2) -> F1[k1][*] (N2), deconvolved
+            czt_line<float, -1, false, kBatchCztU>(o.y, lds, d.Tm, 2 * d.N2, 1, d.F1, d.N2, 1, d.D1, d.D2,
+                                                   czt_tables_of<float>(d.czt2, d.N2, d.cztm2, false));
+        else
+            t1_rows_direct_body<float, -1>(o.y, lds, d.Tm, d.F1, d.D1, d.D2, d.tw2, d.N1, d.N2,
+                                           4 * d.N2 <= batch_lds_rows() ? lds + 2 * d.N2 : nullptr);
         return;
     }
     batch_dispatch_n(d.N2, [&](auto n) {

@@ -32,6 +32,45 @@ struct CztTables {
 
 __host__ __device__ constexpr int czt_lds_entries(int m) { return m + (m >= 64 ? m / 64 : 1) + 64; }
 
+// Smallest power of two m >= 3N - 1 (P + Q - 1 = N + 2N - 1): the chirp-z FFT length of a line of N.
+__host__ __device__ constexpr int czt_m(int N)
+{
+    int m = 1;
+    while (m < 3 * N - 1) m <<= 1;
+    return m;
+}
+
+// The chirp-z tables of a line length N inside one table buffer (layout
+// [cin2 N][cout2 L][hhat2 m][cin1 L][cout1 N][hhat1 m][T1 m/64][T2 64],
+// L = 2N), for its type-2 (t2 = true) or type-1 pass.
+template <typename T>
+__host__ __device__ inline CztTables<T> czt_tables_of(const void* buf, int N, int m, bool t2)
+{
+    using C = cplx<T>;
+    const int L = 2 * N;
+    const C* b = (const C*)buf;
+    CztTables<T> t;
+    t.m = m;
+    const C* tw = b + 3 * L + 2 * m;   // after both passes' chirps and kernels
+    t.t1 = tw;
+    t.t2 = tw + (m >= 64 ? m / 64 : 1);
+    if (t2) {
+        t.P = N;
+        t.Q = L;
+        t.cin = b;
+        t.cout = b + N;
+        t.hhat = b + N + L;
+    } else {
+        const C* o = b + N + L + m;
+        t.P = L;
+        t.Q = N;
+        t.cin = o;
+        t.cout = o + L;
+        t.hhat = o + L + N;
+    }
+    return t;
+}
+
 // In-place radix-4 (+ one radix-2) Stockham FFT of m = 2^a points in LDS
 // (Govindaraju et al., SC 2008): stage with span Ns reads v_r = X[j + r m/R],
 // twiddles it by e^{-2 pi i r (j mod Ns) / (Ns R)}, and writes the R-point DFT
@@ -110,20 +149,23 @@ __device__ void czt_fft(cplx<T> *X, int m, const cplx<T> *t1, const cplx<T> *t2)
     }
 }
 
-// One line per block.  Line b reads x[p] = in[b * in_ls + p * in_es] and
-// writes out[b * out_ls + q * out_es].  Type-2 rows scale the input by
-// sl[b] se[p] (the pre-compensation D1[k1] D2[k2]); type-1 rows scale the
-// output by sl[b] se[q] (the deconvolution); null: no scaling.
+// Line b of a chirp-z pass, by the whole block (smem: czt_lds_entries(m)
+// entries; blockDim.x >= m / (4 U)).  The line reads x[p] = in[b * in_ls +
+// p * in_es] and writes out[b * out_ls + q * out_es].  Type-2 rows scale the
+// input by sl[b] se[p] (the pre-compensation D1[k1] D2[k2]); type-1 rows
+// scale the output by sl[b] se[q] (the deconvolution); null: no scaling.
+// Every butterfly and product is the same whatever the block size, so plans
+// and batches (other block sizes) get the same bits.
 template <typename T, int SIGN, bool SCALE_IN, int U>
-__global__ __launch_bounds__(1024) void k_czt(const cplx<T> *__restrict__ in, long long in_ls, long long in_es,
-                                              cplx<T> *__restrict__ out, long long out_ls, long long out_es,
-                                              const T *__restrict__ sl, const T *__restrict__ se, CztTables<T> tb)
+__device__ __forceinline__ void czt_line(int b, cplx<T>* smem, const cplx<T>* __restrict__ in, long long in_ls,
+                                         long long in_es, cplx<T>* __restrict__ out, long long out_ls,
+                                         long long out_es, const T* __restrict__ sl, const T* __restrict__ se,
+                                         const CztTables<T>& tb)
 {
     using C = cplx<T>;
-    extern __shared__ unsigned char smem_raw[];
-    C *X = reinterpret_cast<C *>(smem_raw);
+    C *X = smem;
     C *t1 = X + tb.m, *t2 = t1 + (tb.m >= 64 ? tb.m / 64 : 1);
-    const int NT = blockDim.x, b = blockIdx.x, m = tb.m;
+    const int NT = blockDim.x, m = tb.m;
     for (int k = threadIdx.x; k < 64; k += NT) t2[k] = tb.t2[k];
     for (int k = threadIdx.x; k < (m >= 64 ? m / 64 : 1); k += NT) t1[k] = tb.t1[k];
     const T lsc = sl ? sl[b] : (T)1;
@@ -149,6 +191,17 @@ __global__ __launch_bounds__(1024) void k_czt(const cplx<T> *__restrict__ in, lo
         if (!SCALE_IN && sl) v = cscale(v, lsc * se[q]);
         out[b * out_ls + q * out_es] = v;
     }
+}
+
+// One line per block.
+template <typename T, int SIGN, bool SCALE_IN, int U>
+__global__ __launch_bounds__(1024) void k_czt(const cplx<T> *__restrict__ in, long long in_ls, long long in_es,
+                                              cplx<T> *__restrict__ out, long long out_ls, long long out_es,
+                                              const T *__restrict__ sl, const T *__restrict__ se, CztTables<T> tb)
+{
+    extern __shared__ unsigned char smem_raw[];
+    czt_line<T, SIGN, SCALE_IN, U>((int)blockIdx.x, reinterpret_cast<cplx<T> *>(smem_raw), in, in_ls, in_es, out,
+                                   out_ls, out_es, sl, se, tb);
 }
 
 }  // namespace nfft2d

@@ -138,3 +138,42 @@ def test_pso_generation_far_fields():
             F = oracle.ndft_type1(gen.excitations[p].cpu().numpy().astype(np.complex128), gen.X[p].cpu().numpy(),
                                   gen.Y[p].cpu().numpy(), 2 * Nv, 2 * Nu) * filt.reshape(2 * Nv, 2 * Nu)
             assert oracle.rel_l2(out[p].cpu().numpy(), F) < 1e-5
+
+
+def test_batch_chirp_z_sizes():
+    """Sizes from 128 on that are not powers of two: up to 682 the batch runs its plan's chirp-z
+    passes (type-2 bitwise equal to the plan), beyond it direct passes (equal to rounding); both
+    directions within 1e-5 of the FP64 direct sums."""
+    rng = np.random.default_rng(43)
+    sizes = [(200, 130, 5000), (130, 64, 3000), (682, 66, 4000), (700, 64, 3000), (64, 250, 2000)]
+    xs, ys, fs, ds = [], [], [], []
+    for (N1, N2, M) in sizes:
+        x1, x2 = pts(rng, N1, N2, M)
+        xs.append(T(x1)); ys.append(T(x2)); fs.append(T(cn01(rng, (N1, N2)))); ds.append(T(cn01(rng, M)))
+    out2 = [torch.empty(M, dtype=torch.complex64, device=DEV) for (_, _, M) in sizes]
+    out1 = [torch.empty(N1 * N2, dtype=torch.complex64, device=DEV) for (N1, N2, _) in sizes]
+    b = m.Batch(sizes, device=DEV).set_points(xs, ys).set_io(fs, out2)
+    b.setpts_type2()
+    b.set_io_type1(ds, out1)
+    b.type1()
+    torch.cuda.synchronize()
+    for t, (N1, N2, M) in enumerate(sizes):
+        print(f"case batch chirp-z transform {t}: {N1}x{N2} M={M}", flush=True)
+        plan = m.Plan(N1, N2, device=DEV).setpts(xs[t], ys[t])
+        y_plan = plan.type2(fs[t])
+        if max(N1, N2) <= 682:
+            assert torch.equal(y_plan, out2[t]), t
+        else:
+            assert oracle.rel_l2(out2[t].cpu().numpy(), y_plan.cpu().numpy()) < 1e-5, t
+        sel = rng.choice(M, 256, replace=False)
+        ref2 = oracle.ndft_type2(fs[t].cpu().numpy().astype(np.complex128), xs[t].cpu().numpy()[sel],
+                                 ys[t].cpu().numpy()[sel])
+        assert oracle.rel_l2(out2[t].cpu().numpy()[sel], ref2) < 1e-5, t
+        got1 = out1[t].cpu().numpy().reshape(N1, N2)
+        i1 = rng.integers(0, N1, 256)
+        i2 = rng.integers(0, N2, 256)
+        ref1 = oracle.ndft_type1_modes(ds[t].cpu().numpy().astype(np.complex128), xs[t].cpu().numpy(),
+                                       ys[t].cpu().numpy(), N1, N2, i1, i2)
+        assert oracle.rel_l2(got1[i1, i2], ref1) < 1e-5, t
+        tol = 1e-6 if max(N1, N2) <= 682 else 3e-6    # the same passes, or chirp-z against direct
+        assert oracle.rel_l2(got1.ravel(), plan.type1(ds[t]).cpu().numpy().ravel()) < tol, t
