@@ -118,6 +118,14 @@ void Destroy_cuFFT_plan_NFFT2_2D(void);
 int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, const double *u, const double *v,
                  const nfft2d_c128 *in, nfft2d_c128 *out, int Nu, int Nv, int M, int N);
 
+/* No counterpart in the reference; for callers and tests that check which path
+ * an NDFT2_2D_GPU call took.  Synchronises the current device and reports the
+ * detection of its last NDFT2_2D_GPU call with N > 0: *is_grid = 1 when the
+ * outputs were a grid of *nu x *nv values (*vmajor = 0: n = i nv + j, 1:
+ * n = j nu + i) and took the matrix-core contraction, 0 when they took the
+ * direct sum.  Before any such call: all zero. */
+int nfft2d_ndft_grid_info(int *is_grid, int *nu, int *nv, int *vmajor);
+
 /* ------------------------------------------------------------------ plan API */
 
 /* A plan holds the twiddle and deconvolution tables, the oversampled grid and

@@ -15,8 +15,8 @@ Everything here runs through ``libnfft2d.so`` (built in-tree by
 ``miscellaneous_amd.build``); there is no CPU fallback.
 """
 from ._lib import NFFT2DError, load_library, LIB_PATH
-from .nfft import (Plan, NFFT2_2D_GPU, NFFT1_2D_GPU, NDFT2_2D_GPU, Calculate_cuFFT_plan_C2C_NFFT2_2D,
-                   Calculate_cuFFT_plan_Z2Z_NFFT2_2D, Destroy_cuFFT_plan_NFFT2_2D)
+from .nfft import (Plan, NFFT2_2D_GPU, NFFT1_2D_GPU, NDFT2_2D_GPU, ndft_grid_info,
+                   Calculate_cuFFT_plan_C2C_NFFT2_2D, Calculate_cuFFT_plan_Z2Z_NFFT2_2D, Destroy_cuFFT_plan_NFFT2_2D)
 from .autograd import NFFT2DType2, nfft2d_type2
 from .ndft import NdftPlan
 from .batch import Batch
@@ -26,6 +26,6 @@ from .pso import FarFieldGeneration
 from .nufft3 import NUFFT_2D, NUFFT_2D_host, nufft3_params
 
 __all__ = ["NFFT2DError", "load_library", "LIB_PATH", "Plan", "NFFT2_2D_GPU", "NFFT1_2D_GPU", "NDFT2_2D_GPU",
-           "Calculate_cuFFT_plan_C2C_NFFT2_2D", "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D",
+           "ndft_grid_info", "Calculate_cuFFT_plan_C2C_NFFT2_2D", "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D",
            "NFFT2DType2", "nfft2d_type2", "NdftPlan", "Batch", "knabNFFT2D", "rectNFFT2D", "knab_window",
            "Plan1D", "NFFT1_1D", "NFFT2_1D", "FarFieldGeneration", "NUFFT_2D", "NUFFT_2D_host", "nufft3_params"]

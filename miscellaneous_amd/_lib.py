@@ -50,6 +50,7 @@ _SIGS = {
     "Calculate_cuFFT_plan_Z2Z_NFFT2_2D": (None, [_i, _i]),
     "Destroy_cuFFT_plan_NFFT2_2D": (None, []),
     "NDFT2_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i]),
+    "nfft2d_ndft_grid_info": (_i, [ctypes.POINTER(_i)] * 4),
     "nfft2d_debug_fail_alloc": (None, [ctypes.c_longlong]),
     "nfft2d_debug_alloc_count": (ctypes.c_longlong, []),
     "NUFFT_2D_GPU": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i, _i]),

@@ -60,18 +60,29 @@ __global__ __launch_bounds__(kDetThreads) void k_grid_detect(const double *__res
     }
     __syncthreads();
     const double u0 = u[0], v0 = v[0];
-    // first change of u and of v: each thread's first hit is its smallest
-    // (its indices increase), so it stops once it has both, and the block
-    // takes the minimum with one LDS atomic per thread and value (an atomic
-    // per element serialised the scan: 425 us at 65,536 outputs)
-    int fu = N, fv = N;
-    for (int n = 1 + t; n < N && (fu == N || fv == N); n += kDetThreads) {
-        if (fu == N && u[n] != u0) fu = n;
-        if (fv == N && v[n] != v0) fv = n;
+    // first change of u and of v, block-wide over chunks of kDetThreads
+    // outputs: each wave's first hit is the lowest set bit of its ballot, one
+    // LDS atomic per wave and value takes the block's minimum, and the block
+    // stops at the first chunk after which both are found (later chunks hold
+    // only larger indices).  An atomic per element serialised on one address
+    // (425 us at 65,536 outputs); threads scanning on their own until their
+    // own first hits ran the whole array where a stride of kDetThreads never
+    // meets a change (29 us on a 256 x 256 grid).
+    const int lane = t & 63;
+    for (int base = 1; base < N; base += kDetThreads) {
+        const int n = base + t;
+        const bool in = n < N;
+        const unsigned long long bu = __ballot(in && u[n] != u0);
+        const unsigned long long bv = __ballot(in && v[n] != v0);
+        if (lane == 0) {   // n is the wave's first index
+            if (bu) atomicMin(&s_run[0], n + __ffsll((long long)bu) - 1);
+            if (bv) atomicMin(&s_run[1], n + __ffsll((long long)bv) - 1);
+        }
+        __syncthreads();
+        const bool done = s_run[0] < N && s_run[1] < N;
+        __syncthreads();   // every thread has read before the next chunk's atomics
+        if (done) break;
     }
-    if (fu < N) atomicMin(&s_run[0], fu);
-    if (fv < N) atomicMin(&s_run[1], fv);
-    __syncthreads();
     const int ru = s_run[0], rv = s_run[1];
     // u-major: v changes at once and u after nv = ru outputs; v-major: the converse
     int vmajor = -1, nu = 1, nv = 1;
@@ -407,6 +418,20 @@ extern "C" int NDFT2_2D_GPU(void *handle, const double *X, const double *Y, cons
                        reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), 1.0 / Nu, 1.0 / Nv, M,
                        N);
     NFFT2D_LAUNCH_CHECK();
+    return NFFT2D_OK;
+}
+
+extern "C" int nfft2d_ndft_grid_info(int *is_grid, int *nu, int *nv, int *vmajor)
+{
+    if (!is_grid || !nu || !nv || !vmajor) return nfft2d_fail(NFFT2D_ERR_ARG, "nfft2d_ndft_grid_info: null pointer");
+    GridInfo g;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpyFromSymbol(&g, HIP_SYMBOL(g_grid), sizeof(g), 0, hipMemcpyDeviceToHost) != hipSuccess)
+        return nfft2d_fail(NFFT2D_ERR_HIP, "nfft2d_ndft_grid_info: reading the detector's finding failed");
+    *is_grid = g.ok;
+    *nu = g.nu;
+    *nv = g.nv;
+    *vmajor = g.vmajor;
     return NFFT2D_OK;
 }
 

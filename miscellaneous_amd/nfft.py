@@ -30,8 +30,9 @@ import torch
 from . import _lib
 from ._lib import NFFT2D_C64, NFFT2D_C128, NFFT2DError, check
 
-__all__ = ["Plan", "NFFT2_2D_GPU", "NFFT1_2D_GPU", "NDFT2_2D_GPU", "Calculate_cuFFT_plan_C2C_NFFT2_2D",
-           "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D", "NFFT2DError"]
+__all__ = ["Plan", "NFFT2_2D_GPU", "NFFT1_2D_GPU", "NDFT2_2D_GPU", "ndft_grid_info",
+           "Calculate_cuFFT_plan_C2C_NFFT2_2D", "Calculate_cuFFT_plan_Z2Z_NFFT2_2D", "Destroy_cuFFT_plan_NFFT2_2D",
+           "NFFT2DError"]
 
 _REAL = {torch.complex64: torch.float32, torch.complex128: torch.float64}
 _CODE = {torch.complex64: NFFT2D_C64, torch.complex128: NFFT2D_C128}
@@ -351,6 +352,16 @@ def NDFT2_2D_GPU(handle, X, Y, u, v, in_, out, Nu: int, Nv: int, M: int, N: int)
             torch.cuda.current_stream(dev).synchronize()
         check(lib.NDFT2_2D_GPU(ctypes.c_void_p(handle if isinstance(handle, int) else 0), _ptr(X), _ptr(Y), _ptr(u),
                                _ptr(v), _ptr(in_), _ptr(out), int(Nu), int(Nv), int(M), int(N)), "NDFT2_2D_GPU")
+
+
+def ndft_grid_info(device=None) -> tuple[bool, int, int, int]:
+    """(is_grid, nu, nv, vmajor) of the last ``NDFT2_2D_GPU`` call with outputs on ``device``: whether its
+    outputs were detected as a grid of nu x nv values and took the matrix-core contraction.  Synchronises."""
+    lib = _lib.load_library()
+    vals = [ctypes.c_int(0) for _ in range(4)]
+    with torch.cuda.device(device if device is not None else torch.cuda.current_device()):
+        check(lib.nfft2d_ndft_grid_info(*[ctypes.byref(x) for x in vals]), "nfft2d_ndft_grid_info")
+    return bool(vals[0].value), vals[1].value, vals[2].value, vals[3].value
 
 
 def Calculate_cuFFT_plan_C2C_NFFT2_2D(N1: int, N2: int) -> None:

@@ -139,10 +139,12 @@ def _meshgrid(nu, nv, u0, v0, vmajor):
 
 
 @pytest.mark.parametrize("nu,nv,M,vmajor", [(88, 88, 1936, 0), (88, 88, 1936, 1), (16, 40, 777, 0), (40, 16, 5, 1),
-                                            (1, 33, 300, 0), (33, 1, 300, 1), (7, 9, 0, 0)])
+                                            (1, 33, 300, 0), (33, 1, 300, 1), (7, 9, 0, 0), (3, 2000, 50, 0),
+                                            (1500, 2, 50, 1), (2, 2049, 20, 0)])
 def test_ndft2_2d_dropin_on_grids(nu, nv, M, vmajor):
     """NDFT2_2D_GPU on meshgrid outputs (both flattenings, ragged tiles, a single row or column, no
-    points): the FP64 matrix-core contraction against the oracle's direct sum (NFFT/NDFT2_2D.cu:28-29,67)."""
+    points, runs longer than the detector's 1024-output chunks): the FP64 matrix-core contraction,
+    detected as the grid it is, against the oracle's direct sum (NFFT/NDFT2_2D.cu:28-29,67)."""
     rng = np.random.default_rng(nu * 1000 + nv + M)
     X = rng.uniform(-nu / 2, nu / 2, M)
     Y = rng.uniform(-nv / 2, nv / 2, M)
@@ -152,6 +154,7 @@ def test_ndft2_2d_dropin_on_grids(nu, nv, M, vmajor):
     print(f"case NDFT2_2D_GPU grid {nu}x{nv} M={M} vmajor={vmajor}", flush=True)
     m.NDFT2_2D_GPU(0, T(X), T(Y), T(u), T(v), T(d), out, nu, nv, M, nu * nv)
     torch.cuda.synchronize()
+    assert m.ndft_grid_info() == (True, nu, nv, vmajor)
     ref = oracle.ndft2_2d(X, Y, u, v, d, nu, nv)
     got = out.cpu().numpy()
     if M == 0:
@@ -175,12 +178,15 @@ def test_ndft2_2d_dropin_grid_detection_falls_back():
     p = rng.permutation(nu * nv)
     cases.append((u[p], v[p]))
     cases.append((u[:nu * nv - 3], v[:nu * nv - 3]))
+    cases.append((np.full(3000, 1.0), np.full(3000, -2.0)))   # no change of u or v in any chunk
     cases.append((u, v))
-    for uu, vv in cases:
+    for k, (uu, vv) in enumerate(cases):
         out = torch.empty(len(uu), dtype=torch.complex128, device=DEV)
         m.NDFT2_2D_GPU(0, T(X), T(Y), T(uu), T(vv), T(d), out, nu, nv, M, len(uu))
         torch.cuda.synchronize()
+        assert m.ndft_grid_info()[0] == (k == len(cases) - 1)
         assert oracle.rel_l2(out.cpu().numpy(), oracle.ndft2_2d(X, Y, uu, vv, d, nu, nv)) < 1e-12
+    assert m.ndft_grid_info() == (True, nu, nv, 0)
 
 
 def test_ndft2_2d_dropin_256_grid_65536_points():
@@ -196,6 +202,7 @@ def test_ndft2_2d_dropin_256_grid_65536_points():
     print("case NDFT2_2D_GPU grid 256x256 M=65536", flush=True)
     m.NDFT2_2D_GPU(0, T(X), T(Y), T(u), T(v), T(d), out, nu, nv, M, nu * nv)
     torch.cuda.synchronize()
+    assert m.ndft_grid_info() == (True, nu, nv, 0)
     sel = rng.choice(nu * nv, 1024, replace=False)
     ref = oracle.ndft2_2d(X, Y, u[sel], v[sel], d, nu, nv)
     e = oracle.rel_l2(out.cpu().numpy()[sel], ref)
