@@ -864,6 +864,15 @@ def profile_workload(args):
         graph = gen.capture()
         for _ in range(args.steps):
             graph.replay()
+    elif args.profile == "batch":
+        sizes, rng = batch_sizes(SEED + 3)
+        fs = [cn01(gen, (n1, n2), dev) for (n1, n2, _) in sizes]
+        pts = [batch_points(rng, n1, mm) for (n1, _, mm) in sizes]
+        outs = [torch.empty(mm, dtype=torch.complex64, device=dev) for (_, _, mm) in sizes]
+        b = m.Batch(sizes, device=dev).set_points([torch.from_numpy(a).to(dev) for a, _ in pts],
+                                                  [torch.from_numpy(c).to(dev) for _, c in pts]).set_io(fs, outs)
+        for _ in range(args.steps):
+            b.setpts_type2()
     elif args.profile == "ndft_dropin":
         nu, M = 256, 65536
         rng = np.random.default_rng(SEED + 9)
@@ -929,7 +938,8 @@ def main():
                     help="JSON with rocprofv3-measured HBM bytes per gather launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (to scale FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "ndft_dropin", "pso", "copy", "type3"],
+    ap.add_argument("--profile", choices=["type2", "type1", "ndft_bf16x3", "ndft_dropin", "pso", "copy", "type3",
+                                          "batch"],
                     help="run only --steps eager calls of one workload (polar 512^2 type-2 / type-1, the "
                          "BF16x3 NDFT at 256^2, M = 65536, the FP64 type-3 at 1M -> 1M, or a 256 MiB device "
                          "copy) and exit: the command "
