@@ -34,6 +34,12 @@ remaining smooth function to FP64 rounding.
 
 Sizes: a N and b M must be even (the library's even-size plans; the reference
 also has an odd branch), and at most 4096 (complex64) or 2048 (complex128).
+
+Repeated calls on one geometry (the same grid tensor objects, not modified in
+place since, on the same stream: the version counters tell) reuse the grid
+statistics, the output points and the plan's tile sort: such a call does not
+synchronise with the host, and costs the spectrum, its windowing, the type-2's
+grid passes and gather, and the final pass.
 """
 from __future__ import annotations
 
@@ -128,6 +134,7 @@ def clear_plan_cache() -> None:
     while _plans:
         _plans.pop(next(iter(_plans))).close()
     _windows.clear()
+    _geoms.clear()
 
 
 def knabNFFT2D(ExtrapVal, RHO_IN: torch.Tensor, COSALPHA_IN: torch.Tensor, INPUT_DATA: torch.Tensor,
@@ -160,10 +167,60 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
         raise ValueError("RHO_OUT and COSALPHA_OUT must be matrices of one shape")
     if min(RHO_OUT.shape) < 2:   # the output spacing (knabNFFT2D.m:36-37) divides by len - 1
         raise ValueError("RHO_OUT must be at least 2-by-2.")
-    f64 = torch.float64
     lib = _lib.load_library()
     st = _stream(dev)
     cdt = torch.complex128 if INPUT_DATA.dtype == torch.complex128 else torch.complex64
+    ct = NFFT2D_C128 if cdt == torch.complex128 else NFFT2D_C64
+    M, N = INPUT_DATA.shape
+    g = _geometry(window, RHO_IN, COSALPHA_IN, RHO_OUT, COSALPHA_OUT, B1, B2, P, M, N, cdt, dev)
+    a, b, len1, len2, C2 = g["a"], g["b"], g["len1"], g["len2"], g["C2"]
+    data = INPUT_DATA.to(cdt).contiguous()           # M (cos alpha) x N (rho)
+    # the spectrum of the samples (cos alpha x rho); the kernel below reads it transposed
+    S = torch.fft.fft2(data).contiguous()
+    F = torch.empty(a * N, b * M, dtype=cdt, device=dev)
+    res = torch.empty(len1, len2, dtype=cdt, device=dev)
+    if window == "knab":
+        Gn, Gm = _window_tables(g["wn"], dev), _window_tables(g["wm"], dev)
+        scale = float(N * a) * float(M * b)
+    else:   # G = 1 and the samples divided by N M (rectNFFT2D.m:70)
+        Gn = Gm = None
+        scale = float(a) * float(b)
+    ex = complex(ExtrapVal)
+    with torch.cuda.device(dev):
+        # repeated periodically to 2 kg + 1 frequencies, windowed (Selva eq. 25), zero padded to a N x b M
+        # with frequency 0 at (a N / 2, b M / 2) (the library's centred layout) and conjugated
+        check(lib.nfft2d_knab_spectrum(ct, _ptr(S), M, N, g["kgn"], g["kgm"], _ptr(Gn), _ptr(Gm), scale, _ptr(F),
+                                       a * N, b * M, st), "nfft2d_knab_spectrum")
+    # the tile sort only when the plan's points are not already this geometry's
+    y = _plan(a * N, b * M, cdt, dev).setpts_type2_if_changed(g["x1"], g["x2"], F)
+    with torch.cuda.device(dev):
+        check(lib.nfft2d_knab_finish(ct, _ptr(y), len1, len2, C2, float(a * b * N * M), _ptr(g["ro"]), _ptr(g["co"]),
+                                     g["strides"], g["rmin"], g["rmax"], g["cmin"], g["cmax"], ex.real, ex.imag,
+                                     _ptr(res), st), "nfft2d_knab_finish")
+    return res
+
+
+_geoms: list = []      # geometry entries, most recently used last
+_MAX_GEOMS = 2
+
+
+def _geometry(window, RHO_IN, COSALPHA_IN, RHO_OUT, COSALPHA_OUT, B1, B2, P, M, N, cdt, dev) -> dict:
+    """What a call computes from the grids and sizes alone: the grid steps, origins and extents
+    (one reduction, one transfer to the host, which sizes the transform), the oversampling a, b,
+    the window half-widths and the output points in units of the padded grid.  Kept for repeated
+    calls on the same geometry: the same tensor objects, unmodified since (their version
+    counters), on the same stream; such a call neither synchronises nor re-sorts the points.  An
+    entry holds references to its tensors, so their memory is not reused while it stands."""
+    st = _stream(dev)
+    params = (window, float(B1), float(B2), int(P), M, N, cdt, dev, st.value)
+    srcs = (RHO_IN, COSALPHA_IN, RHO_OUT, COSALPHA_OUT)
+    vers = tuple(t._version for t in srcs)
+    for i, e in enumerate(_geoms):
+        if e["params"] == params and e["vers"] == vers and all(p is q for p, q in zip(e["srcs"], srcs)):
+            _geoms.append(_geoms.pop(i))
+            return e
+    f64 = torch.float64
+    lib = _lib.load_library()
     rdt = torch.float64 if cdt == torch.complex128 else torch.float32
     ct = NFFT2D_C128 if cdt == torch.complex128 else NFFT2D_C64
     rax, cax = RHO_IN[0, :], COSALPHA_IN[:, 0]
@@ -196,12 +253,11 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
     if b < 2:
         b = 2
         R2 = _round(b * Mm)
-    data = INPUT_DATA.to(cdt).contiguous()           # M (cos alpha) x N (rho)
-    M, N = data.shape
     if (a * N) % 2 or (b * M) % 2:
         raise ValueError(f"{window}NFFT2D: a*N = {a * N} and b*M = {b * M} must be even")
     if C2 < len2 or R2 < len1:
         raise ValueError(f"{window}NFFT2D: padded output grid {R2} x {C2} smaller than the output")
+    wn = wm = None
     if window == "knab":
         wn, wm = (T1, float(B1), int(P), N), (T2, float(B2), int(P), M)
         kgn, kgm = knab_window(*wn)[1], knab_window(*wm)[1]
@@ -210,29 +266,15 @@ def _interp(window: str, ExtrapVal, RHO_IN, COSALPHA_IN, INPUT_DATA, RHO_OUT, CO
         kgm = int(math.floor((1 / T2 - float(B2) / 2) / (1 / (Mm * T2))))
     if 2 * kgn + 1 > a * N or 2 * kgm + 1 > b * M or kgn < 0 or kgm < 0:
         raise ValueError(f"{window}NFFT2D: the window is wider than the padded spectrum")
-    # the spectrum of the samples (cos alpha x rho); the kernel below reads it transposed
-    S = torch.fft.fft2(data).contiguous()
-    F = torch.empty(a * N, b * M, dtype=cdt, device=dev)
     x1 = torch.empty(R2 * C2, dtype=rdt, device=dev)
     x2 = torch.empty(R2 * C2, dtype=rdt, device=dev)
-    res = torch.empty(len1, len2, dtype=cdt, device=dev)
-    if window == "knab":
-        Gn, Gm = _window_tables(wn, dev), _window_tables(wm, dev)
-        scale = float(N * a) * float(M * b)
-    else:   # G = 1 and the samples divided by N M (rectNFFT2D.m:70)
-        Gn = Gm = None
-        scale = float(a) * float(b)
-    ex = complex(ExtrapVal)
     with torch.cuda.device(dev):
-        # repeated periodically to 2 kg + 1 frequencies, windowed (Selva eq. 25), zero padded to a N x b M
-        # with frequency 0 at (a N / 2, b M / 2) (the library's centred layout) and conjugated
-        check(lib.nfft2d_knab_spectrum(ct, _ptr(S), M, N, kgn, kgm, _ptr(Gn), _ptr(Gm), scale, _ptr(F), a * N, b * M,
-                                       st), "nfft2d_knab_spectrum")
         # points in units of the padded grid, wrapped to one period in FP64 before the cast
         check(lib.nfft2d_knab_points(ct, _ptr(ro), _ptr(co), len1, len2, strides, R2, C2, r0, c0, T1 / a, T2 / b,
                                      float(a * N), float(b * M), _ptr(x1), _ptr(x2), st), "nfft2d_knab_points")
-    y = _plan(a * N, b * M, cdt, dev).setpts_type2(x1, x2, F)
-    with torch.cuda.device(dev):
-        check(lib.nfft2d_knab_finish(ct, _ptr(y), len1, len2, C2, float(a * b * N * M), _ptr(ro), _ptr(co), strides,
-                                     rmin, rmax, cmin, cmax, ex.real, ex.imag, _ptr(res), st), "nfft2d_knab_finish")
-    return res
+    e = dict(params=params, srcs=srcs, vers=vers, ro=ro, co=co, strides=strides, len1=len1, len2=len2, a=a, b=b,
+             C2=C2, wn=wn, wm=wm, kgn=kgn, kgm=kgm, rmin=rmin, rmax=rmax, cmin=cmin, cmax=cmax, x1=x1, x2=x2)
+    while len(_geoms) >= _MAX_GEOMS:
+        _geoms.pop(0)
+    _geoms.append(e)
+    return e

@@ -132,3 +132,34 @@ def test_meshgrid_views_and_fp32_coordinates(fn):
           0.5 / T1, 0.5 / T2)
     assert torch.equal(a, b)
     assert (a == 0.25).any() and (a != 0.25).any()
+
+
+def test_repeated_geometry_reuses_points_and_sees_in_place_changes():
+    """Repeated calls on the same grid tensors reuse the grid statistics, the output points and
+    the plan's tile sort: a second call equals the first bitwise, new samples on the same
+    geometry match the oracle, an in-place change of the output grid is seen (version counter),
+    and a same-shaped output grid in new tensors is computed afresh."""
+    rng = np.random.default_rng(77)
+    nr, nc = 64, 96
+    rho, cs, RHO, COS = grid(nr, nc)
+    T1, T2 = rho[1] - rho[0], cs[1] - cs[0]
+    B1, B2 = 0.5 / T1, 0.5 / T2
+    d1 = (rng.standard_normal((nr, nc)) + 1j * rng.standard_normal((nr, nc))) / np.sqrt(2)
+    d2 = (rng.standard_normal((nr, nc)) + 1j * rng.standard_normal((nr, nc))) / np.sqrt(2)
+    RO, CO = np.meshgrid(np.linspace(rho[2], rho[-3], 120), np.linspace(cs[2], cs[-3], 90))
+    tR, tC, tRO, tCO = T(RHO), T(COS), T(RO), T(CO)
+    print("case knab repeated geometry 64x96 -> 90x120", flush=True)
+    a = m.knabNFFT2D(0.0, tR, tC, T(d1, torch.complex64), tRO, tCO, B1, B2)
+    b = m.knabNFFT2D(0.0, tR, tC, T(d1, torch.complex64), tRO, tCO, B1, B2)
+    assert torch.equal(a, b)
+    c = m.knabNFFT2D(0.0, tR, tC, T(d2, torch.complex64), tRO, tCO, B1, B2).cpu().numpy()
+    assert rel(c, oknab.knab_nfft2d(0.0, RHO, COS, d2, RO, CO, B1, B2)) < TOL
+    # the same tensors changed in place: new points
+    RO2, CO2 = np.meshgrid(np.linspace(rho[5], rho[-8], 120), np.linspace(cs[4], cs[-6], 90))
+    tRO.copy_(T(RO2))
+    tCO.copy_(T(CO2))
+    e = m.knabNFFT2D(0.0, tR, tC, T(d2, torch.complex64), tRO, tCO, B1, B2).cpu().numpy()
+    assert rel(e, oknab.knab_nfft2d(0.0, RHO, COS, d2, RO2, CO2, B1, B2)) < TOL
+    # new tensors of the first grid: the first geometry's result again
+    f = m.knabNFFT2D(0.0, T(RHO), T(COS), T(d1, torch.complex64), T(RO), T(CO), B1, B2)
+    assert rel(f.cpu().numpy(), a.cpu().numpy()) < 1e-6
