@@ -1463,7 +1463,7 @@ struct nfft2d_batch_s {
     std::vector<BatchDesc> hdesc;
     BatchDesc *ddesc = nullptr;
     int2 *own = nullptr;                  // owner tables of the 5 per-transform stages, back to back
-    int off_hist = 0, n_hist = 0, off_col = 0, n_col = 0, off_scat = 0, n_scat = 0;
+    int off_hist = 0, n_hist = 0, off_col = 0, n_col = 0;   // the scatter takes the histogram's blocks
     int off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0, off_merge = 0, n_merge = 0;
     int *base = nullptr;                  // count + 1: subproblem offsets, base[count] = total
     int4 *bsub = nullptr;                 // the batch's subproblem list
@@ -1589,9 +1589,7 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     };
     add_stage(b->off_hist, b->n_hist, [&](int t) { return b->hdesc[t].nblk; });
     add_stage(b->off_col, b->n_col, [&](int t) { return b->hdesc[t].ngroups; });
-    add_stage(b->off_scat, b->n_scat, [&](int t) {
-        return (int)grid1d(std::max(b->hdesc[t].M, b->hdesc[t].nbins), 256);
-    });
+    // (the scatter takes the histogram stage's blocks)
     // the Stockham passes take R rows / B columns per block, the direct ones one
     add_stage(b->off_rows, b->n_rows, [&](int t) {
         const BatchDesc &d = b->hdesc[t];
@@ -1723,7 +1721,10 @@ extern "C" int nfft2d_batch_setpts(nfft2d_batch *b, void *stream)
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_sub_prefix, dim3(1), dim3(1024), 0, s, b->ddesc, b->count, b->base);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_scatter, dim3(b->n_scat), dim3(256), 0, s, b->ddesc, b->own + b->off_scat, b->base, b->bsub);
+    // one scatter block per histogram block (the same owner table)
+    if (b->n_hist > 0)
+        hipLaunchKernelGGL(kb_scatter_lds, dim3(b->n_hist), dim3(kScatThreads), sizeof(int) * b->max_nbins, s,
+                           b->ddesc, b->own + b->off_hist, b->base, b->bsub);
     NFFT2D_LAUNCH_CHECK();
     b->sorted = true;
     return NFFT2D_OK;

@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kHistThreads) void kb_hist(const BatchDesc* __restr
     extern __shared__ int h[];
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
-    hist_body<float>(o.y, h, d.x, d.y, d.M, d.chunk, d.L1, d.L2, d.inv1, d.inv2, d.nb1, d.nbins, d.hist, d.rank);
+    // no per-point rank: kb_scatter_lds ranks the points again in LDS
+    hist_body<float>(o.y, h, d.x, d.y, d.M, d.chunk, d.L1, d.L2, d.inv1, d.inv2, d.nb1, d.nbins, d.hist, nullptr);
 }
 
 __global__ __launch_bounds__(256) void kb_colscan(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
@@ -153,13 +154,36 @@ __global__ __launch_bounds__(1024) void kb_sub_prefix(const BatchDesc* __restric
 // Scatter of every transform's points, and each transform's subproblems into
 // the batch list: entry base[t] + s = (bin, start, count, t), with ~t (< 0) in
 // place of t when count <= the transform's direct-gather cut-off dmax.
-__global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ bd, const int2* __restrict__ own,
-                                                  const int* __restrict__ base, int4* __restrict__ bsub)
+// Block (t, k) takes the points of histogram block k of transform t (the
+// histogram stage's owner table) and, as k_scatter_lds does for a plan,
+// builds pos[bin] = first sorted position of (block k, bin) in LDS from its row
+// of column offsets, then gives each point pos[bin] + its rank by incrementing
+// pos[bin] (one LDS atomic per run of equal bins in a wave).  So kb_hist
+// stores no per-point rank (4 bytes per point written and read back), and no
+// point's position waits on a scattered global read of the offsets.  The
+// transform's bins' subproblem lists are split evenly over its blocks.
+__global__ __launch_bounds__(kScatThreads) void kb_scatter_lds(const BatchDesc* __restrict__ bd,
+                                                               const int2* __restrict__ own,
+                                                               const int* __restrict__ base, int4* __restrict__ bsub)
 {
+    extern __shared__ int pos_lds[];                         // the transform's nbins (at most kMaxLdsBins)
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
-    const int i = o.y * 256 + threadIdx.x;
-    if (i < d.nbins) {
+    const int t = threadIdx.x;
+    const int lo = o.y * d.chunk, hi = min(lo + d.chunk, d.M);
+    // the first batch of points: loads in flight while the positions are built
+    float xv[kScatPer], yv[kScatPer];
+#pragma unroll
+    for (int q = 0; q < kScatPer; ++q) {
+        const int i = lo + q * kScatThreads + t;
+        xv[q] = i < hi ? d.x[i] : 0.f;
+        yv[q] = i < hi ? d.y[i] : 0.f;
+    }
+    const int* __restrict__ orow = d.hist + (size_t)o.y * d.nbins;
+    for (int b = t; b < d.nbins; b += kScatThreads) pos_lds[b] = d.gsum[b / kColBins].x + d.gexcl[b].x + orow[b];
+    const int bpb = (d.nbins + d.nblk - 1) / d.nblk;
+    const int blo = o.y * bpb, bhi = min(blo + bpb, d.nbins);
+    for (int i = blo + t; i < bhi; i += kScatThreads) {
         const int c = d.tot[i];
         const int2 e = d.gexcl[i], gb = d.gsum[i / kColBins];
         const int pos = gb.x + e.x, sp = base[o.x] + gb.y + e.y;
@@ -171,17 +195,34 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchDesc* __restrict__ 
             bsub[sp + k] = make_int4(i, pos + k * d.maxpts, n, n <= d.dmax ? ~o.x : o.x);
         }
     }
-    if (i >= d.M) return;
-    int m1, m2;
-    float t1, t2;
-    const int bin = bin_of(d.x[i], d.y[i], d.L1, d.L2, d.inv1, d.inv2, d.nb1, m1, m2, t1, t2);
-    const int p = d.gsum[bin / kColBins].x + d.gexcl[bin].x + d.hist[(size_t)(i / d.chunk) * d.nbins + bin] + d.rank[i];
-    PtRec<float> rec;
-    rec.t1 = t1;
-    rec.t2 = t2;
-    rec.cell = (m2 << 16) | m1;
-    rec.idx = i;
-    d.pts[p] = rec;
+    __syncthreads();
+    for (int b0 = lo; b0 < hi; b0 += kScatPer * kScatThreads) {
+        if (b0 != lo) {
+#pragma unroll
+            for (int q = 0; q < kScatPer; ++q) {
+                const int i = b0 + q * kScatThreads + t;
+                xv[q] = i < hi ? d.x[i] : 0.f;
+                yv[q] = i < hi ? d.y[i] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kScatPer; ++q) {
+            const int i = b0 + q * kScatThreads + t;
+            const bool act = i < hi;
+            int m1, m2;
+            float t1, t2;
+            const int bin = bin_of(xv[q], yv[q], d.L1, d.L2, d.inv1, d.inv2, d.nb1, m1, m2, t1, t2);
+            const int p = wave_run_rank(pos_lds, bin, act);
+            if (act) {
+                PtRec<float> rec;
+                rec.t1 = t1;
+                rec.t2 = t2;
+                rec.cell = (m2 << 16) | m1;
+                rec.idx = i;
+                d.pts[p] = rec;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- grid passes
