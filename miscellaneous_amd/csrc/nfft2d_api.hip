@@ -1465,6 +1465,9 @@ struct nfft2d_batch_s {
     int2 *own = nullptr;                  // owner tables of the 5 per-transform stages, back to back
     int off_hist = 0, n_hist = 0, off_col = 0, n_col = 0;   // the scatter takes the histogram's blocks
     int off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0, off_merge = 0, n_merge = 0;
+    // type-1 passes: the transforms whose lines go to the matrix cores (t1_pass_on_mfma)
+    // in their own stages, the others in stages of the type-2 passes' shape
+    int off_t1c = 0, n_t1c = 0, off_t1cm = 0, n_t1cm = 0, off_t1r = 0, n_t1r = 0, off_t1rm = 0, n_t1rm = 0;
     int *base = nullptr;                  // count + 1: subproblem offsets, base[count] = total
     int4 *bsub = nullptr;                 // the batch's subproblem list
     long long bsubcap = 0;
@@ -1598,6 +1601,23 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
     add_stage(b->off_cols, b->n_cols, [&](int t) {
         const BatchDesc &d = b->hdesc[t];
         return batch_fft_n(d.N1) ? (int)grid1d(d.L2, batch_cols_per_block(d.N1)) : d.L2;
+    });
+    // type-1: 16-column tiles (column pass) and 16-row tiles (row pass) on the matrix cores
+    add_stage(b->off_t1c, b->n_t1c, [&](int t) {
+        const BatchDesc &d = b->hdesc[t];
+        return t1_pass_on_mfma(d.N1) ? 0 : batch_fft_n(d.N1) ? (int)grid1d(d.L2, batch_cols_per_block(d.N1)) : d.L2;
+    });
+    add_stage(b->off_t1cm, b->n_t1cm, [&](int t) {
+        const BatchDesc &d = b->hdesc[t];
+        return t1_pass_on_mfma(d.N1) ? (int)grid1d(d.L2, 16) : 0;
+    });
+    add_stage(b->off_t1r, b->n_t1r, [&](int t) {
+        const BatchDesc &d = b->hdesc[t];
+        return t1_pass_on_mfma(d.N2) ? 0 : batch_fft_n(d.N2) ? (int)grid1d(d.N1, batch_rows_per_block(d.N2)) : d.N1;
+    });
+    add_stage(b->off_t1rm, b->n_t1rm, [&](int t) {
+        const BatchDesc &d = b->hdesc[t];
+        return t1_pass_on_mfma(d.N2) ? (int)grid1d(d.N1, 16) : 0;
     });
     add_stage(b->off_merge, b->n_merge, [&](int t) { return b->hdesc[t].nbins; });
     b->bsubcap = std::max<long long>(subcap, 1);
@@ -1814,9 +1834,18 @@ extern "C" int nfft2d_batch_type1(nfft2d_batch *b, void *stream)
     NFFT2D_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_spread_merge, dim3(b->n_merge), dim3(256), 0, s, b->ddesc, b->own + b->off_merge);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_t1_cols, dim3(b->n_cols), dim3(256), 0, s, b->ddesc, b->own + b->off_cols);
+    // column passes (FFT / chirp-z / direct lines, and the matrix-core tiles), then row passes
+    if (b->n_t1c > 0)
+        hipLaunchKernelGGL(kb_t1_cols, dim3(b->n_t1c), dim3(256), 0, s, b->ddesc, b->own + b->off_t1c);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_t1_rows, dim3(b->n_rows), dim3(128), 0, s, b->ddesc, b->own + b->off_rows);
+    if (b->n_t1cm > 0)
+        hipLaunchKernelGGL(kb_t1_cols_mm, dim3(b->n_t1cm), dim3(256), 0, s, b->ddesc, b->own + b->off_t1cm);
+    NFFT2D_LAUNCH_CHECK();
+    if (b->n_t1r > 0)
+        hipLaunchKernelGGL(kb_t1_rows, dim3(b->n_t1r), dim3(128), 0, s, b->ddesc, b->own + b->off_t1r);
+    NFFT2D_LAUNCH_CHECK();
+    if (b->n_t1rm > 0)
+        hipLaunchKernelGGL(kb_t1_rows_mm, dim3(b->n_t1rm), dim3(256), 0, s, b->ddesc, b->own + b->off_t1rm);
     NFFT2D_LAUNCH_CHECK();
     return NFFT2D_OK;
 }

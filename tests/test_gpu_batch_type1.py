@@ -35,7 +35,7 @@ def pts(rng, N1, N2, M):
 
 
 SIZES1 = [(88, 88, 1936), (64, 64, 5000), (26, 34, 700), (128, 96, 20000), (22, 22, 1), (256, 256, 0),
-          (100, 64, 3000), (512, 512, 60000)]
+          (100, 64, 3000), (512, 512, 60000), (24, 30, 500), (126, 30, 4000)]
 
 
 def test_batch_type1_mixed_sizes():
