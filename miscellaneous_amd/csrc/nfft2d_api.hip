@@ -1619,7 +1619,11 @@ extern "C" int nfft2d_batch_create(nfft2d_batch **out, int count, const int *N1,
         const BatchDesc &d = b->hdesc[t];
         return t1_pass_on_mfma(d.N2) ? (int)grid1d(d.N1, 16) : 0;
     });
-    add_stage(b->off_merge, b->n_merge, [&](int t) { return b->hdesc[t].nbins; });
+    // the tile merge, except where the matrix-core column pass sums the tiles itself
+    add_stage(b->off_merge, b->n_merge, [&](int t) {
+        const BatchDesc &d = b->hdesc[t];
+        return t1_cols_merge_tiles(d.N1, d.r1, d.r2) ? 0 : d.nbins;
+    });
     b->bsubcap = std::max<long long>(subcap, 1);
     if (nfft2d_malloc(&b->ddesc, sizeof(BatchDesc) * count) != hipSuccess ||
         nfft2d_malloc(&b->own, sizeof(int2) * own.size()) != hipSuccess ||
@@ -1832,7 +1836,8 @@ extern "C" int nfft2d_batch_type1(nfft2d_batch *b, void *stream)
     hipLaunchKernelGGL(kb_spread, dim3((unsigned)b->bsubcap), dim3(256), 0, s, b->ddesc, b->bsub, b->base, b->count,
                        b->plans[0]->wa);
     NFFT2D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_spread_merge, dim3(b->n_merge), dim3(256), 0, s, b->ddesc, b->own + b->off_merge);
+    if (b->n_merge > 0)
+        hipLaunchKernelGGL(kb_spread_merge, dim3(b->n_merge), dim3(256), 0, s, b->ddesc, b->own + b->off_merge);
     NFFT2D_LAUNCH_CHECK();
     // column passes (FFT / chirp-z / direct lines, and the matrix-core tiles), then row passes
     if (b->n_t1c > 0)

@@ -369,6 +369,12 @@ __global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ 
 // batch is held to its plan's result within rounding, not to its bits.
 constexpr int kT1MmMaxN = 126;
 __host__ __device__ inline bool t1_pass_on_mfma(int N) { return !batch_fft_n(N) && N <= kT1MmMaxN; }
+// The matrix-core column pass sums the spread tiles itself (and the tile merge
+// skips the transform) when the merge needs the 3 x 3 neighbours only.
+__host__ __device__ inline bool t1_cols_merge_tiles(int N1, int r1, int r2)
+{
+    return t1_pass_on_mfma(N1) && r1 == 1 && r2 == 1;
+}
 typedef float mm4v __attribute__((ext_vector_type(4)));
 constexpr int kT1MmLd = 2 * kT1MmMaxN + 1;                 // padded line of the LDS panel (odd)
 constexpr int kT1MmLds = 16 * kT1MmLd + 2 * kT1MmMaxN;     // panel + twiddles, float2 entries
@@ -389,17 +395,35 @@ __device__ __forceinline__ void mm_tile(int L, AF&& a_of, BF&& b_of, mm4v& rr, m
 
 // Column pass, column tile o.y of transform o.x: Tm[i1][n2] for the 16
 // columns n2 of the tile, every row tile of i1 (wave w takes w, w + 4, ...).
+// The 16 lines are one bin row of the grid: each of their cells is summed
+// from the spread tiles that cover it (merge_cell, the tile merge's sum in
+// its order), so these transforms skip kb_spread_merge and the grid's write
+// and re-read.
+template <int R1, int R2>
+__device__ __forceinline__ void mm_stage_merged(const BatchDesc& d, int ct, float2* panel, int L1, int L2)
+{
+    const int nb2 = d.nbins / d.geo.nb1;
+    for (int e = threadIdx.x; e < 16 * L1; e += blockDim.x) {
+        const int j = e / L1, n1 = e - j * L1, n2 = 16 * ct + j;
+        panel[j * kT1MmLd + n1] =
+            n2 < L2 ? merge_cell<float, R1, R2>(n2, n1, d.scratch, d.subbeg, d.geo, nb2) : make_float2(0.f, 0.f);
+    }
+}
 __global__ __launch_bounds__(256) void kb_t1_cols_mm(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
 {
     __shared__ float2 lds[kT1MmLds];
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
     const int N1 = d.N1, L1 = 2 * N1, L2 = d.L2;
-    float2* panel = lds;                                     // [16][kT1MmLd]: rows n2 of Gt
+    float2* panel = lds;                                     // [16][kT1MmLd]: rows n2 of the grid
     float2* wt = lds + 16 * kT1MmLd;                         // L1 twiddles
-    for (int e = threadIdx.x; e < 16 * L1; e += blockDim.x) {
-        const int j = e / L1, n1 = e - j * L1, n2 = 16 * o.y + j;
-        panel[j * kT1MmLd + n1] = n2 < L2 ? d.Gt[(size_t)n2 * L1 + n1] : make_float2(0.f, 0.f);
+    if (t1_cols_merge_tiles(d.N1, d.r1, d.r2)) {
+        mm_stage_merged<1, 1>(d, o.y, panel, L1, L2);
+    } else {   // grids with a thin last bin (5 x 5 neighbours): the merged grid from kb_spread_merge
+        for (int e = threadIdx.x; e < 16 * L1; e += blockDim.x) {
+            const int j = e / L1, n1 = e - j * L1, n2 = 16 * o.y + j;
+            panel[j * kT1MmLd + n1] = n2 < L2 ? d.Gt[(size_t)n2 * L1 + n1] : make_float2(0.f, 0.f);
+        }
     }
     for (int k = threadIdx.x; k < L1; k += blockDim.x) wt[k] = d.tw1[k];
     __syncthreads();

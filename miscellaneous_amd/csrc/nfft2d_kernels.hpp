@@ -1895,7 +1895,8 @@ __global__ __launch_bounds__(256) void k_spread_halo64(const PtRec<double>* __re
 // Grid cell (n2, n1) = sum over the neighbouring bins b' (fixed order) and
 // their tiles (every kSpUnit-th subproblem, in order) of
 // tile_s[(n2 - O2' + K) mod L2][(n1 - O1' + K) mod L1] where that index lies
-// in [0, 28).  One block per bin writes the bin's cells.
+// in [0, 28) (merge_cell).  k_spread_merge writes every cell to the grid; a
+// batch's matrix-core column pass sums the cells of its lines itself.
 // The neighbours are the bins at offsets -R..R per dimension.  R = 1 covers
 // every tile that reaches a bin when each bin is at least K cells wide; when
 // the last bin of a dimension is thinner than K (L mod 16 = 2 or 4), the
@@ -1905,16 +1906,13 @@ __global__ __launch_bounds__(256) void k_spread_halo64(const PtRec<double>* __re
 __host__ __device__ constexpr int merge_radius(int L) { return (L % kBin != 0 && L % kBin < kK) ? 2 : 1; }
 
 template <typename T, int R1, int R2>
-__device__ __forceinline__ void spread_merge_body(int bin, const cplx<T>* __restrict__ scratch,
-                                                  const int* __restrict__ subbeg, const BinGeom& geo, int nb2,
-                                                  cplx<T>* __restrict__ Gt)
+__device__ __forceinline__ cplx<T> merge_cell(int n2, int n1, const cplx<T>* __restrict__ scratch,
+                                              const int* __restrict__ subbeg, const BinGeom& geo, int nb2)
 {
     using C = cplx<T>;
     constexpr int W1 = 2 * R1 + 1, W2 = 2 * R2 + 1, NN = W1 * W2;
     const int nb1 = geo.nb1;
-    const int b1 = bin % nb1, b2 = bin / nb1;
-    const int n2 = b2 * kBin + threadIdx.x / kBin, n1 = b1 * kBin + threadIdx.x % kBin;
-    if (n2 >= geo.L2 || n1 >= geo.L1) return;
+    const int b1 = n1 / kBin, b2 = n2 / kBin;
     const int s1 = W1 > nb1 ? 0 : b1 - R1 + nb1, c1 = W1 > nb1 ? nb1 : W1;   // first offset, count
     const int s2 = W2 > nb2 ? 0 : b2 - R2 + nb2, c2 = W2 > nb2 ? nb2 : W2;
     int beg[NN], end[NN], off[NN];
@@ -1954,7 +1952,19 @@ __device__ __forceinline__ void spread_merge_body(int bin, const cplx<T>* __rest
             si += v.y;
         }
     }
-    Gt[(size_t)n2 * geo.L1 + n1] = mk(sr, si);
+    return mk(sr, si);
+}
+// One block per bin writes the bin's cells.
+template <typename T, int R1, int R2>
+__device__ __forceinline__ void spread_merge_body(int bin, const cplx<T>* __restrict__ scratch,
+                                                  const int* __restrict__ subbeg, const BinGeom& geo, int nb2,
+                                                  cplx<T>* __restrict__ Gt)
+{
+    const int nb1 = geo.nb1;
+    const int b1 = bin % nb1, b2 = bin / nb1;
+    const int n2 = b2 * kBin + threadIdx.x / kBin, n1 = b1 * kBin + threadIdx.x % kBin;
+    if (n2 >= geo.L2 || n1 >= geo.L1) return;
+    Gt[(size_t)n2 * geo.L1 + n1] = merge_cell<T, R1, R2>(n2, n1, scratch, subbeg, geo, nb2);
 }
 template <typename T, int R1, int R2>
 __global__ __launch_bounds__(256) void k_spread_merge(const cplx<T>* __restrict__ scratch,
