@@ -1642,6 +1642,12 @@ __device__ __forceinline__ void spread_halo_body(int s, int4 d, const PtRec<floa
     if (k % kSpUnit != 0) return;
     const int cnt = min(kSpUnit * maxpts, tot[d.x] - k * maxpts);   // points of the unit from d.y on
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // a unit of at most 64 points is wave 0's alone: the other waves leave
+    // before any block barrier, and wave 0 stores its tile from its
+    // accumulators (the 4-wave sum would add their zero tiles to it); sparse
+    // point sets have ~16 points per bin
+    const bool solo = cnt <= 64;
+    if (solo && w != 0) return;
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
     SpreadWaveLds& L = sm.w[w];
     {
@@ -1738,6 +1744,15 @@ __device__ __forceinline__ void spread_halo_body(int s, int4 d, const PtRec<floa
             wave_sync_lds();
         }
     }
+    float2* out = scratch + (size_t)s * kTileCells;
+    if (solo) {   // wave 0: unscale and store the 28x28 used part
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int ti = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), tj = lane & 31;
+            if (ti < kHalo && tj < kHalo) out[ti * kHalo + tj] = make_float2(acc_re[r] * inv_sc, acc_im[r] * inv_sc);
+        }
+        return;
+    }
     // unscale, reduce the 4 waves' tiles in a fixed order, store the 28x28 used part
     __syncthreads();
 #pragma unroll
@@ -1747,7 +1762,6 @@ __device__ __forceinline__ void spread_halo_body(int s, int4 d, const PtRec<floa
         sm.red[(w * 1024 + ti * 32 + tj) * 2 + 1] = acc_im[r] * inv_sc;
     }
     __syncthreads();
-    float2* out = scratch + (size_t)s * kTileCells;
     for (int e = threadIdx.x; e < kTileCells; e += 256) {
         const int ti = e / kHalo, tj = e - ti * kHalo;
         const int c = ti * 32 + tj;
