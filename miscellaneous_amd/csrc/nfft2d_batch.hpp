@@ -517,14 +517,14 @@ constexpr int kDirectDefault = 8;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NFFT2D_BATCH_GATHER_WAVES, 8))) void
 kb_gather_w(const BatchDesc* __restrict__ bd, const int4* __restrict__ bsub, const int* __restrict__ total, WinArgs wa)
 {
-    __shared__ WavePairLds lds[4];
+    __shared__ WaveTileLds lds[4];
     const int w = threadIdx.x >> 6;
     const int n = *total;
     for (int s = blockIdx.x * 4 + w; s < n; s += gridDim.x * 4) {
         const int4 d = bsub[s];
         if (d.w < 0) continue;                           // kb_gather_direct's
         const BatchDesc& t = bd[d.w];
-        gather_wave_pairs(d, lds[w], t.Gt, t.geo, t.pts, wa, t.out);
+        gather_wave_single(d, lds[w], t.Gt, t.geo, t.pts, wa, t.out);
     }
 }
 

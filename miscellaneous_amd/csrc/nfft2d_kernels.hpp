@@ -1421,33 +1421,75 @@ __device__ __forceinline__ void wave_sync_lds()
 }
 
 // (c) type-2 gather, one WAVE per subproblem (complex64, MODE 0), for the
-// persistent batch gather.  The wave stages the subproblem's 28 x 28 tile in
-// its own LDS tile (13 loads per lane, all in flight), then takes its points
-// 64 at a time: it counting-sorts them by cell pair in LDS (wave-level: LDS
-// atomics, a wave scan, no block barrier) and runs pairs_mac, as the block
-// form does.  Many subproblems are in flight per CU (four per block), which
-// pays off when they are small (sparse point sets).  Same arithmetic per point
-// as k_gather_pairs, so the same bits.
-struct __attribute__((aligned(16))) WavePairLds {
-    float2 tile[kTileW * kTileLd];            // 16-byte aligned: the window reads are 16-byte loads
-    int cnt[kPairs];
-    float t1[2][64], t2[2][64];
-    int cell[2][64], idx[2][64];
-};
+// persistent batch gather.  Many subproblems are in flight per CU (four per
+// block), which pays off when they are small (sparse point sets).  Same
+// arithmetic per point as k_gather_pairs, so the same bits.
 
-__device__ __forceinline__ void gather_wave_pairs(const int4& d, WavePairLds& L, const float2* __restrict__ Gt,
-                                                  const BinGeom& geo, const PtRec<float>* __restrict__ pts,
-                                                  const WinArgs& wa, float2* __restrict__ out)
+// One point's 13 x 13 sum from a bin's LDS tile (row stride kTileLd): the
+// fmaf sequence of pairs_mac for a lane's first point (its 13 column weights
+// in a row of 14 with a zero at the end its window does not reach, then the
+// 13 rows), so the same bits.
+__device__ __forceinline__ float2 tile_point(const float2* tile, const PtRec<float>& rec, int O1, int O2,
+                                             const WinArgs& wa)
+{
+    constexpr int W = kTileLd;
+    const int cell = min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin + min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
+    const int r0 = cell / kBin, c0 = (cell % kBin) & ~1;
+    const bool odd = (cell & 1) != 0;
+    float w1[kW], w2[kW];
+    taps(rec.t1, wa, w1);
+    taps(rec.t2, wa, w2);
+    float e[kW + 1];
+#pragma unroll
+    for (int j = 0; j <= kW; ++j) {
+        const float a0 = j < kW ? w1[j] : 0.f, a1 = j > 0 ? w1[j - 1] : 0.f;
+        e[j] = odd ? a1 : a0;
+    }
+    const float2* base = tile + r0 * W + c0;
+    float x = 0.f, y = 0.f;
+#pragma unroll
+    for (int b = 0; b < kW; ++b) {
+        float sx = 0.f, sy = 0.f;
+#pragma unroll
+        for (int j = 0; j <= kW; ++j) {
+            const float2 u = base[b * W + j];
+            sx = fmaf(e[j], u.x, sx);
+            sy = fmaf(e[j], u.y, sy);
+        }
+        x = fmaf(w2[b], sx, x);
+        y = fmaf(w2[b], sy, y);
+    }
+    return make_float2(x, y);
+}
+
+// The persistent batch gather's subproblem, one point per lane: the wave
+// stages the 28 x 28 tile in its own LDS tile (13 loads per lane, all in
+// flight), then each lane sums its own point from it, 64 points per pass.  A
+// pass of the pairs_mac form (a wave-level counting sort by cell pair, two
+// points per lane) also covered at most 64 points and issued the same window
+// reads per lane, so pairing bought no pass and no read here; on sparse sets
+// (mostly one point per cell pair) it ran two points' FMAs per lane for one
+// point: config 3 took 1.78 ms per step with it, 1.60 without.
+struct __attribute__((aligned(16))) WaveTileLds {
+    float2 tile[kTileW * kTileLd];
+};
+__device__ __forceinline__ void gather_wave_single(const int4& d, WaveTileLds& L, const float2* __restrict__ Gt,
+                                                   const BinGeom& geo, const PtRec<float>* __restrict__ pts,
+                                                   const WinArgs& wa, float2* __restrict__ out)
 {
     constexpr int W = kTileW;
     constexpr int kPer = (W * W + 63) / 64;                  // 13
     const int lane = threadIdx.x & 63;
     const int O1 = (d.x % geo.nb1) * kBin, O2 = (d.x / geo.nb1) * kBin;
-    // point records of the first chunk and the tile, all loads in flight
-    PtRec<float> rec;
-    rec.t1 = rec.t2 = 0.f;
-    rec.cell = rec.idx = 0;
-    if (lane < d.z) rec = pts[d.y + lane];
+    auto fetch = [&](int k) {
+        PtRec<float> r;
+        r.t1 = r.t2 = 0.f;
+        r.cell = r.idx = 0;
+        if (k < d.z) r = pts[d.y + k];
+        return r;
+    };
+    // the first pass's records and the tile, all loads in flight
+    PtRec<float> rec = fetch(lane);
     float2 v[kPer];
     const bool big = geo.L1 >= W && geo.L2 >= W;
 #pragma unroll
@@ -1470,48 +1512,13 @@ __device__ __forceinline__ void gather_wave_pairs(const int4& d, WavePairLds& L,
             L.tile[r * kTileLd + c] = v[q];
         }
     }
+    wave_sync_lds();
     for (int base = 0; base < d.z; base += 64) {
-        const bool have = base + lane < d.z;
-        PtRec<float> nxt;
-        nxt.t1 = nxt.t2 = 0.f;
-        nxt.cell = nxt.idx = 0;
-        if (base + 64 + lane < d.z) nxt = pts[d.y + base + 64 + lane];   // next chunk, in flight during this one
-        const int cell = have ? min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin +
-                                    min(max((rec.cell & 0xffff) - O1, 0), kBin - 1)
-                              : 0;
-        const int pr = cell >> 1;
-        L.cnt[lane] = 0;
-        L.cnt[lane + 64] = 0;
-        L.idx[1][lane] = -1;
-        wave_sync_lds();
-        const int rk = have ? atomicAdd(&L.cnt[pr], 1) : 0;
-        wave_sync_lds();
-        // lanes per cell pair (ceil(n / 2)), exclusive prefix over the 128 pairs (2 per lane)
-        const int l0 = (L.cnt[2 * lane] + 1) >> 1, l1 = (L.cnt[2 * lane + 1] + 1) >> 1;
-        int inc = l0 + l1;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += t;
-        }
-        const int nl = __shfl(inc, 63, 64);
-        const int ex = inc - l0 - l1;
-        wave_sync_lds();
-        L.cnt[2 * lane] = ex;
-        L.cnt[2 * lane + 1] = ex + l0;
-        wave_sync_lds();
-        if (have) {
-            const int l = L.cnt[pr] + (rk >> 1), k = rk & 1;
-            L.t1[k][l] = rec.t1;
-            L.t2[k][l] = rec.t2;
-            L.cell[k][l] = cell;
-            L.idx[k][l] = rec.idx;
-        }
-        wave_sync_lds();
-        if (lane < nl) pairs_mac(L, lane, wa, out);
-        wave_sync_lds();                                     // slots reused by the next chunk
+        const PtRec<float> nxt = fetch(base + 64 + lane);    // the next pass's, in flight during this one
+        if (base + lane < d.z) out[rec.idx] = tile_point(L.tile, rec, O1, O2, wa);
         rec = nxt;
     }
+    wave_sync_lds();                                         // the tile is reused by the next subproblem
 }
 
 // (d) type-1 spread (interim form): each block owns one subproblem, accumulates
