@@ -974,10 +974,13 @@ __device__ __forceinline__ double inv2(const BinGeom& g, double) { return g.inv2
 // o1, o2 are in [-K, L); when L >= W one adjustment by L wraps every index.
 __device__ __forceinline__ int wrap1(int g, int L) { return g < 0 ? g + L : (g >= L ? g - L : g); }
 
+// LDS row stride of a gather tile (complex elements).  40: consecutive rows
+// start 16 banks apart (with 32 every row starts at the same bank); polar
+// 512^2 gather 31.6 -> 30.7 us (36: 30.5, 44: 30.3).
 #ifndef NFFT2D_TILE_LD
-#define NFFT2D_TILE_LD 32
+#define NFFT2D_TILE_LD 40
 #endif
-constexpr int kTileLd = NFFT2D_TILE_LD;     // LDS row stride of a gather tile (complex elements)
+constexpr int kTileLd = NFFT2D_TILE_LD;
 
 // Launched with 256 threads; the 784 loads of a thread group are issued
 // together (a thread has at most 4): tile_fetch into registers, tile_store
@@ -1425,14 +1428,14 @@ __device__ __forceinline__ void wave_sync_lds()
 // block), which pays off when they are small (sparse point sets).  Same
 // arithmetic per point as k_gather_pairs, so the same bits.
 
-// One point's 13 x 13 sum from a bin's LDS tile (row stride kTileLd): the
-// fmaf sequence of pairs_mac for a lane's first point (its 13 column weights
-// in a row of 14 with a zero at the end its window does not reach, then the
-// 13 rows), so the same bits.
+// One point's 13 x 13 sum from a bin's LDS tile (row stride W): the fmaf
+// sequence of pairs_mac for a lane's first point (its 13 column weights in a
+// row of 14 with a zero at the end its window does not reach, then the 13
+// rows), so the same bits.
+template <int W>
 __device__ __forceinline__ float2 tile_point(const float2* tile, const PtRec<float>& rec, int O1, int O2,
                                              const WinArgs& wa)
 {
-    constexpr int W = kTileLd;
     const int cell = min(max((rec.cell >> 16) - O2, 0), kBin - 1) * kBin + min(max((rec.cell & 0xffff) - O1, 0), kBin - 1);
     const int r0 = cell / kBin, c0 = (cell % kBin) & ~1;
     const bool odd = (cell & 1) != 0;
@@ -1470,8 +1473,17 @@ __device__ __forceinline__ float2 tile_point(const float2* tile, const PtRec<flo
 // reads per lane, so pairing bought no pass and no read here; on sparse sets
 // (mostly one point per cell pair) it ran two points' FMAs per lane for one
 // point: config 3 took 1.78 ms per step with it, 1.60 without.
+// Its tile rows are kWaveTileLd complex values apart: with 32 every row
+// starts at the same LDS bank, so the windows of a 16-lane read group (one
+// point per lane, at scattered cells) fell on at most 8 distinct 16-byte bank
+// groups.  36: each row starts 8 banks (of 64) further; config 3 1.60 -> 1.44 ms per step
+// (40 and 44 the same, 48 1.75 ms: the larger tiles cost a block per CU).
+#ifndef NFFT2D_WAVE_TILE_LD
+#define NFFT2D_WAVE_TILE_LD 36
+#endif
+constexpr int kWaveTileLd = NFFT2D_WAVE_TILE_LD;
 struct __attribute__((aligned(16))) WaveTileLds {
-    float2 tile[kTileW * kTileLd];
+    float2 tile[kTileW * kWaveTileLd];
 };
 __device__ __forceinline__ void gather_wave_single(const int4& d, WaveTileLds& L, const float2* __restrict__ Gt,
                                                    const BinGeom& geo, const PtRec<float>* __restrict__ pts,
@@ -1509,13 +1521,13 @@ __device__ __forceinline__ void gather_wave_single(const int4& d, WaveTileLds& L
         const int e = lane + 64 * q;
         if (e < W * W) {
             const int r = e / W, c = e - r * W;
-            L.tile[r * kTileLd + c] = v[q];
+            L.tile[r * kWaveTileLd + c] = v[q];
         }
     }
     wave_sync_lds();
     for (int base = 0; base < d.z; base += 64) {
         const PtRec<float> nxt = fetch(base + 64 + lane);    // the next pass's, in flight during this one
-        if (base + lane < d.z) out[rec.idx] = tile_point(L.tile, rec, O1, O2, wa);
+        if (base + lane < d.z) out[rec.idx] = tile_point<kWaveTileLd>(L.tile, rec, O1, O2, wa);
         rec = nxt;
     }
     wave_sync_lds();                                         // the tile is reused by the next subproblem
