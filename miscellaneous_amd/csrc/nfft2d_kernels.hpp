@@ -1472,11 +1472,12 @@ __device__ __forceinline__ float2 tile_point(const float2* tile, const PtRec<flo
 // points per lane) also covered at most 64 points and issued the same window
 // reads per lane, so pairing bought no pass and no read here; on sparse sets
 // (mostly one point per cell pair) it ran two points' FMAs per lane for one
-// point: config 3 took 1.78 ms per step with it, 1.60 without.
+// point: a batch of 256 uneven transforms (16M points) took 1.78 ms per step
+// with it, 1.60 without.
 // Its tile rows are kWaveTileLd complex values apart: with 32 every row
 // starts at the same LDS bank, so the windows of a 16-lane read group (one
 // point per lane, at scattered cells) fell on at most 8 distinct 16-byte bank
-// groups.  36: each row starts 8 banks (of 64) further; config 3 1.60 -> 1.44 ms per step
+// groups.  36: each row starts 8 banks (of 64) further; that batch 1.60 -> 1.44 ms per step
 // (40 and 44 the same, 48 1.75 ms: the larger tiles cost a block per CU).
 #ifndef NFFT2D_WAVE_TILE_LD
 #define NFFT2D_WAVE_TILE_LD 36
