@@ -725,6 +725,14 @@ static int t2_grid(nfft2d_plan *p, const void *data, hipStream_t s)
     return NFFT2D_OK;
 }
 
+// complex64 lines of at most kT1MmMaxN modes that take neither the
+// fixed-size FFT nor the chirp-z passes run on the matrix cores.
+template <typename T>
+static bool t1_plan_mfma(int N)
+{
+    return std::is_same<T, float>::value && N <= kT1MmMaxN && (N & 1) == 0;
+}
+
 // Type-1 passes: transposed grid -> crop + deconvolve -> result (N1 x N2).
 template <typename T, int SIGN>
 static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
@@ -746,6 +754,9 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
         int rc = czt_launch<T, SIGN, false>(czt_tables<T>(p, 0, false), p->L2, G, p->L1, 1, Tp, 1, p->L2,
                                             (const T *)nullptr, (const T *)nullptr, s);
         if (rc) return rc;
+    } else if (t1_plan_mfma<T>(p->N1)) {
+        hipLaunchKernelGGL((k_t1_cols_mm<SIGN>), dim3(grid1d(p->L2, 16), grid1d(p->N1, 16)), dim3(256), 0, s, (const float2 *)G,
+                           (float2 *)Tp, (const float2 *)tw1, p->N1, p->L2);
     } else {
         hipLaunchKernelGGL((k_t1_cols_direct<T, SIGN>), dim3(p->L2), dim3(256), sizeof(C) * 2 * p->L1, s, G, Tp, tw1, p->N1,
                            p->L2);
@@ -763,6 +774,9 @@ static int t1_passes(nfft2d_plan *p, void *result, hipStream_t s)
     } else if (p->czt[1]) {   // rows: Tp[k1][*] (L2) -> F[k1][*] (N2), deconvolved
         int rc = czt_launch<T, SIGN, false>(czt_tables<T>(p, 1, false), p->N1, Tp, p->L2, 1, F, p->N2, 1, D1, D2, s);
         if (rc) return rc;
+    } else if (t1_plan_mfma<T>(p->N2)) {
+        hipLaunchKernelGGL((k_t1_rows_mm<SIGN>), dim3(grid1d(p->N1, 16), grid1d(p->N2, 16)), dim3(256), 0, s, (const float2 *)Tp,
+                           (float2 *)F, (const float *)D1, (const float *)D2, (const float2 *)tw2, p->N1, p->N2);
     } else {
         hipLaunchKernelGGL((k_t1_rows_direct<T, SIGN>), dim3(p->N1), dim3(256), sizeof(C) * 2 * p->L2, s, Tp, F, D1, D2,
                            tw2, p->N1, p->N2);

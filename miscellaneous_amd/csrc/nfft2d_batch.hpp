@@ -352,22 +352,12 @@ __global__ __launch_bounds__(128) void kb_t1_rows(const BatchDesc* __restrict__ 
 }
 
 // ---------------------------------------------------------------- type-1 direct passes on MFMA
-// The type-1 passes of short lines that are not powers of two (up to
-// kT1MmMaxN modes, below the chirp-z threshold: e.g. 88 in Synthesis/PSO) as
-// complex matrix products on the FP32 matrix cores (v_mfma_f32_16x16x4f32:
-// A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, accumulator element
-// r of lane l at row 4 (l / 16) + r, column l % 16).  Column pass of a
-// transform: Tm (N1 x L2) = W1 (N1 x L1) . Gt^T, W1[i1][n1] = w((k1 n1) mod
-// L1), k1 = i1 - N1/2; row pass: F1 (N1 x N2) = D1 D2 (Tm (N1 x L2) . W2^T),
-// W2[i2][n2] = w((k2 n2) mod L2).  Each takes 16 x 16 output tiles: the 16
-// lines of the tile are staged in LDS (rows padded to an odd length), the
-// twiddle operand is read from the LDS twiddle table at an index advanced by
-// 4 k per K-step of 4, and four real products (Re Re, Im Im, Re Im, Im Re)
-// accumulate per K-step.  The direct passes on the VALU read a line value and
-// a twiddle from LDS for every 4 FMAs and ran at ~0.19 of the VALU peak.
-// Sums run in another order than the plans' direct passes: a type-1 in a
-// batch is held to its plan's result within rounding, not to its bits.
-constexpr int kT1MmMaxN = 126;
+// The matrix-core passes (t1_cols_mm_compute, t1_rows_mm_compute) over a
+// batch: one block per 16-line tile of a transform, listed in the owner
+// tables.  The direct passes on the VALU read a line value and a twiddle from
+// LDS for every 4 FMAs and ran at ~0.19 of the VALU peak.  Sums run in another
+// order than the VALU direct passes: a type-1 in a batch is held to its
+// plan's result within rounding, not to its bits.
 __host__ __device__ inline bool t1_pass_on_mfma(int N) { return !batch_fft_n(N) && N <= kT1MmMaxN; }
 // The matrix-core column pass sums the spread tiles itself (and the tile merge
 // skips the transform) when the merge needs the 3 x 3 neighbours only.
@@ -375,30 +365,13 @@ __host__ __device__ inline bool t1_cols_merge_tiles(int N1, int r1, int r2)
 {
     return t1_pass_on_mfma(N1) && r1 == 1 && r2 == 1;
 }
-typedef float mm4v __attribute__((ext_vector_type(4)));
-constexpr int kT1MmLd = 2 * kT1MmMaxN + 1;                 // padded line of the LDS panel (odd)
-constexpr int kT1MmLds = 16 * kT1MmLd + 2 * kT1MmMaxN;     // panel + twiddles, float2 entries
-
-// 16 x 16 complex tile, K-steps of 4 over K = L (a multiple of 4): lane (i, kk)
-// = (l % 16, l / 16).  Left operand a(k0) and right operand b(k0) of this lane.
-template <typename AF, typename BF>
-__device__ __forceinline__ void mm_tile(int L, AF&& a_of, BF&& b_of, mm4v& rr, mm4v& ii, mm4v& ri, mm4v& ir)
-{
-    for (int k0 = 0; k0 < L; k0 += 4) {
-        const float2 a = a_of(k0), b = b_of(k0);
-        rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, rr, 0, 0, 0);
-        ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, ii, 0, 0, 0);
-        ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, ri, 0, 0, 0);
-        ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, ir, 0, 0, 0);
-    }
-}
 
 // Column pass, column tile o.y of transform o.x: Tm[i1][n2] for the 16
-// columns n2 of the tile, every row tile of i1 (wave w takes w, w + 4, ...).
-// The 16 lines are one bin row of the grid: each of their cells is summed
-// from the spread tiles that cover it (merge_cell, the tile merge's sum in
-// its order), so these transforms skip kb_spread_merge and the grid's write
-// and re-read.
+// columns n2 of the tile, every row tile of i1.  When the merge needs the
+// 3 x 3 neighbours only, the 16 lines (one bin row of the grid) are summed
+// here from the spread tiles that cover each cell (merge_cell, the tile
+// merge's sum in its order), so these transforms skip kb_spread_merge and the
+// grid's write and re-read.
 template <int R1, int R2>
 __device__ __forceinline__ void mm_stage_merged(const BatchDesc& d, int ct, float2* panel, int L1, int L2)
 {
@@ -419,82 +392,24 @@ __global__ __launch_bounds__(256) void kb_t1_cols_mm(const BatchDesc* __restrict
     float2* wt = lds + 16 * kT1MmLd;                         // L1 twiddles
     if (t1_cols_merge_tiles(d.N1, d.r1, d.r2)) {
         mm_stage_merged<1, 1>(d, o.y, panel, L1, L2);
+        for (int k = threadIdx.x; k < L1; k += blockDim.x) wt[k] = d.tw1[k];
     } else {   // grids with a thin last bin (5 x 5 neighbours): the merged grid from kb_spread_merge
-        for (int e = threadIdx.x; e < 16 * L1; e += blockDim.x) {
-            const int j = e / L1, n1 = e - j * L1, n2 = 16 * o.y + j;
-            panel[j * kT1MmLd + n1] = n2 < L2 ? d.Gt[(size_t)n2 * L1 + n1] : make_float2(0.f, 0.f);
-        }
+        mm_stage_rows(d.Gt, 16 * o.y, L2, L1, d.tw1, panel, wt);
     }
-    for (int k = threadIdx.x; k < L1; k += blockDim.x) wt[k] = d.tw1[k];
     __syncthreads();
-    const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-    for (int rt = threadIdx.x >> 6; rt < (N1 + 15) / 16; rt += blockDim.x >> 6) {
-        const int i1 = 16 * rt + i;
-        const int k1 = i1 - N1 / 2, step = k1 < 0 ? k1 + L1 : k1;
-        const int st4 = (4 * step) % L1;
-        int e = (step * kk) % L1;                            // twiddle index (k1 n1) mod L1, n1 = k0 + kk
-        const bool in = i1 < N1;
-        mm4v rr = {}, ii = {}, ri = {}, ir = {};
-        mm_tile(
-            L1,
-            [&](int) {
-                const float2 w = in ? twid<-1>(wt, e) : make_float2(0.f, 0.f);
-                e += st4;
-                e -= e >= L1 ? L1 : 0;
-                return w;
-            },
-            [&](int k0) { return panel[i * kT1MmLd + k0 + kk]; }, rr, ii, ri, ir);
-        const int n2 = 16 * o.y + i;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int io = 16 * rt + 4 * kk + r;
-            if (io < N1 && n2 < L2) d.Tm[(size_t)io * L2 + n2] = make_float2(rr[r] - ii[r], ri[r] + ir[r]);
-        }
-    }
+    t1_cols_mm_compute<-1>(o.y, panel, wt, d.Tm, N1, L2);
 }
 
 // Row pass, row tile o.y of transform o.x: F1[i1][i2] for the 16 rows i1 of
-// the tile, every column tile of i2 (wave w takes w, w + 4, ...), deconvolved.
+// the tile, every column tile of i2, deconvolved.
 __global__ __launch_bounds__(256) void kb_t1_rows_mm(const BatchDesc* __restrict__ bd, const int2* __restrict__ own)
 {
     __shared__ float2 lds[kT1MmLds];
     const int2 o = own[blockIdx.x];
     const BatchDesc& d = bd[o.x];
-    const int N1 = d.N1, N2 = d.N2, L2 = 2 * N2;
-    float2* panel = lds;                                     // [16][kT1MmLd]: rows i1 of Tm
-    float2* wt = lds + 16 * kT1MmLd;                         // L2 twiddles
-    for (int e = threadIdx.x; e < 16 * L2; e += blockDim.x) {
-        const int j = e / L2, n2 = e - j * L2, i1 = 16 * o.y + j;
-        panel[j * kT1MmLd + n2] = i1 < N1 ? d.Tm[(size_t)i1 * L2 + n2] : make_float2(0.f, 0.f);
-    }
-    for (int k = threadIdx.x; k < L2; k += blockDim.x) wt[k] = d.tw2[k];
+    mm_stage_rows(d.Tm, 16 * o.y, d.N1, 2 * d.N2, d.tw2, lds, lds + 16 * kT1MmLd);
     __syncthreads();
-    const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-    for (int ct = threadIdx.x >> 6; ct < (N2 + 15) / 16; ct += blockDim.x >> 6) {
-        const int i2 = 16 * ct + i;
-        const int k2 = i2 - N2 / 2, step = k2 < 0 ? k2 + L2 : k2;
-        const int st4 = (4 * step) % L2;
-        int e = (step * kk) % L2;                            // twiddle index (k2 n2) mod L2, n2 = k0 + kk
-        const bool in = i2 < N2;
-        mm4v rr = {}, ii = {}, ri = {}, ir = {};
-        mm_tile(
-            L2, [&](int k0) { return panel[i * kT1MmLd + k0 + kk]; },
-            [&](int) {
-                const float2 w = in ? twid<-1>(wt, e) : make_float2(0.f, 0.f);
-                e += st4;
-                e -= e >= L2 ? L2 : 0;
-                return w;
-            },
-            rr, ii, ri, ir);
-        const int i2o = 16 * ct + i;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i1 = 16 * o.y + 4 * kk + r;
-            if (i1 < N1 && i2o < N2)
-                d.F1[(size_t)i1 * N2 + i2o] =
-                    cscale(make_float2(rr[r] - ii[r], ri[r] + ir[r]), d.D1[i1] * d.D2[i2o]);
-        }
-    }
+    t1_rows_mm_compute<-1>(o.y, lds, lds + 16 * kT1MmLd, d.F1, d.D1, d.D2, d.N1, d.N2);
 }
 
 #ifndef NFFT2D_BATCH_GATHER_WAVES

@@ -469,6 +469,202 @@ __global__ __launch_bounds__(256) void k_t1_rows_direct(
     t1_rows_direct_body<T, SIGN>(blockIdx.x, lds, Tp, F, D1, D2, tw, N1, N2, lds + 2 * N2);
 }
 
+// ================================================================= type-1 direct passes on MFMA
+// The type-1 passes of short complex64 lines that are not powers of two (up
+// to kT1MmMaxN modes, below the chirp-z threshold: e.g. 88 in Synthesis/PSO)
+// as complex matrix products on the FP32 matrix cores (v_mfma_f32_16x16x4f32:
+// A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, accumulator element
+// r of lane l at row 4 (l / 16) + r, column l % 16).  Column pass: Tm (N1 x
+// L2) = W1 (N1 x L1) . G^T, W1[i1][n1] = w((k1 n1) mod L1), k1 = i1 - N1/2;
+// row pass: F (N1 x N2) = D1 D2 (Tm (N1 x L2) . W2^T), W2[i2][n2] = w((k2 n2)
+// mod L2).  A block takes 16 lines (staged in LDS, rows padded to an odd
+// length) and their 16 x 16 output tiles; the twiddle operand is read from
+// the LDS twiddle table at an index advanced by 4 k per K-step of 4, and four
+// real products (Re Re, Im Im, Re Im, Im Re) accumulate per K-step.  The VALU
+// direct passes read a line value and a twiddle from LDS for every 4 FMAs.
+constexpr int kT1MmMaxN = 126;
+typedef float mm4v __attribute__((ext_vector_type(4)));
+constexpr int kT1MmLd = 2 * kT1MmMaxN + 1;                 // padded line of the LDS panel (odd)
+constexpr int kT1MmLds = 16 * kT1MmLd + 2 * kT1MmMaxN;     // panel + twiddles, float2 entries
+
+// 16 x 16 complex tile, K-steps of 4 over K = L (a multiple of 4), this wave
+// taking k0 = kf, kf + ks, ...: lane (i, kk) = (l % 16, l / 16).  Left operand
+// a(k0) and right operand b(k0) of this lane.
+template <typename AF, typename BF>
+__device__ __forceinline__ void mm_tile(int L, int kf, int ks, AF&& a_of, BF&& b_of, mm4v& rr, mm4v& ii, mm4v& ri,
+                                        mm4v& ir)
+{
+    for (int k0 = kf; k0 < L; k0 += ks) {
+        const float2 a = a_of(k0), b = b_of(k0);
+        rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, rr, 0, 0, 0);
+        ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, ii, 0, 0, 0);
+        ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, ri, 0, 0, 0);
+        ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, ir, 0, 0, 0);
+    }
+}
+
+// One output tile over this wave's K-steps.  The twiddle line of lane i is
+// that of frequency index 16 t + i (k = that - N/2) over L = 2N samples; the
+// panel holds the 16 data lines.  Twiddles are the left operand of the column
+// pass (Tm = W1 . G^T) and the right operand of the row pass (F = Tm . W2^T).
+template <int SIGN, bool TW_LEFT>
+__device__ __forceinline__ void t1_mm_tile(int t, const float2* panel, const float2* wt, int N, int kf, int ks,
+                                           mm4v& rr, mm4v& ii, mm4v& ri, mm4v& ir)
+{
+    const int L = 2 * N;
+    const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+    const int f = 16 * t + i;
+    const int k = f - N / 2, step = k < 0 ? k + L : k;
+    const int adv = (step * ks) % L;
+    int e = (step * (kf + kk)) % L;                          // twiddle index (k n) mod L, n = k0 + kk
+    const bool in = f < N;
+    auto tw = [&](int) {
+        const float2 w = in ? twid<SIGN>(wt, e) : make_float2(0.f, 0.f);
+        e += adv;
+        e -= e >= L ? L : 0;
+        return w;
+    };
+    auto line = [&](int k0) { return panel[i * kT1MmLd + k0 + kk]; };
+    rr = ii = ri = ir = mm4v{0.f, 0.f, 0.f, 0.f};
+    if constexpr (TW_LEFT)
+        mm_tile(L, kf, ks, tw, line, rr, ii, ri, ir);
+    else
+        mm_tile(L, kf, ks, line, tw, rr, ii, ri, ir);
+}
+
+// Stores of a column-pass tile (row tile rt of i1, column tile ct of n2) and
+// of a row-pass tile (row tile rt of i1, column tile ct of i2, deconvolved).
+__device__ __forceinline__ void t1_cols_mm_store(int rt, int ct, const mm4v& rr, const mm4v& ii, const mm4v& ri,
+                                                 const mm4v& ir, float2* __restrict__ Tm, int N1, int L2)
+{
+    const int lane = threadIdx.x & 63, n2 = 16 * ct + (lane & 15), kk = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int io = 16 * rt + 4 * kk + r;
+        if (io < N1 && n2 < L2) Tm[(size_t)io * L2 + n2] = make_float2(rr[r] - ii[r], ri[r] + ir[r]);
+    }
+}
+__device__ __forceinline__ void t1_rows_mm_store(int rt, int ct, const mm4v& rr, const mm4v& ii, const mm4v& ri,
+                                                 const mm4v& ir, float2* __restrict__ F, const float* __restrict__ D1,
+                                                 const float* __restrict__ D2, int N1, int N2)
+{
+    const int lane = threadIdx.x & 63, i2 = 16 * ct + (lane & 15), kk = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i1 = 16 * rt + 4 * kk + r;
+        if (i1 < N1 && i2 < N2)
+            F[(size_t)i1 * N2 + i2] = cscale(make_float2(rr[r] - ii[r], ri[r] + ir[r]), D1[i1] * D2[i2]);
+    }
+}
+
+// Column pass of column tile ct, the 16 lines (rows n2 = 16 ct + j of the
+// grid) in panel[j * kT1MmLd + n1] and the L1 twiddles in wt: Tm[i1][n2] for
+// every row tile of i1, one tile per wave at a time (w, w + waves, ...).
+template <int SIGN>
+__device__ __forceinline__ void t1_cols_mm_compute(int ct, const float2* panel, const float2* wt,
+                                                   float2* __restrict__ Tm, int N1, int L2)
+{
+    for (int rt = threadIdx.x >> 6; rt < (N1 + 15) / 16; rt += blockDim.x >> 6) {
+        mm4v rr, ii, ri, ir;
+        t1_mm_tile<SIGN, true>(rt, panel, wt, N1, 0, 4, rr, ii, ri, ir);
+        t1_cols_mm_store(rt, ct, rr, ii, ri, ir, Tm, N1, L2);
+    }
+}
+
+// Row pass of row tile rt, the 16 lines (rows i1 = 16 rt + j of Tm) in
+// panel[j * kT1MmLd + n2] and the L2 twiddles in wt: F[i1][i2], deconvolved,
+// for every column tile of i2, one tile per wave at a time.
+template <int SIGN>
+__device__ __forceinline__ void t1_rows_mm_compute(int rt, const float2* panel, const float2* wt,
+                                                   float2* __restrict__ F, const float* __restrict__ D1,
+                                                   const float* __restrict__ D2, int N1, int N2)
+{
+    for (int ct = threadIdx.x >> 6; ct < (N2 + 15) / 16; ct += blockDim.x >> 6) {
+        mm4v rr, ii, ri, ir;
+        t1_mm_tile<SIGN, false>(ct, panel, wt, N2, 0, 4, rr, ii, ri, ir);
+        t1_rows_mm_store(rt, ct, rr, ii, ri, ir, F, D1, D2, N1, N2);
+    }
+}
+
+// 16 rows of a row-major matrix (row stride len <= 2 kT1MmMaxN, nrows rows)
+// from row r0 on, zero past the last, into the panel; and the len twiddles
+// into wt.  Blocks of 256 threads; every load is issued before the first LDS
+// store (a loop of dependent load-store rounds took 8.3 us a pass at 88 x 88
+// against 6.5 us for the VALU direct pass).
+constexpr int kT1MmStage = (16 * 2 * kT1MmMaxN + 255) / 256;
+__device__ __forceinline__ void mm_stage_rows(const float2* __restrict__ src, int r0, int nrows, int len,
+                                              const float2* __restrict__ tw, float2* panel, float2* wt)
+{
+    float2 v[kT1MmStage];
+#pragma unroll
+    for (int q = 0; q < kT1MmStage; ++q) {
+        const int e = threadIdx.x + 256 * q, j = e / len, c = e - j * len, r = r0 + j;
+        v[q] = e < 16 * len && r < nrows ? src[(size_t)r * len + c] : make_float2(0.f, 0.f);
+    }
+    const float2 w = threadIdx.x < len ? tw[threadIdx.x] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < kT1MmStage; ++q) {
+        const int e = threadIdx.x + 256 * q, j = e / len, c = e - j * len;
+        if (e < 16 * len) panel[j * kT1MmLd + c] = v[q];
+    }
+    if (threadIdx.x < len) wt[threadIdx.x] = w;
+}
+
+// The 4 waves' partial tiles summed in LDS (wave 0 adds waves 1, 2, 3 in turn).
+__device__ __forceinline__ bool mm_sum_waves(mm4v* part, mm4v& rr, mm4v& ii, mm4v& ri, mm4v& ir)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (w > 0) {
+        mm4v* q = part + (w - 1) * 256 + lane;
+        q[0] = rr;
+        q[64] = ii;
+        q[128] = ri;
+        q[192] = ir;
+    }
+    __syncthreads();
+    if (w > 0) return false;
+    for (int v = 0; v < 3; ++v) {
+        const mm4v* q = part + v * 256 + lane;
+        rr += q[0];
+        ii += q[64];
+        ri += q[128];
+        ir += q[192];
+    }
+    return true;
+}
+
+// A plan's passes: one block per 16 x 16 output tile (a single transform has
+// too few tiles to fill the chip one tile per wave), its 4 waves taking the
+// K-steps w, w + 4, ... and their partial tiles summed in LDS.  Column pass:
+// blockIdx.x the column tile of n2, blockIdx.y the row tile of i1; row pass:
+// blockIdx.x the row tile of i1, blockIdx.y the column tile of i2.
+template <int SIGN>
+__global__ __launch_bounds__(256) void k_t1_cols_mm(const float2* __restrict__ Gt, float2* __restrict__ Tm,
+                                                    const float2* __restrict__ tw1, int N1, int L2)
+{
+    __shared__ float2 lds[kT1MmLds];
+    __shared__ mm4v part[3 * 256];
+    mm_stage_rows(Gt, 16 * blockIdx.x, L2, 2 * N1, tw1, lds, lds + 16 * kT1MmLd);
+    __syncthreads();
+    mm4v rr, ii, ri, ir;
+    t1_mm_tile<SIGN, true>(blockIdx.y, lds, lds + 16 * kT1MmLd, N1, 4 * (threadIdx.x >> 6), 16, rr, ii, ri, ir);
+    if (mm_sum_waves(part, rr, ii, ri, ir)) t1_cols_mm_store(blockIdx.y, blockIdx.x, rr, ii, ri, ir, Tm, N1, L2);
+}
+template <int SIGN>
+__global__ __launch_bounds__(256) void k_t1_rows_mm(const float2* __restrict__ Tm, float2* __restrict__ F,
+                                                    const float* __restrict__ D1, const float* __restrict__ D2,
+                                                    const float2* __restrict__ tw2, int N1, int N2)
+{
+    __shared__ float2 lds[kT1MmLds];
+    __shared__ mm4v part[3 * 256];
+    mm_stage_rows(Tm, 16 * blockIdx.x, N1, 2 * N2, tw2, lds, lds + 16 * kT1MmLd);
+    __syncthreads();
+    mm4v rr, ii, ri, ir;
+    t1_mm_tile<SIGN, false>(blockIdx.y, lds, lds + 16 * kT1MmLd, N2, 4 * (threadIdx.x >> 6), 16, rr, ii, ri, ir);
+    if (mm_sum_waves(part, rr, ii, ri, ir))
+        t1_rows_mm_store(blockIdx.x, blockIdx.y, rr, ii, ri, ir, F, D1, D2, N1, N2);
+}
+
 // ================================================================= setpts (bin sort)
 // Points are counting-sorted by bin: a bin is a kBin x kBin block of cells of
 // the oversampled grid, holding mu = rint(c x) mod L.  Bins are the tiles of the
