@@ -316,6 +316,28 @@ def test_spread_thin_last_bin(N1, N2, dtype):
     assert e1 < tol and ea < tol
 
 
+@pytest.mark.parametrize("N1,N2", [(126, 126), (124, 22), (22, 124), (120, 96)])
+def test_type1_matrix_core_line_lengths(N1, N2):
+    """complex64 type-1 passes of even lengths up to 126 that are not powers of two run as
+    complex matrix products on the FP32 matrix cores (16 x 16 output tiles, the K sum split
+    over 4 waves): the largest such length, lopsided shapes with one short side, and 120 x 96.
+    Type-1 and adjoint, every mode, against FP64 direct sums."""
+    rng = np.random.default_rng(N1 * 1000 + N2)
+    M = 3000
+    x1 = ((rng.random(M) - 0.5) * N1).astype(np.float32)
+    x2 = ((rng.random(M) - 0.5) * N2).astype(np.float32)
+    d = cn01(rng, M)
+    print(f"case matrix-core type-1 {N1}x{N2}", flush=True)
+    plan = m.Plan(N1, N2, torch.complex64, device=DEV).setpts(T(x1), T(x2))
+    F = plan.type1(T(d)).cpu().numpy()
+    Fa = plan.adjoint(T(d)).cpu().numpy()
+    ref = oracle.ndft_type1(d.astype(np.complex128), x1, x2, N1, N2)
+    ref_a = np.conj(oracle.ndft_type1(np.conj(d.astype(np.complex128)), x1, x2, N1, N2))
+    e1, ea = oracle.rel_l2(F, ref), oracle.rel_l2(Fa, ref_a)
+    print(f"matrix-core type-1 {N1}x{N2}: type-1 {e1:.2e}, adjoint {ea:.2e}")
+    assert e1 < TOL32 and ea < TOL32
+
+
 @pytest.mark.parametrize("N1,N2,M", [(2048, 1024, 300000), (2048, 2048, 200000)])
 def test_large_grids_sort_paths(N1, N2, M):
     """Oversampled grids above 16,384 bins: the point histogram goes through
